@@ -1,0 +1,139 @@
+/*
+ * segsort.h - C-ABI of the MI355X (gfx950) segmented sort library libsegsort.so.
+ *
+ * Every entry point takes plain device pointers, sizes and an opaque stream
+ * handle (a hipStream_t passed as void*, NULL = the null stream), and returns
+ * an int holding a hipError_t value (0 = hipSuccess). Nothing throws or exits
+ * across this boundary.
+ *
+ * The sort entry points replace cub::DeviceSegmentedRadixSort as the
+ * reference's drivers call it:
+ *   SortPairs  lib/cub/device/device_segmented_radix_sort.cuh:136-177
+ *   SortKeys   lib/cub/device/device_segmented_radix_sort.cuh:532-570
+ *   SortPairsDescending / SortKeysDescending  :302-350, :685-726
+ *   callers    src/radixseg.cu:116-124, src/fixcub.cu:141-165,
+ *              src/mergeseg.cu:118, src/bb_segsort/main.cu:102
+ * with cub's temp-storage protocol (util_device.cuh:58-101,
+ * dispatch_radix_sort.cuh:1349-1356): d_temp == NULL writes the required size
+ * to *temp_bytes and returns success; a too-small *temp_bytes returns
+ * hipErrorInvalidValue. Inputs are never modified; output positions outside
+ * every segment are never written; a segment with end <= begin is empty.
+ *
+ * The caller owns all device memory, including temp; no entry point except
+ * the in-place adapters (segsort_bb_segsort, segsort_mgpu_segmented_sort)
+ * allocates. Calls are stream-ordered and asynchronous, except the PLAIN
+ * algorithm, which reads its class counts back to the host (as the
+ * reference's bb_segsort does, src/bb_segsort/bb_bin.h:46).
+ */
+#ifndef SEGSORT_H
+#define SEGSORT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Key types (the cub key types of the reference's drivers are u32). */
+enum segsort_key_type {
+    SEGSORT_KEY_U32 = 0,
+    SEGSORT_KEY_I32 = 1,
+    SEGSORT_KEY_F32 = 2,
+    SEGSORT_KEY_U64 = 3,
+    SEGSORT_KEY_I64 = 4,
+    SEGSORT_KEY_F64 = 5
+};
+
+/* Algorithms. BATCHED: device binning into tiers, no host sync, capturable.
+ * PLAIN: one launch per segment-length class after a host read-back of the
+ * class counts, one workgroup per segment beyond the largest LDS tile.
+ * FIXPASS: segment id packed above the key in a 64-bit composite and one
+ * global LSD radix sort (src/fixcub.cu:140-166, widened); 32-bit keys only,
+ * segments must tile [d_begin[0], d_end[num_segments-1]) in order. */
+enum segsort_algo {
+    SEGSORT_ALGO_BATCHED = 0,
+    SEGSORT_ALGO_PLAIN = 1,
+    SEGSORT_ALGO_FIXPASS = 2
+};
+
+/* Stable segmented sort of (key, value) pairs or of keys alone.
+ * value_bytes: 0 (keys only; vals_in/vals_out ignored), 4 or 8.
+ * begin_bit/end_bit: key bits [begin_bit, end_bit) of the radix order, as in
+ * cub (0 and 8*sizeof(key) for the whole key). descending: 0 or 1.
+ * Replaces DeviceSegmentedRadixSort::SortPairs/SortKeys(/Descending),
+ * device_segmented_radix_sort.cuh:136-177, 302-350, 532-570, 685-726. */
+int segsort_sort(void* d_temp, size_t* temp_bytes,
+                 const void* d_keys_in, void* d_keys_out,
+                 const void* d_vals_in, void* d_vals_out,
+                 int num_items, int num_segments,
+                 const int* d_begin_offsets, const int* d_end_offsets,
+                 int begin_bit, int end_bit,
+                 int key_type, int value_bytes, int descending, int algo,
+                 void* stream);
+
+/* Typed shorthands of segsort_sort for the reference's u32/u32 drivers
+ * (src/radixseg.cu:116-124 calls SortPairs<uint, uint>). */
+int segsort_sort_pairs_u32(void* d_temp, size_t* temp_bytes,
+                           const uint32_t* d_keys_in, uint32_t* d_keys_out,
+                           const uint32_t* d_vals_in, uint32_t* d_vals_out,
+                           int num_items, int num_segments,
+                           const int* d_begin_offsets, const int* d_end_offsets,
+                           int algo, void* stream);
+int segsort_sort_keys_u32(void* d_temp, size_t* temp_bytes,
+                          const uint32_t* d_keys_in, uint32_t* d_keys_out,
+                          int num_items, int num_segments,
+                          const int* d_begin_offsets, const int* d_end_offsets,
+                          int algo, void* stream);
+
+/* In-place adapter for the bb_segsort driver: int bb_segsort(K*, T*, int n,
+ * int* d_segs, int length) (src/bb_segsort/bb_segsort.h:36). d_segs holds
+ * `num_segments` begin offsets; the last segment ends at num_items. Sorts
+ * keys and values in place, allocating and freeing its own temp (as the
+ * reference does, bb_segsort.h:39-56). Stable, unlike the reference.
+ * Returns 1 on success and 0 on failure, as the reference does. */
+int segsort_bb_segsort(void* d_keys, void* d_vals, int num_items, const int* d_segs, int num_segments,
+                       int key_type, int value_bytes, void* stream);
+
+/* In-place adapter for mgpu::segmented_sort(keys, vals, count, segments,
+ * num_segments, less, context) (lib/moderngpu/kernel_segsort.hxx:405-416,
+ * called from src/mergeseg.cu:118): begin offsets only, the last segment
+ * ends at count; d_vals may be NULL (keys only, kernel_segsort.hxx:432-439).
+ * Allocates and frees its own temp. Returns a hipError_t value. */
+int segsort_mgpu_segmented_sort(void* d_keys, void* d_vals, int count, const int* d_segments, int num_segments,
+                                int key_type, int value_bytes, void* stream);
+
+/* Differentiable pair sort, forward helper: d_perm[i] = i - d_begin[r] for
+ * every i of row r (the row-local index vector that a stable pair sort then
+ * carries as values). One workgroup per row. */
+int segsort_row_iota(const int* d_begin_offsets, const int* d_end_offsets, int rows, int* d_perm, void* stream);
+
+/* Differentiable pair sort, backward: d_grad_in[d_begin[r] + d_perm[i]] =
+ * d_grad_out[i] for every i of row r; positions outside every row are left
+ * untouched. d_perm holds the row-local stable permutation of the forward. */
+int segsort_perm_scatter_f32(const float* d_grad_out, const int* d_perm,
+                             const int* d_begin_offsets, const int* d_end_offsets, int rows,
+                             float* d_grad_in, void* stream);
+
+/* Tuning of the batched and plain forms (read by the autotuner).
+ * wave_cap: ranges of 2..wave_cap items use the wave tier (2 <= wave_cap <= 64).
+ * tile_enable: bit c enables LDS tile class c (classes 0..3, capacities given
+ * by segsort_tile_caps); a disabled class's ranges go to the next enabled one.
+ * grid_mult: persistent workgroups per CU per resident slot (1..8).
+ * key_width: 4 or 8; value_bytes: 0, 4 or 8. Returns a hipError_t value. */
+int segsort_set_tuning(int key_width, int value_bytes, int wave_cap, int tile_enable, int grid_mult);
+int segsort_get_tuning(int key_width, int value_bytes, int* wave_cap, int* tile_enable, int* grid_mult);
+/* Capacities (items) of the LDS tile classes for a key width / value size;
+ * writes up to 4 values and returns how many. */
+int segsort_tile_caps(int key_width, int value_bytes, int* caps);
+
+/* Human-readable text of the last error this thread's calls returned. */
+const char* segsort_last_error(void);
+/* Library version (major*10000 + minor*100 + patch). */
+int segsort_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SEGSORT_H */

@@ -1,0 +1,300 @@
+"""MI355X segmented sort: Python host side over the libsegsort.so C-ABI.
+
+The product is the HIP library ``lib/libsegsort.so`` (sources in ``csrc/``), whose
+C-ABI is declared in ``include/segsort.h``. This module only binds it:
+
+* ``DeviceSegmentedRadixSort`` mirrors ``cub::DeviceSegmentedRadixSort``
+  (lib/cub/device/device_segmented_radix_sort.cuh:136-855): ``SortPairs``,
+  ``SortKeys``, ``SortPairsDescending``, ``SortKeysDescending`` with cub's
+  argument meaning and temp-storage protocol, returning a hipError_t code.
+* ``segmented_sort`` is the tensor-level convenience call.
+* ``SegmentedPairSort`` is the differentiable pair sort (torch.autograd.Function)
+  whose backward is the library's gradient-scatter kernel.
+
+There is no CPU fallback: every entry point raises if the library is not built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+from . import gen  # noqa: F401  (seeded workload generators)
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "lib" / "libsegsort.so"
+
+KEY_U32, KEY_I32, KEY_F32, KEY_U64, KEY_I64, KEY_F64 = range(6)
+ALGO_BATCHED, ALGO_PLAIN, ALGO_FIXPASS = 0, 1, 2
+ALGOS = {"batched": ALGO_BATCHED, "plain": ALGO_PLAIN, "fixpass": ALGO_FIXPASS}
+
+# Every symbol include/segsort.h declares.
+EXPORTS = (
+    "segsort_sort", "segsort_sort_pairs_u32", "segsort_sort_keys_u32", "segsort_bb_segsort",
+    "segsort_mgpu_segmented_sort", "segsort_row_iota", "segsort_perm_scatter_f32", "segsort_set_tuning",
+    "segsort_get_tuning", "segsort_tile_caps", "segsort_last_error", "segsort_version",
+)
+
+_lib = None
+
+
+class SegsortError(RuntimeError):
+    pass
+
+
+def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
+    """Load libsegsort.so (raises if it is not built: there is no fallback path)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise SegsortError(f"HIP library not built: {p} is missing (run __graft_entry__.build())")
+    lib = ctypes.CDLL(str(p))
+    vp, sz, i32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int
+    lib.segsort_sort.argtypes = [vp, ctypes.POINTER(sz), vp, vp, vp, vp, i32, i32, vp, vp, i32, i32, i32, i32, i32,
+                                 i32, vp]
+    lib.segsort_sort.restype = i32
+    lib.segsort_sort_pairs_u32.argtypes = [vp, ctypes.POINTER(sz), vp, vp, vp, vp, i32, i32, vp, vp, i32, vp]
+    lib.segsort_sort_pairs_u32.restype = i32
+    lib.segsort_sort_keys_u32.argtypes = [vp, ctypes.POINTER(sz), vp, vp, i32, i32, vp, vp, i32, vp]
+    lib.segsort_sort_keys_u32.restype = i32
+    lib.segsort_bb_segsort.argtypes = [vp, vp, i32, vp, i32, i32, i32, vp]
+    lib.segsort_bb_segsort.restype = i32
+    lib.segsort_mgpu_segmented_sort.argtypes = [vp, vp, i32, vp, i32, i32, i32, vp]
+    lib.segsort_mgpu_segmented_sort.restype = i32
+    lib.segsort_row_iota.argtypes = [vp, vp, i32, vp, vp]
+    lib.segsort_row_iota.restype = i32
+    lib.segsort_perm_scatter_f32.argtypes = [vp, vp, vp, vp, i32, vp, vp]
+    lib.segsort_perm_scatter_f32.restype = i32
+    lib.segsort_set_tuning.argtypes = [i32, i32, i32, i32, i32]
+    lib.segsort_set_tuning.restype = i32
+    lib.segsort_get_tuning.argtypes = [i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    lib.segsort_get_tuning.restype = i32
+    lib.segsort_tile_caps.argtypes = [i32, i32, ctypes.POINTER(i32)]
+    lib.segsort_tile_caps.restype = i32
+    lib.segsort_last_error.argtypes = []
+    lib.segsort_last_error.restype = ctypes.c_char_p
+    lib.segsort_version.argtypes = []
+    lib.segsort_version.restype = i32
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    return load_library().segsort_last_error().decode()
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise SegsortError(f"{what} failed with hipError {rc}: {last_error()}")
+
+
+def _ptr(x) -> int | None:
+    """Device address of a torch tensor (or an int address / None)."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+def _stream(stream) -> int | None:
+    if stream is None:
+        import torch
+
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+_KEY_OF_DTYPE = None
+
+
+def key_type_of(dtype) -> int:
+    global _KEY_OF_DTYPE
+    import torch
+
+    if _KEY_OF_DTYPE is None:
+        _KEY_OF_DTYPE = {
+            torch.uint32: KEY_U32, torch.int32: KEY_I32, torch.float32: KEY_F32,
+            torch.uint64: KEY_U64, torch.int64: KEY_I64, torch.float64: KEY_F64,
+        }
+    if dtype not in _KEY_OF_DTYPE:
+        raise SegsortError(f"unsupported key dtype {dtype}")
+    return _KEY_OF_DTYPE[dtype]
+
+
+class DeviceSegmentedRadixSort:
+    """cub::DeviceSegmentedRadixSort over libsegsort.so.
+
+    Arguments keep cub's meaning (device_segmented_radix_sort.cuh:136-177): pass
+    ``d_temp_storage=None`` to get the required size back in ``temp_storage_bytes``
+    (a one-element list, cub's size_t& out-parameter). Buffers are torch tensors or
+    raw device addresses; ``key_type``/``value_bytes`` must be given for raw
+    addresses. Returns the hipError_t code (0 = success), as cub returns cudaError_t.
+    """
+
+    @staticmethod
+    def _call(d_temp_storage, temp_storage_bytes, d_keys_in, d_keys_out, d_values_in, d_values_out, num_items,
+              num_segments, d_begin_offsets, d_end_offsets, begin_bit, end_bit, stream, descending, key_type,
+              value_bytes, algo):
+        lib = load_library()
+        if key_type is None:
+            key_type = key_type_of(d_keys_in.dtype)
+        if value_bytes is None:
+            value_bytes = 0 if d_values_in is None else d_values_in.element_size()
+        kbits = 64 if key_type >= KEY_U64 else 32
+        if end_bit is None:
+            end_bit = kbits
+        tb = ctypes.c_size_t(int(temp_storage_bytes[0]) if temp_storage_bytes else 0)
+        rc = lib.segsort_sort(_ptr(d_temp_storage), ctypes.byref(tb), _ptr(d_keys_in), _ptr(d_keys_out),
+                              _ptr(d_values_in), _ptr(d_values_out), int(num_items), int(num_segments),
+                              _ptr(d_begin_offsets), _ptr(d_end_offsets), int(begin_bit), int(end_bit), int(key_type),
+                              int(value_bytes), int(descending), int(ALGOS.get(algo, algo)),
+                              _stream(stream) if d_temp_storage is not None else None)
+        if temp_storage_bytes is not None and d_temp_storage is None:
+            temp_storage_bytes[0] = tb.value
+        return rc
+
+    @staticmethod
+    def SortPairs(d_temp_storage, temp_storage_bytes, d_keys_in, d_keys_out, d_values_in, d_values_out, num_items,
+                  num_segments, d_begin_offsets, d_end_offsets, begin_bit=0, end_bit=None, stream=None,
+                  key_type=None, value_bytes=None, algo="batched"):
+        return DeviceSegmentedRadixSort._call(d_temp_storage, temp_storage_bytes, d_keys_in, d_keys_out, d_values_in,
+                                              d_values_out, num_items, num_segments, d_begin_offsets, d_end_offsets,
+                                              begin_bit, end_bit, stream, 0, key_type, value_bytes, algo)
+
+    @staticmethod
+    def SortPairsDescending(d_temp_storage, temp_storage_bytes, d_keys_in, d_keys_out, d_values_in, d_values_out,
+                            num_items, num_segments, d_begin_offsets, d_end_offsets, begin_bit=0, end_bit=None,
+                            stream=None, key_type=None, value_bytes=None, algo="batched"):
+        return DeviceSegmentedRadixSort._call(d_temp_storage, temp_storage_bytes, d_keys_in, d_keys_out, d_values_in,
+                                              d_values_out, num_items, num_segments, d_begin_offsets, d_end_offsets,
+                                              begin_bit, end_bit, stream, 1, key_type, value_bytes, algo)
+
+    @staticmethod
+    def SortKeys(d_temp_storage, temp_storage_bytes, d_keys_in, d_keys_out, num_items, num_segments,
+                 d_begin_offsets, d_end_offsets, begin_bit=0, end_bit=None, stream=None, key_type=None,
+                 algo="batched"):
+        return DeviceSegmentedRadixSort._call(d_temp_storage, temp_storage_bytes, d_keys_in, d_keys_out, None, None,
+                                              num_items, num_segments, d_begin_offsets, d_end_offsets, begin_bit,
+                                              end_bit, stream, 0, key_type, 0, algo)
+
+    @staticmethod
+    def SortKeysDescending(d_temp_storage, temp_storage_bytes, d_keys_in, d_keys_out, num_items, num_segments,
+                           d_begin_offsets, d_end_offsets, begin_bit=0, end_bit=None, stream=None, key_type=None,
+                           algo="batched"):
+        return DeviceSegmentedRadixSort._call(d_temp_storage, temp_storage_bytes, d_keys_in, d_keys_out, None, None,
+                                              num_items, num_segments, d_begin_offsets, d_end_offsets, begin_bit,
+                                              end_bit, stream, 1, key_type, 0, algo)
+
+
+def temp_bytes(num_items, num_segments, key_dtype, value_bytes=0, algo="batched", begin_bit=0, end_bit=None):
+    """Temp storage the call would need (host-only query, no GPU work)."""
+    tb = [0]
+    kt = key_type_of(key_dtype) if not isinstance(key_dtype, int) else key_dtype
+    rc = DeviceSegmentedRadixSort._call(None, tb, None, None, None, None, num_items, num_segments, None, None,
+                                        begin_bit, end_bit, None, 0, kt, value_bytes, algo)
+    _check(rc, "temp size query")
+    return tb[0]
+
+
+def segmented_sort(keys, offsets=None, values=None, *, begin=None, end=None, descending=False, begin_bit=0,
+                   end_bit=None, algo="batched", keys_out=None, values_out=None, temp=None, stream=None):
+    """Stable per-segment sort of cuda tensors; returns (keys_out, values_out).
+
+    Segments are [offsets[i], offsets[i+1]) or [begin[i], end[i]); positions outside
+    every segment are copied from the input (the kernels never write them).
+    """
+    import torch
+
+    if begin is None:
+        begin, end = offsets[:-1], offsets[1:]
+    n = keys.numel()
+    s = begin.numel()
+    if keys_out is None:
+        keys_out = keys.clone()
+    if values is not None and values_out is None:
+        values_out = values.clone()
+    kt = key_type_of(keys.dtype)
+    vb = 0 if values is None else values.element_size()
+    need = temp_bytes(n, s, kt, vb, algo, begin_bit, end_bit)
+    if temp is None or temp.numel() < need:
+        temp = torch.empty(need, dtype=torch.uint8, device=keys.device)
+    tb = [temp.numel()]
+    rc = DeviceSegmentedRadixSort._call(temp, tb, keys, keys_out, values, values_out, n, s, begin, end, begin_bit,
+                                        end_bit, stream, 1 if descending else 0, kt, vb, algo)
+    _check(rc, "segsort_sort")
+    return keys_out, values_out
+
+
+def set_tuning(key_width, value_bytes, wave_cap, tile_enable, grid_mult):
+    _check(load_library().segsort_set_tuning(key_width, value_bytes, wave_cap, tile_enable, grid_mult), "set_tuning")
+
+
+def get_tuning(key_width, value_bytes):
+    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _check(load_library().segsort_get_tuning(key_width, value_bytes, ctypes.byref(a), ctypes.byref(b),
+                                             ctypes.byref(c)), "get_tuning")
+    return {"wave_cap": a.value, "tile_enable": b.value, "grid_mult": c.value}
+
+
+def tile_caps(key_width, value_bytes):
+    arr = (ctypes.c_int * 4)()
+    n = load_library().segsort_tile_caps(key_width, value_bytes, arr)
+    return [arr[i] for i in range(n)]
+
+
+def _autograd():
+    import torch
+
+    class SegmentedPairSort(torch.autograd.Function):
+        """Stable ascending per-row sort of float32 values: (sorted, perm).
+
+        perm is row-local (int32), as torch.sort's indices along a row. The backward
+        scatters the upstream gradient through perm with the library's gradient
+        kernel: grad_in[begin[r] + perm[i]] = grad_sorted[i].
+        """
+
+        @staticmethod
+        def forward(ctx, x, offsets, algo="batched"):
+            if x.dtype != torch.float32 or offsets.dtype != torch.int32:
+                raise SegsortError("SegmentedPairSort expects float32 values and int32 offsets")
+            x = x.contiguous()
+            begin, end = offsets[:-1], offsets[1:]
+            rows = begin.numel()
+            lib = load_library()
+            st = _stream(None)
+            iota = torch.empty(x.numel(), dtype=torch.int32, device=x.device)
+            _check(lib.segsort_row_iota(begin.data_ptr(), end.data_ptr(), rows, iota.data_ptr(), st), "row_iota")
+            y, perm = segmented_sort(x, begin=begin, end=end, values=iota, algo=algo)
+            ctx.save_for_backward(perm, offsets)
+            ctx.mark_non_differentiable(perm)
+            return y, perm
+
+        @staticmethod
+        def backward(ctx, grad_y, grad_perm):
+            perm, offsets = ctx.saved_tensors
+            grad_y = grad_y.contiguous().float()
+            gin = torch.zeros_like(grad_y)
+            begin, end = offsets[:-1], offsets[1:]
+            _check(load_library().segsort_perm_scatter_f32(grad_y.data_ptr(), perm.data_ptr(), begin.data_ptr(),
+                                                           end.data_ptr(), begin.numel(), gin.data_ptr(),
+                                                           _stream(None)), "perm_scatter")
+            return gin, None, None
+
+    return SegmentedPairSort
+
+
+_SPS = None
+
+
+def segmented_pair_sort(x, offsets, algo="batched"):
+    """Differentiable stable per-row sort: returns (sorted values, row-local perm)."""
+    global _SPS
+    if _SPS is None:
+        _SPS = _autograd()
+    return _SPS.apply(x, offsets, algo)

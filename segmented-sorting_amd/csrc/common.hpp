@@ -1,0 +1,163 @@
+// Shared device primitives for the gfx950 segmented sort: key twiddling
+// (radix order), wave64 helpers and the buffer/work-item records that the
+// kernels exchange through device memory.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ssort {
+
+constexpr int WAVE = 64;
+constexpr int RADIX_BITS = 8;
+constexpr int RADIX = 1 << RADIX_BITS;
+
+// Key classes, as the radix path orders them (lib/cub/util_type.cuh:889-1036).
+enum : int { KIND_UNSIGNED = 0, KIND_SIGNED = 1, KIND_FLOAT = 2 };
+
+template <typename U>
+struct Bits {
+    static constexpr int N = int(sizeof(U) * 8);
+    static constexpr U HIGH = U(U(1) << (N - 1));
+    static constexpr U ALL = U(~U(0));
+};
+
+// Map a raw key to the unsigned pattern whose integer order is the sort order:
+// unsigned identity, signed ^ sign bit, float (neg ? ~b : b ^ sign), and every
+// bit flipped for a descending sort (util_type.cuh:904-911, 947-955, 1017-1027).
+template <typename U>
+__device__ __forceinline__ U twiddle_in(U b, int kind, bool desc) {
+    U r = b;
+    if (kind == KIND_SIGNED) r = U(b ^ Bits<U>::HIGH);
+    else if (kind == KIND_FLOAT) r = U(b ^ ((b & Bits<U>::HIGH) ? Bits<U>::ALL : Bits<U>::HIGH));
+    return desc ? U(~r) : r;
+}
+
+template <typename U>
+__device__ __forceinline__ U twiddle_out(U r, int kind, bool desc) {
+    if (desc) r = U(~r);
+    if (kind == KIND_SIGNED) return U(r ^ Bits<U>::HIGH);
+    if (kind == KIND_FLOAT) return U(r ^ ((r & Bits<U>::HIGH) ? Bits<U>::HIGH : Bits<U>::ALL));
+    return r;
+}
+
+// How the keys of one call are ordered: key class, direction and the
+// [begin_bit, end_bit) range of twiddled bits that takes part in the order.
+struct KeyMode {
+    int kind;
+    int desc;
+    int begin_bit;
+    int end_bit;
+};
+
+template <typename U>
+__device__ __forceinline__ U range_mask(int begin_bit, int end_bit) {
+    int nb = end_bit - begin_bit;
+    if (nb <= 0) return U(0);
+    U m = (nb >= Bits<U>::N) ? Bits<U>::ALL : U((U(1) << nb) - 1);
+    return U(m << begin_bit);
+}
+
+template <typename U>
+__device__ __forceinline__ int bit_length(U x) {
+    if (x == 0) return 0;
+    if constexpr (sizeof(U) == 8) return 64 - __clzll((long long)x);
+    else return 32 - __clz((int)x);
+}
+
+template <typename U>
+__device__ __forceinline__ int lowest_bit(U x) {
+    if constexpr (sizeof(U) == 8) return __ffsll((long long)x) - 1;
+    else return __ffs((int)x) - 1;
+}
+
+// ---- wave64 helpers --------------------------------------------------------
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Number of set bits of m at lanes strictly below the calling lane.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+    const uint32_t l = lane_id();
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, WAVE);
+        if (l >= (uint32_t)o) x += y;
+    }
+    return x;
+}
+
+template <typename U>
+__device__ __forceinline__ U wave_or(U x) {
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) x |= __shfl_xor(x, o, WAVE);
+    return x;
+}
+
+template <typename U>
+__device__ __forceinline__ U wave_and(U x) {
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) x &= __shfl_xor(x, o, WAVE);
+    return x;
+}
+
+template <typename U>
+__device__ __forceinline__ U read_lane(U v, int l) {
+    if constexpr (sizeof(U) == 8) {
+        uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+        uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+        return (U(hi) << 32) | U(lo);
+    } else {
+        return (U)(uint32_t)__builtin_amdgcn_readlane((int)v, l);
+    }
+}
+
+// ---- buffers and work items -----------------------------------------------
+
+// The four places a range of keys/values can live in during one call.
+enum : int { BUF_IN = 0, BUF_TMP_A = 1, BUF_TMP_B = 2, BUF_OUT = 3 };
+
+struct Bufs {
+    const void* k[4];
+    const void* v[4];
+};
+
+template <typename T>
+__device__ __forceinline__ T* buf_ptr(const void* p) {
+    return (T*)const_cast<void*>(p);
+}
+
+// A range that one tile (or one wave) sorts from buffer `src` into BUF_OUT.
+// n < 2^30; the two high bits of n_src hold the source buffer id.
+struct TileItem {
+    int begin;
+    int n_src;
+};
+
+__device__ __forceinline__ int ti_n(TileItem t) { return t.n_src & 0x3FFFFFFF; }
+__device__ __forceinline__ int ti_src(TileItem t) { return (int)((uint32_t)t.n_src >> 30); }
+__device__ __forceinline__ TileItem make_ti(int begin, int n, int src) {
+    TileItem t;
+    t.begin = begin;
+    t.n_src = n | (src << 30);
+    return t;
+}
+
+// A range larger than one tile, partitioned by its top varying radix digit.
+enum : int { LMODE_PARTITION = 0, LMODE_FINAL = 1, LMODE_COPY = 2 };
+
+struct LargeItem {
+    int begin;       // first element (same index in every buffer)
+    int n;           // element count
+    int src;         // buffer holding the range now
+    int hi_bit;      // bits at and above hi_bit are equal within the range
+    int chunk_base;  // first chunk id of this range in its level
+    int shift;       // chosen digit position      (written by the scan kernel)
+    int bits;        // chosen digit width         (written by the scan kernel)
+    int mode;        // LMODE_*                    (written by the scan kernel)
+};
+
+}  // namespace ssort

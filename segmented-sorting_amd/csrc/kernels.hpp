@@ -1,0 +1,567 @@
+// Kernels of the three forms (batched, plain per-class, fix-pass) and of the
+// differentiable pair sort. All scheduling state lives in device memory, so
+// the batched sequence needs no host synchronisation and can be captured.
+#pragma once
+
+#include "tile.hpp"
+
+namespace ssort {
+
+constexpr int MAX_TILE_CLASSES = 6;  // class 0 = wave tier, 1.. = LDS tiles
+constexpr int MAX_LEVELS = 8;        // MSD levels of the large tier (64-bit keys)
+
+// Device-resident counters of one call; cleared by k_init at the start.
+struct Counters {
+    uint32_t tile_count[MAX_TILE_CLASSES];
+    uint32_t pad0[2];
+    unsigned long long level[MAX_LEVELS + 1];  // (items << 32) | chunks, per MSD level
+};
+
+// Length caps of the classes of one call: class c holds ranges with
+// cap[c-1] < n <= cap[c] (cap[-1] = 0); n > cap[nclass-1] is large.
+struct ClassCaps {
+    int cap[MAX_TILE_CLASSES];
+    int nclass;
+};
+
+// Where the queues of one call live (all inside the caller's temp storage).
+struct Sched {
+    Counters* ctr;
+    TileItem* tq[MAX_TILE_CLASSES];
+    LargeItem* lq[MAX_LEVELS + 1];
+    uint32_t* hist;  // [chunk][npos_max][RADIX]
+    uint32_t* off;   // [chunk][RADIX]
+    int chunk;       // items per large-tier chunk
+    int npos_max;    // digit positions per chunk histogram
+    // Capacities: a push beyond them is dropped and readers clamp their counts,
+    // so overlapping (invalid) segments cannot write outside the temp storage.
+    uint32_t tq_cap[MAX_TILE_CLASSES];
+    uint32_t lq_cap;
+    uint32_t chunk_cap;
+};
+
+__device__ __forceinline__ int tile_count(const Sched& sc, int cls) {
+    const uint32_t c = sc.ctr->tile_count[cls];
+    return (int)(c < sc.tq_cap[cls] ? c : sc.tq_cap[cls]);
+}
+
+__device__ __forceinline__ void level_counts(const Sched& sc, int level, int& count, int& chunks) {
+    const unsigned long long packed = sc.ctr->level[level];
+    const uint32_t c = (uint32_t)(packed >> 32), g = (uint32_t)(packed & 0xFFFFFFFFull);
+    count = (int)(c < sc.lq_cap ? c : sc.lq_cap);
+    chunks = (int)(g < sc.chunk_cap ? g : sc.chunk_cap);
+}
+
+__device__ __forceinline__ int class_of(const ClassCaps& cc, int n) {
+    for (int c = 0; c < cc.nclass; ++c)
+        if (n <= cc.cap[c]) return c;
+    return -1;
+}
+
+__global__ void k_init(Counters* ctr) {
+    uint32_t* p = (uint32_t*)ctr;
+    for (int i = threadIdx.x; i < (int)(sizeof(Counters) / 4); i += blockDim.x) p[i] = 0;
+}
+
+// Append to a tile queue with one atomic per wave and class.
+__device__ __forceinline__ void push_tile(const Sched& sc, int cls, bool act, TileItem it) {
+    for (int c = 0; c < MAX_TILE_CLASSES; ++c) {
+        const uint64_t m = __ballot(act && cls == c);
+        if (m == 0) continue;
+        const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
+        uint32_t base = 0;
+        if (lane_id() == leader) base = atomicAdd(&sc.ctr->tile_count[c], (uint32_t)__popcll(m));
+        base = (uint32_t)__shfl((int)base, (int)leader, WAVE);
+        const uint32_t slot = base + lanes_below(m);
+        if (act && cls == c && slot < sc.tq_cap[c]) sc.tq[c][slot] = it;
+    }
+}
+
+__device__ __forceinline__ void push_large(const Sched& sc, int level, int begin, int n, int src, int hi_bit) {
+    const unsigned long long nch = (unsigned long long)((n + sc.chunk - 1) / sc.chunk);
+    const unsigned long long old = atomicAdd(&sc.ctr->level[level], (1ull << 32) | nch);
+    LargeItem li;
+    li.begin = begin;
+    li.n = n;
+    li.src = src;
+    li.hi_bit = hi_bit;
+    li.chunk_base = (int)(old & 0xFFFFFFFFull);
+    li.shift = 0;
+    li.bits = 0;
+    li.mode = 0;
+    if ((old >> 32) < sc.lq_cap && (old & 0xFFFFFFFFull) + nch <= sc.chunk_cap) sc.lq[level][old >> 32] = li;
+}
+
+// Bin every segment by length: length 1 is copied here, the rest go to the
+// queue of their class (batched and plain forms) or to MSD level 0 (large).
+template <typename U, typename V, bool HAS_V>
+__global__ void k_bin(const int* __restrict__ d_begin, const int* __restrict__ d_end, int num_segments,
+                      Bufs b, Sched sc, ClassCaps cc, int end_bit, int large_to_level0) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    int beg = 0, n = 0;
+    if (s < num_segments) {
+        beg = d_begin[s];
+        n = d_end[s] - beg;
+        if (n < 0) n = 0;
+    }
+    if (n == 1) {
+        buf_ptr<U>(b.k[BUF_OUT])[beg] = ((const U*)b.k[BUF_IN])[beg];
+        if constexpr (HAS_V) buf_ptr<V>(b.v[BUF_OUT])[beg] = ((const V*)b.v[BUF_IN])[beg];
+    }
+    const int cls = n >= 2 ? class_of(cc, n) : -2;
+    push_tile(sc, cls, cls >= 0, make_ti(beg, cls >= 0 ? n : 0, BUF_IN));
+    if (cls == -1) {
+        if (large_to_level0) push_large(sc, 0, beg, n, BUF_IN, end_bit);
+    }
+}
+
+// Plain form: class ids listed per segment for the one-workgroup-per-segment
+// global LSD (ranges larger than the biggest LDS tile).
+__global__ void k_bin_large_ids(const int* __restrict__ d_begin, const int* __restrict__ d_end, int num_segments,
+                                Sched sc, ClassCaps cc) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    int n = 0, beg = 0;
+    if (s < num_segments) {
+        beg = d_begin[s];
+        n = d_end[s] - beg;
+    }
+    const bool big = s < num_segments && n > cc.cap[cc.nclass - 1];
+    const uint64_t m = __ballot(big);
+    if (m == 0) return;
+    const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if (lane_id() == leader) base = (uint32_t)atomicAdd(&sc.ctr->level[MAX_LEVELS], (unsigned long long)__popcll(m));
+    base = (uint32_t)__shfl((int)base, (int)leader, WAVE);
+    if (big) {
+        LargeItem li;
+        li.begin = beg;
+        li.n = n;
+        li.src = BUF_IN;
+        li.hi_bit = 0;
+        li.chunk_base = 0;
+        li.shift = li.bits = li.mode = 0;
+        const uint32_t slot = base + lanes_below(m);
+        if (slot < sc.lq_cap) sc.lq[MAX_LEVELS][slot] = li;
+    }
+}
+
+// ---- tile tiers --------------------------------------------------------------
+
+// Wave tier: each wave sorts one queued range of n <= 64 items.
+template <typename U, typename V, bool HAS_V>
+__global__ __launch_bounds__(256) void k_wave_tier(Bufs b, Sched sc, int cls, KeyMode km, int persistent) {
+    const int waves_per_block = blockDim.x / WAVE;
+    const int wid = blockIdx.x * waves_per_block + (threadIdx.x >> 6);
+    const int stride = persistent ? gridDim.x * waves_per_block : 0x7FFFFFFF;
+    const int count = tile_count(sc, cls);
+    for (int i = wid; i < count; i += stride) {
+        const TileItem it = sc.tq[cls][i];
+        const int src = ti_src(it);
+        wave_rank_sort<U, V, HAS_V>((const U*)b.k[src], (const V*)b.v[src], buf_ptr<U>(b.k[BUF_OUT]),
+                                    buf_ptr<V>(b.v[BUF_OUT]), it.begin, ti_n(it), km);
+        if (!persistent) break;
+    }
+}
+
+// LDS tier: each workgroup sorts one queued range of n <= NT*IPT items.
+template <int NT, int IPT, typename U, typename V, bool HAS_V>
+__global__ __launch_bounds__(NT) void k_tile_tier(Bufs b, Sched sc, int cls, KeyMode km, int persistent) {
+    using T = RadixTile<NT, IPT, U, V, HAS_V>;
+    __shared__ typename T::Smem sm;
+    const int count = tile_count(sc, cls);
+    const int stride = persistent ? (int)gridDim.x : 0x7FFFFFFF;
+    for (int i = blockIdx.x; i < count; i += stride) {
+        const TileItem it = sc.tq[cls][i];
+        const int src = ti_src(it);
+        T::sort(sm, (const U*)b.k[src], (const V*)b.v[src], buf_ptr<U>(b.k[BUF_OUT]), buf_ptr<V>(b.v[BUF_OUT]),
+                it.begin, ti_n(it), km);
+        if (!persistent) break;
+    }
+}
+
+// ---- large tier (MSD partition over many workgroups) --------------------------
+
+__device__ __forceinline__ int find_item(const LargeItem* items, int count, int g) {
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].chunk_base <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int npos_of(int hi_bit, int begin_bit) {
+    return (hi_bit - begin_bit + RADIX_BITS - 1) / RADIX_BITS;
+}
+
+// Pass 1 of a level: digit histograms of every chunk for every digit position
+// below the range's hi_bit.
+template <int NT, int IPT, typename U, typename V, bool HAS_V>
+__global__ __launch_bounds__(NT) void k_large_hist(Bufs b, Sched sc, int level, KeyMode km) {
+    using T = RadixTile<NT, IPT, U, V, HAS_V>;
+    __shared__ uint32_t hist[(sizeof(U) * 8 / RADIX_BITS) * RADIX];
+    int count, chunks;
+    level_counts(sc, level, count, chunks);
+    const LargeItem* items = sc.lq[level];
+    for (int g = blockIdx.x; g < chunks; g += gridDim.x) {
+        const LargeItem it = items[find_item(items, count, g)];
+        const int c = g - it.chunk_base;
+        const int64_t start = (int64_t)it.begin + (int64_t)c * sc.chunk;
+        const int n = (it.n - c * sc.chunk) < sc.chunk ? (it.n - c * sc.chunk) : sc.chunk;
+        const int npos = npos_of(it.hi_bit, km.begin_bit);
+        for (int i = threadIdx.x; i < npos * RADIX; i += NT) hist[i] = 0;
+        __syncthreads();
+        T::histogram(hist, (const U*)b.k[it.src], start, n, km, km.begin_bit, npos, it.hi_bit);
+        __syncthreads();
+        uint32_t* out = sc.hist + (size_t)g * sc.npos_max * RADIX;
+        for (int i = threadIdx.x; i < npos * RADIX; i += NT) out[i] = hist[i];
+        __syncthreads();
+    }
+}
+
+// Pass 2 of a level: one workgroup (256 threads, one per digit) per range.
+// Chooses the highest digit position whose digit takes more than one value,
+// writes every chunk's destination offsets, and queues the child ranges.
+__global__ __launch_bounds__(RADIX) void k_large_scan(Sched sc, int level, KeyMode km, ClassCaps cc,
+                                                      int tile_max) {
+    __shared__ uint32_t nz[MAX_LEVELS];
+    __shared__ uint32_t wsum[RADIX / WAVE];
+    const int d = threadIdx.x;
+    const int w = d >> 6;
+    int count, chunks;
+    level_counts(sc, level, count, chunks);
+    LargeItem* items = sc.lq[level];
+    const int child_buf = 1 + (level & 1);  // BUF_TMP_A on even levels, BUF_TMP_B on odd
+    for (int i = blockIdx.x; i < count; i += gridDim.x) {
+        LargeItem it = items[i];
+        const int nch = (it.n + sc.chunk - 1) / sc.chunk;
+        const int npos = npos_of(it.hi_bit, km.begin_bit);
+        if (d < MAX_LEVELS) nz[d] = 0;
+        __syncthreads();
+        uint32_t tot[MAX_LEVELS];
+        for (int p = 0; p < MAX_LEVELS; ++p) tot[p] = 0;
+        for (int c = 0; c < nch; ++c) {
+            const uint32_t* h = sc.hist + (size_t)(it.chunk_base + c) * sc.npos_max * RADIX;
+            for (int p = 0; p < npos; ++p) tot[p] += h[p * RADIX + d];
+        }
+        for (int p = 0; p < npos; ++p) {
+            const uint64_t m = __ballot(tot[p] > 0);
+            if (lane_id() == 0 && m) atomicAdd(&nz[p], (uint32_t)__popcll(m));
+        }
+        __syncthreads();
+        int pstar = -1;
+        for (int p = npos - 1; p >= 0; --p)
+            if (nz[p] > 1) {
+                pstar = p;
+                break;
+            }
+        if (pstar < 0) {
+            if (d == 0) {
+                it.mode = LMODE_COPY;
+                items[i] = it;
+            }
+            __syncthreads();
+            continue;
+        }
+        bool final_ = true;
+        for (int p = 0; p < pstar; ++p)
+            if (nz[p] > 1) final_ = false;
+        const int shift = km.begin_bit + pstar * RADIX_BITS;
+        const int bits = (it.hi_bit - shift) < RADIX_BITS ? (it.hi_bit - shift) : RADIX_BITS;
+        const uint32_t mine = tot[pstar];
+        const uint32_t incl = wave_incl_scan(mine);
+        if (lane_id() == 63) wsum[w] = incl;
+        __syncthreads();
+        uint32_t base = incl - mine;
+        for (int ww = 0; ww < w; ++ww) base += wsum[ww];
+        uint32_t run = (uint32_t)it.begin + base;
+        for (int c = 0; c < nch; ++c) {
+            const size_t g = (size_t)(it.chunk_base + c);
+            sc.off[g * RADIX + d] = run;
+            run += sc.hist[g * sc.npos_max * RADIX + pstar * RADIX + d];
+        }
+        if (!final_) {
+            const int cb = it.begin + (int)base;
+            const int cn = (int)mine;
+            const int cls = (cn >= 1 && cn <= tile_max) ? class_of(cc, cn) : -1;
+            push_tile(sc, cls, cls >= 0, make_ti(cb, cls >= 0 ? cn : 0, child_buf));
+            if (cn > tile_max) push_large(sc, level + 1, cb, cn, child_buf, shift);
+        }
+        if (d == 0) {
+            it.shift = shift;
+            it.bits = bits;
+            it.mode = final_ ? LMODE_FINAL : LMODE_PARTITION;
+            items[i] = it;
+        }
+        __syncthreads();
+    }
+}
+
+// Pass 3 of a level: stable scatter of every chunk by the chosen digit into
+// the child buffer (or straight into the output when the digit is the last
+// varying one); ranges whose keys are all equal are copied to the output.
+template <int NT, int IPT, typename U, typename V, bool HAS_V>
+__global__ __launch_bounds__(NT) void k_large_scatter(Bufs b, Sched sc, int level, KeyMode km) {
+    using T = RadixTile<NT, IPT, U, V, HAS_V>;
+    __shared__ typename T::Smem sm;
+    int count, chunks;
+    level_counts(sc, level, count, chunks);
+    const LargeItem* items = sc.lq[level];
+    const int child_buf = 1 + (level & 1);
+    for (int g = blockIdx.x; g < chunks; g += gridDim.x) {
+        const LargeItem it = items[find_item(items, count, g)];
+        const int c = g - it.chunk_base;
+        const int64_t start = (int64_t)it.begin + (int64_t)c * sc.chunk;
+        const int n = (it.n - c * sc.chunk) < sc.chunk ? (it.n - c * sc.chunk) : sc.chunk;
+        const U* ks = (const U*)b.k[it.src];
+        const V* vs = (const V*)b.v[it.src];
+        if (it.mode == LMODE_COPY) {
+            U* ko = buf_ptr<U>(b.k[BUF_OUT]);
+            V* vo = buf_ptr<V>(b.v[BUF_OUT]);
+            for (int i = threadIdx.x; i < n; i += NT) {
+                ko[start + i] = ks[start + i];
+                if constexpr (HAS_V) vo[start + i] = vs[start + i];
+            }
+            continue;
+        }
+        const int dbuf = it.mode == LMODE_FINAL ? BUF_OUT : child_buf;
+        T::partition(sm, ks, vs, buf_ptr<U>(b.k[dbuf]), buf_ptr<V>(b.v[dbuf]), start, n, km, it.shift, it.bits,
+                     sc.off + (size_t)g * RADIX);
+    }
+}
+
+// ---- plain form: one workgroup per large segment, global LSD ---------------
+
+// All digit passes of one segment by one workgroup (the reference's one-CTA-
+// per-segment radix, dispatch_radix_sort.cuh:313-426, with 8-bit digits):
+// histogram the whole segment, scan, then rank and scatter tile by tile.
+template <int NT, int IPT, typename U, typename V, bool HAS_V>
+__global__ __launch_bounds__(NT) void k_seg_lsd(Bufs b, Sched sc, KeyMode km) {
+    using T = RadixTile<NT, IPT, U, V, HAS_V>;
+    __shared__ typename T::Smem sm;
+    __shared__ uint32_t hist[RADIX];
+    __shared__ uint32_t run[RADIX];
+    __shared__ uint32_t wsum[RADIX / WAVE > NT / WAVE ? RADIX / WAVE : NT / WAVE];
+    const uint32_t nl = (uint32_t)sc.ctr->level[MAX_LEVELS];
+    const int count = (int)(nl < sc.lq_cap ? nl : sc.lq_cap);
+    if ((int)blockIdx.x >= count) return;
+    const LargeItem it = sc.lq[MAX_LEVELS][blockIdx.x];
+    const int nbits = km.end_bit - km.begin_bit;
+    const int passes = (nbits + RADIX_BITS - 1) / RADIX_BITS;
+    for (int ps = 0; ps < passes; ++ps) {
+        // pass ps reads IN (ps==0) or the ping buffer, writes OUT on the last pass
+        const int sbuf = ps == 0 ? BUF_IN : ((ps & 1) ? BUF_TMP_A : BUF_TMP_B);
+        const int dbuf = ps == passes - 1 ? BUF_OUT : ((ps & 1) ? BUF_TMP_B : BUF_TMP_A);
+        const int shift = km.begin_bit + ps * RADIX_BITS;
+        const int bits = (km.end_bit - shift) < RADIX_BITS ? (km.end_bit - shift) : RADIX_BITS;
+        const U* ks = (const U*)b.k[sbuf];
+        const V* vs = (const V*)b.v[sbuf];
+        for (int i = threadIdx.x; i < RADIX; i += NT) hist[i] = 0;
+        __syncthreads();
+        T::histogram(hist, ks, it.begin, it.n, km, shift, 1, shift + bits);
+        __syncthreads();
+        // exclusive scan of hist -> run (begin-relative global offsets)
+        uint32_t mine = 0;
+        if ((int)threadIdx.x < RADIX) mine = hist[threadIdx.x];
+        const uint32_t incl = wave_incl_scan(mine);
+        if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = incl;
+        __syncthreads();
+        if ((int)threadIdx.x < RADIX) {
+            uint32_t base = incl - mine;
+            for (int ww = 0; ww < (int)(threadIdx.x >> 6); ++ww) base += wsum[ww];
+            run[threadIdx.x] = (uint32_t)it.begin + base;
+        }
+        __syncthreads();
+        for (int t0 = 0; t0 < it.n; t0 += T::TILE) {
+            const int n = (it.n - t0) < T::TILE ? (it.n - t0) : T::TILE;
+            T::partition(sm, ks, vs, buf_ptr<U>(b.k[dbuf]), buf_ptr<V>(b.v[dbuf]), (int64_t)it.begin + t0, n, km,
+                         shift, bits, run);
+            // advance every digit's running offset by this tile's count
+            for (int d = threadIdx.x; d < RADIX; d += NT) {
+                const uint32_t next = (d + 1 < RADIX) ? sm.dstart[d + 1] : (uint32_t)n;
+                run[d] += next - sm.dstart[d];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ---- fix-pass: one global LSD over (segment id, key) composites ---------------
+
+// Max of the order keys over [begin[0], end[S-1]) (fixcub.cu:141-143 takes the
+// max to size the segment-id shift).
+template <typename U>
+__global__ void k_fix_max(const U* __restrict__ keys, const int* d_begin, const int* d_end, int S, KeyMode km,
+                          unsigned long long* dmax) {
+    const int lo = d_begin[0];
+    const int hi = d_end[S - 1];
+    const U rm = range_mask<U>(km.begin_bit, km.end_bit);
+    unsigned long long m = 0;
+    for (int i = lo + blockIdx.x * blockDim.x + threadIdx.x; i < hi; i += gridDim.x * blockDim.x) {
+        const unsigned long long okey = (unsigned long long)((twiddle_in<U>(keys[i], km.kind, km.desc) & rm) >> km.begin_bit);
+        m = okey > m ? okey : m;
+    }
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const unsigned long long y = __shfl_xor(m, o, WAVE);
+        m = y > m ? y : m;
+    }
+    if (lane_id() == 0) atomicMax(dmax, m);
+}
+
+// composite[i - lo] = (segment id << kbits) | order key, idx[i - lo] = i - lo.
+// Segments must tile [begin[0], end[S-1]) in order (CSR offsets), as the
+// reference's fix-pass assumes (fixcub.cu:103-108).
+template <typename U>
+__global__ void k_fix_composite(const U* __restrict__ keys, const int* __restrict__ d_begin, const int* d_end, int S,
+                                KeyMode km, const unsigned long long* dmax, unsigned long long* comp, uint32_t* idx) {
+    const int lo = d_begin[0];
+    const int hi = d_end[S - 1];
+    const U rm = range_mask<U>(km.begin_bit, km.end_bit);
+    const int kbits = bit_length<unsigned long long>(*dmax);
+    for (int i = lo + blockIdx.x * blockDim.x + threadIdx.x; i < hi; i += gridDim.x * blockDim.x) {
+        // last segment whose begin <= i
+        int a = 0, z = S - 1;
+        while (a < z) {
+            const int mid = (a + z + 1) >> 1;
+            if (d_begin[mid] <= i) a = mid;
+            else z = mid - 1;
+        }
+        const unsigned long long okey = (unsigned long long)((twiddle_in<U>(keys[i], km.kind, km.desc) & rm) >> km.begin_bit);
+        comp[i - lo] = ((unsigned long long)a << kbits) | okey;
+        idx[i - lo] = (uint32_t)(i - lo);
+    }
+}
+
+// Upsweep of one fix-pass digit pass: per-chunk digit counts, digit-major.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_fix_upsweep(const unsigned long long* __restrict__ comp, const int* d_begin,
+                                                    const int* d_end, int S, const unsigned long long* dmax,
+                                                    int segbits, int pass, int chunk, int nchunks_max,
+                                                    uint32_t* hist) {
+    __shared__ uint32_t h[RADIX];
+    const int m = d_end[S - 1] - d_begin[0];
+    const int total_bits = bit_length<unsigned long long>(*dmax) + segbits;
+    const int shift = pass * RADIX_BITS;
+    if (shift >= total_bits) return;
+    const int nch = (m + chunk - 1) / chunk;
+    for (int g = blockIdx.x; g < nch; g += gridDim.x) {
+        for (int i = threadIdx.x; i < RADIX; i += NT) h[i] = 0;
+        __syncthreads();
+        const int start = g * chunk;
+        const int n = (m - start) < chunk ? (m - start) : chunk;
+        for (int i = threadIdx.x; i < n; i += NT) atomicAdd(&h[(uint32_t)(comp[start + i] >> shift) & 0xFFu], 1u);
+        __syncthreads();
+        for (int i = threadIdx.x; i < RADIX; i += NT) hist[(size_t)i * nchunks_max + g] = h[i];
+        __syncthreads();
+    }
+}
+
+// Spine of one fix-pass digit pass: exclusive scan over hist[digit][chunk] in
+// digit-major order (one workgroup).
+template <int NT>
+__global__ __launch_bounds__(NT) void k_fix_spine(const int* d_begin, const int* d_end, int S,
+                                                  const unsigned long long* dmax, int segbits, int pass, int chunk,
+                                                  int nchunks_max, uint32_t* hist) {
+    __shared__ uint32_t wsum[NT / WAVE];
+    __shared__ uint32_t carry;
+    const int m = d_end[S - 1] - d_begin[0];
+    const int total_bits = bit_length<unsigned long long>(*dmax) + segbits;
+    if (pass * RADIX_BITS >= total_bits) return;
+    const int nch = (m + chunk - 1) / chunk;
+    const int total = RADIX * nch;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int t0 = 0; t0 < total; t0 += NT) {
+        const int i = t0 + (int)threadIdx.x;
+        uint32_t x = 0;
+        size_t addr = 0;
+        if (i < total) {
+            addr = (size_t)(i / nch) * nchunks_max + (i % nch);
+            x = hist[addr];
+        }
+        const uint32_t incl = wave_incl_scan(x);
+        if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = incl;
+        __syncthreads();
+        uint32_t base = carry + incl - x;
+        for (int ww = 0; ww < (int)(threadIdx.x >> 6); ++ww) base += wsum[ww];
+        if (i < total) hist[addr] = base;
+        __syncthreads();
+        if (threadIdx.x == NT - 1) carry = base + x;
+        __syncthreads();
+    }
+}
+
+// Downsweep of one fix-pass digit pass: stable partition of each chunk of
+// (composite, index) pairs by the pass digit into the other ping-pong buffer.
+template <int NT, int IPT>
+__global__ __launch_bounds__(NT) void k_fix_downsweep(const unsigned long long* comp_in, const uint32_t* idx_in,
+                                                      unsigned long long* comp_out, uint32_t* idx_out,
+                                                      const int* d_begin, const int* d_end, int S,
+                                                      const unsigned long long* dmax, int segbits, int pass,
+                                                      int nchunks_max, const uint32_t* hist) {
+    using T = RadixTile<NT, IPT, unsigned long long, uint32_t, true>;
+    __shared__ typename T::Smem sm;
+    __shared__ uint32_t goff[RADIX];
+    const int m = d_end[S - 1] - d_begin[0];
+    const int total_bits = bit_length<unsigned long long>(*dmax) + segbits;
+    const int shift = pass * RADIX_BITS;
+    if (shift >= total_bits) return;
+    const int bits = (total_bits - shift) < RADIX_BITS ? (total_bits - shift) : RADIX_BITS;
+    const int nch = (m + T::TILE - 1) / T::TILE;
+    KeyMode km;
+    km.kind = KIND_UNSIGNED;
+    km.desc = 0;
+    km.begin_bit = 0;
+    km.end_bit = 64;
+    for (int g = blockIdx.x; g < nch; g += gridDim.x) {
+        for (int d = threadIdx.x; d < RADIX; d += NT) goff[d] = hist[(size_t)d * nchunks_max + g];
+        __syncthreads();
+        const int start = g * T::TILE;
+        const int n = (m - start) < T::TILE ? (m - start) : T::TILE;
+        T::partition(sm, comp_in, idx_in, comp_out, idx_out, start, n, km, shift, bits, goff);
+    }
+}
+
+// Final fix-pass gather: sorted position k takes the key and value of input
+// element lo + idx[k]; the data sits in ping-pong buffer (passes run) & 1.
+template <typename U, typename V, bool HAS_V>
+__global__ void k_fix_gather(Bufs b, const uint32_t* idx0, const uint32_t* idx1, const int* d_begin, const int* d_end,
+                             int S, const unsigned long long* dmax, int segbits) {
+    const int lo = d_begin[0];
+    const int m = d_end[S - 1] - lo;
+    const int total_bits = bit_length<unsigned long long>(*dmax) + segbits;
+    const int passes = (total_bits + RADIX_BITS - 1) / RADIX_BITS;
+    const uint32_t* idx = (passes & 1) ? idx1 : idx0;
+    const U* ki = (const U*)b.k[BUF_IN];
+    const V* vi = (const V*)b.v[BUF_IN];
+    U* ko = buf_ptr<U>(b.k[BUF_OUT]);
+    V* vo = buf_ptr<V>(b.v[BUF_OUT]);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) {
+        const uint32_t src = idx[k];
+        ko[lo + k] = ki[lo + src];
+        if constexpr (HAS_V) vo[lo + k] = vi[lo + src];
+    }
+}
+
+// ---- differentiable pair sort helpers ---------------------------------------
+
+// vals[i] = i - begin[r] for every i of row r (one workgroup per row).
+__global__ void k_row_iota(const int* d_begin, const int* d_end, int rows, int* vals) {
+    for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+        const int b0 = d_begin[r], e0 = d_end[r];
+        for (int i = b0 + threadIdx.x; i < e0; i += blockDim.x) vals[i] = i - b0;
+    }
+}
+
+// Backward of the pair sort: gin[begin[r] + perm[i]] = gout[i] for every i of
+// row r, perm holding row-local source indices (one workgroup per row).
+__global__ void k_perm_scatter_f32(const float* __restrict__ gout, const int* __restrict__ perm, const int* d_begin,
+                                   const int* d_end, int rows, float* __restrict__ gin) {
+    for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+        const int b0 = d_begin[r], e0 = d_end[r];
+        for (int i = b0 + threadIdx.x; i < e0; i += blockDim.x) gin[b0 + perm[i]] = gout[i];
+    }
+}
+
+}  // namespace ssort

@@ -1,0 +1,572 @@
+// C-ABI of libsegsort.so: argument checks, temp-storage layout, tuning table
+// and the launch sequences of the batched, plain and fix-pass forms.
+#include "kernels.hpp"
+#include "../../include/segsort.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+using namespace ssort;
+
+namespace {
+
+thread_local std::string g_last_error = "no error";
+
+int fail(hipError_t e, const char* what) {
+    g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+    return (int)e;
+}
+
+int ok() {
+    g_last_error = "no error";
+    return 0;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// ---- tuning ------------------------------------------------------------------
+
+struct Tuning {
+    int wave_cap;
+    int tile_enable;
+    int grid_mult;
+};
+
+// index: (key_width == 8) * 3 + value-bytes slot (0, 4, 8 -> 0, 1, 2)
+Tuning g_tuning[6] = {{32, 0xF, 1}, {32, 0xF, 1}, {32, 0xF, 1}, {32, 0xF, 1}, {32, 0xF, 1}, {32, 0xF, 1}};
+std::mutex g_tuning_mu;
+
+int tuning_slot(int key_width, int value_bytes) {
+    const int vs = value_bytes == 0 ? 0 : value_bytes == 4 ? 1 : value_bytes == 8 ? 2 : -1;
+    if (vs < 0 || (key_width != 4 && key_width != 8)) return -1;
+    return (key_width == 8 ? 3 : 0) + vs;
+}
+
+Tuning get_tuning(int key_width, int value_bytes) {
+    std::lock_guard<std::mutex> lk(g_tuning_mu);
+    return g_tuning[tuning_slot(key_width, value_bytes)];
+}
+
+int device_cus() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) return 256;
+    return cus;
+}
+
+template <typename K>
+int resident_blocks(K kernel, int threads) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess || per_cu <= 0)
+        per_cu = 1;
+    return per_cu * device_cus();
+}
+
+// ---- per-(key, value) configuration -------------------------------------------
+
+// LDS tile classes: (threads, items per thread). The largest tile also sets the
+// chunk of the large tier and of the plain form's per-segment LSD.
+template <typename U, typename V, bool HAS_V>
+struct Cfg {
+    static constexpr int ITEM_BYTES = int(sizeof(U)) + (HAS_V ? int(sizeof(V)) : 0);
+    static constexpr int NT0 = 64, IPT0 = 4;
+    static constexpr int NT1 = 256, IPT1 = 4;
+    static constexpr int NT2 = 256, IPT2 = 16;
+    static constexpr int NT3 = 1024, IPT3 = ITEM_BYTES <= 8 ? 16 : 8;
+    static constexpr int CAP[4] = {NT0 * IPT0, NT1 * IPT1, NT2 * IPT2, NT3 * IPT3};
+    static constexpr int TILE_MAX = NT3 * IPT3;
+    static constexpr int NPOS = int(sizeof(U));  // 8-bit digit positions of a key
+};
+
+struct Plan {
+    ClassCaps cc;
+    int kind[MAX_TILE_CLASSES];  // 0 = wave tier, 1 + i = LDS tile class i
+    int nclass;
+    int tile_max;
+};
+
+template <typename C>
+Plan make_plan(const Tuning& tu) {
+    Plan p;
+    p.nclass = 0;
+    int wave_cap = std::min(64, std::max(2, tu.wave_cap));
+    p.cc.cap[p.nclass] = wave_cap;
+    p.kind[p.nclass++] = 0;
+    int enable = tu.tile_enable | 0x8;  // the largest tile is always on
+    for (int i = 0; i < 4; ++i) {
+        if (!(enable & (1 << i))) continue;
+        if (C::CAP[i] <= wave_cap) continue;
+        p.cc.cap[p.nclass] = C::CAP[i];
+        p.kind[p.nclass++] = 1 + i;
+    }
+    p.cc.nclass = p.nclass;
+    for (int c = p.nclass; c < MAX_TILE_CLASSES; ++c) p.cc.cap[c] = 0;
+    p.tile_max = C::TILE_MAX;
+    return p;
+}
+
+struct Layout {
+    size_t ctr, tq[MAX_TILE_CLASSES], lq[MAX_LEVELS + 1], hist, off, tk[2], tv[2];
+    size_t total;
+    size_t tq_cap[MAX_TILE_CLASSES];
+    size_t lq_cap;
+    size_t chunks;
+};
+
+template <typename C, typename U, typename V, bool HAS_V>
+Layout make_layout(const Plan& p, int num_items, int num_segments, int levels, bool plain) {
+    Layout L;
+    memset(&L, 0, sizeof(L));
+    size_t at = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = at;
+        at += align256(bytes);
+        return o;
+    };
+    const size_t N = (size_t)num_items, S = (size_t)num_segments;
+    L.lq_cap = N / (size_t)(C::TILE_MAX + 1) + 1;
+    L.chunks = N / (size_t)C::TILE_MAX + L.lq_cap + 1;
+    const size_t bucket_bound = plain ? 0 : (size_t)RADIX * L.lq_cap * (size_t)std::max(levels, 1);
+    L.ctr = take(sizeof(Counters));
+    for (int c = 0; c < p.nclass; ++c) {
+        const size_t lo = c == 0 ? 1 : (size_t)p.cc.cap[c - 1] + 1;
+        (void)lo;
+        L.tq_cap[c] = S + bucket_bound;
+        L.tq[c] = take(L.tq_cap[c] * sizeof(TileItem));
+    }
+    if (plain) {
+        L.lq_cap = std::max<size_t>(std::min(S, L.lq_cap), 1);
+        L.lq[MAX_LEVELS] = take(L.lq_cap * sizeof(LargeItem));
+    } else {
+        for (int l = 0; l < levels; ++l) L.lq[l] = take(L.lq_cap * sizeof(LargeItem));
+        L.hist = take(L.chunks * C::NPOS * RADIX * sizeof(uint32_t));
+        L.off = take(L.chunks * RADIX * sizeof(uint32_t));
+    }
+    const bool need_tmp = plain ? (N > (size_t)C::TILE_MAX) : (levels > 0);
+    for (int i = 0; i < 2; ++i) {
+        L.tk[i] = need_tmp ? take(N * sizeof(U)) : 0;
+        L.tv[i] = (need_tmp && HAS_V) ? take(N * sizeof(V)) : 0;
+    }
+    L.total = at + 255;  // slack for aligning the caller's pointer
+    return L;
+}
+
+template <typename U, typename V, bool HAS_V>
+struct Launcher {
+    using C = Cfg<U, V, HAS_V>;
+
+    static void launch_class(int kind, Bufs b, Sched sc, int cls, KeyMode km, hipStream_t st, int grid, bool persistent,
+                             int grid_mult) {
+        const int pers = persistent ? 1 : 0;
+        switch (kind) {
+        case 0: {
+            auto k = k_wave_tier<U, V, HAS_V>;
+            int g = persistent ? resident_blocks(k, 256) * grid_mult : (grid + 3) / 4;
+            if (g > 0) k<<<g, 256, 0, st>>>(b, sc, cls, km, pers);
+            break;
+        }
+        case 1: {
+            auto k = k_tile_tier<C::NT0, C::IPT0, U, V, HAS_V>;
+            int g = persistent ? resident_blocks(k, C::NT0) * grid_mult : grid;
+            if (g > 0) k<<<g, C::NT0, 0, st>>>(b, sc, cls, km, pers);
+            break;
+        }
+        case 2: {
+            auto k = k_tile_tier<C::NT1, C::IPT1, U, V, HAS_V>;
+            int g = persistent ? resident_blocks(k, C::NT1) * grid_mult : grid;
+            if (g > 0) k<<<g, C::NT1, 0, st>>>(b, sc, cls, km, pers);
+            break;
+        }
+        case 3: {
+            auto k = k_tile_tier<C::NT2, C::IPT2, U, V, HAS_V>;
+            int g = persistent ? resident_blocks(k, C::NT2) * grid_mult : grid;
+            if (g > 0) k<<<g, C::NT2, 0, st>>>(b, sc, cls, km, pers);
+            break;
+        }
+        case 4: {
+            auto k = k_tile_tier<C::NT3, C::IPT3, U, V, HAS_V>;
+            int g = persistent ? resident_blocks(k, C::NT3) * grid_mult : grid;
+            if (g > 0) k<<<g, C::NT3, 0, st>>>(b, sc, cls, km, pers);
+            break;
+        }
+        }
+    }
+
+    static Sched make_sched(char* base, const Layout& L, const Plan& p, bool plain) {
+        Sched sc;
+        memset(&sc, 0, sizeof(sc));
+        sc.ctr = (Counters*)(base + L.ctr);
+        for (int c = 0; c < p.nclass; ++c) sc.tq[c] = (TileItem*)(base + L.tq[c]);
+        if (plain) {
+            sc.lq[MAX_LEVELS] = (LargeItem*)(base + L.lq[MAX_LEVELS]);
+        } else {
+            for (int l = 0; l < MAX_LEVELS; ++l) sc.lq[l] = L.lq[l] ? (LargeItem*)(base + L.lq[l]) : nullptr;
+            sc.hist = (uint32_t*)(base + L.hist);
+            sc.off = (uint32_t*)(base + L.off);
+        }
+        sc.chunk = C::TILE_MAX;
+        sc.npos_max = C::NPOS;
+        for (int c = 0; c < p.nclass; ++c) sc.tq_cap[c] = (uint32_t)std::min<size_t>(L.tq_cap[c], 0xFFFFFFFFu);
+        sc.lq_cap = (uint32_t)L.lq_cap;
+        sc.chunk_cap = (uint32_t)L.chunks;
+        return sc;
+    }
+
+    static Bufs make_bufs(char* base, const Layout& L, const void* ki, void* ko, const void* vi, void* vo) {
+        Bufs b;
+        b.k[BUF_IN] = ki;
+        b.k[BUF_OUT] = ko;
+        b.v[BUF_IN] = vi;
+        b.v[BUF_OUT] = vo;
+        for (int i = 0; i < 2; ++i) {
+            b.k[BUF_TMP_A + i] = L.tk[i] ? base + L.tk[i] : nullptr;
+            b.v[BUF_TMP_A + i] = L.tv[i] ? base + L.tv[i] : nullptr;
+        }
+        return b;
+    }
+
+    static int levels_for(const KeyMode& km, int num_items) {
+        if (num_items <= C::TILE_MAX) return 0;
+        return (km.end_bit - km.begin_bit + RADIX_BITS - 1) / RADIX_BITS;
+    }
+
+    static int batched(void* d_temp, size_t* temp_bytes, const void* ki, void* ko, const void* vi, void* vo, int N,
+                       int S, const int* db, const int* de, KeyMode km, hipStream_t st) {
+        const Tuning tu = get_tuning(sizeof(U), HAS_V ? sizeof(V) : 0);
+        const Plan p = make_plan<C>(tu);
+        const int levels = levels_for(km, N);
+        const Layout L = make_layout<C, U, V, HAS_V>(p, N, S, levels, false);
+        if (!d_temp) {
+            *temp_bytes = L.total;
+            return ok();
+        }
+        if (*temp_bytes < L.total) return fail(hipErrorInvalidValue, "segsort_sort: temp storage too small");
+        if (N == 0 || S == 0) return ok();
+        char* base = (char*)((((uintptr_t)d_temp) + 255) & ~uintptr_t(255));
+        const Sched sc = make_sched(base, L, p, false);
+        const Bufs b = make_bufs(base, L, ki, ko, vi, vo);
+        k_init<<<1, 64, 0, st>>>(sc.ctr);
+        k_bin<U, V, HAS_V><<<(S + 255) / 256, 256, 0, st>>>(db, de, S, b, sc, p.cc, km.end_bit, 1);
+        for (int l = 0; l < levels; ++l) {
+            auto kh = k_large_hist<C::NT3, C::IPT3, U, V, HAS_V>;
+            auto ks = k_large_scatter<C::NT3, C::IPT3, U, V, HAS_V>;
+            const int gh = resident_blocks(kh, C::NT3) * 2;
+            const int gs = resident_blocks(ks, C::NT3) * tu.grid_mult;
+            kh<<<gh, C::NT3, 0, st>>>(b, sc, l, km);
+            k_large_scan<<<device_cus() * 4, RADIX, 0, st>>>(sc, l, km, p.cc, C::TILE_MAX);
+            ks<<<gs, C::NT3, 0, st>>>(b, sc, l, km);
+        }
+        for (int c = 0; c < p.nclass; ++c) launch_class(p.kind[c], b, sc, c, km, st, 0, true, tu.grid_mult);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return fail(e, "segsort_sort(batched): launch");
+        return ok();
+    }
+
+    static int plain(void* d_temp, size_t* temp_bytes, const void* ki, void* ko, const void* vi, void* vo, int N,
+                     int S, const int* db, const int* de, KeyMode km, hipStream_t st) {
+        const Tuning tu = get_tuning(sizeof(U), HAS_V ? sizeof(V) : 0);
+        const Plan p = make_plan<C>(tu);
+        const Layout L = make_layout<C, U, V, HAS_V>(p, N, S, 0, true);
+        if (!d_temp) {
+            *temp_bytes = L.total;
+            return ok();
+        }
+        if (*temp_bytes < L.total) return fail(hipErrorInvalidValue, "segsort_sort: temp storage too small");
+        if (N == 0 || S == 0) return ok();
+        char* base = (char*)((((uintptr_t)d_temp) + 255) & ~uintptr_t(255));
+        const Sched sc = make_sched(base, L, p, true);
+        const Bufs b = make_bufs(base, L, ki, ko, vi, vo);
+        k_init<<<1, 64, 0, st>>>(sc.ctr);
+        k_bin<U, V, HAS_V><<<(S + 255) / 256, 256, 0, st>>>(db, de, S, b, sc, p.cc, km.end_bit, 0);
+        k_bin_large_ids<<<(S + 255) / 256, 256, 0, st>>>(db, de, S, sc, p.cc);
+        Counters h;
+        hipError_t e = hipMemcpyAsync(&h, sc.ctr, sizeof(Counters), hipMemcpyDeviceToHost, st);
+        if (e != hipSuccess) return fail(e, "segsort_sort(plain): count read-back");
+        e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return fail(e, "segsort_sort(plain): sync");
+        for (int c = 0; c < p.nclass; ++c)
+            if (h.tile_count[c]) launch_class(p.kind[c], b, sc, c, km, st, (int)h.tile_count[c], false, 1);
+        const int nlarge = (int)h.level[MAX_LEVELS];
+        if (nlarge > 0) {
+            if (km.end_bit > km.begin_bit)
+                k_seg_lsd<C::NT3, C::IPT3, U, V, HAS_V><<<nlarge, C::NT3, 0, st>>>(b, sc, km);
+        }
+        e = hipGetLastError();
+        if (e != hipSuccess) return fail(e, "segsort_sort(plain): launch");
+        return ok();
+    }
+};
+
+// Fix-pass temp: counters | comp[2][M] | idx[2][M] | hist[RADIX][chunks]
+constexpr int FIX_NT = 1024, FIX_IPT = 8, FIX_TILE = FIX_NT * FIX_IPT;
+
+template <typename U, typename V, bool HAS_V>
+int fixpass(void* d_temp, size_t* temp_bytes, const void* ki, void* ko, const void* vi, void* vo, int N, int S,
+            const int* db, const int* de, KeyMode km, hipStream_t st) {
+    const size_t M = (size_t)N;
+    const size_t nch = M / FIX_TILE + 2;
+    size_t at = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = at;
+        at += align256(bytes);
+        return o;
+    };
+    const size_t o_ctr = take(sizeof(Counters));
+    const size_t o_c0 = take(M * 8), o_c1 = take(M * 8);
+    const size_t o_i0 = take(M * 4), o_i1 = take(M * 4);
+    const size_t o_h = take(nch * RADIX * 4);
+    const size_t total = at + 255;
+    if (!d_temp) {
+        *temp_bytes = total;
+        return ok();
+    }
+    if (*temp_bytes < total) return fail(hipErrorInvalidValue, "segsort_sort(fixpass): temp storage too small");
+    if (N == 0 || S == 0) return ok();
+    char* base = (char*)((((uintptr_t)d_temp) + 255) & ~uintptr_t(255));
+    Counters* ctr = (Counters*)(base + o_ctr);
+    unsigned long long* dmax = &ctr->level[0];
+    unsigned long long* comp[2] = {(unsigned long long*)(base + o_c0), (unsigned long long*)(base + o_c1)};
+    uint32_t* idx[2] = {(uint32_t*)(base + o_i0), (uint32_t*)(base + o_i1)};
+    uint32_t* hist = (uint32_t*)(base + o_h);
+    int segbits = 0;
+    while (segbits < 31 && (1ll << segbits) < (long long)S) ++segbits;  // bit length of S-1
+    const int cus = device_cus();
+    k_init<<<1, 64, 0, st>>>(ctr);
+    k_fix_max<U><<<cus * 4, 256, 0, st>>>((const U*)ki, db, de, S, km, dmax);
+    k_fix_composite<U><<<cus * 8, 256, 0, st>>>((const U*)ki, db, de, S, km, dmax, comp[0], idx[0]);
+    const int kb = km.end_bit - km.begin_bit;
+    const int max_passes = (kb + segbits + RADIX_BITS - 1) / RADIX_BITS;
+    auto kd = k_fix_downsweep<FIX_NT, FIX_IPT>;
+    const int gd = resident_blocks(kd, FIX_NT);
+    for (int ps = 0; ps < max_passes; ++ps) {
+        const int s = ps & 1;
+        k_fix_upsweep<256><<<cus * 4, 256, 0, st>>>(comp[s], db, de, S, dmax, segbits, ps, FIX_TILE, (int)nch, hist);
+        k_fix_spine<1024><<<1, 1024, 0, st>>>(db, de, S, dmax, segbits, ps, FIX_TILE, (int)nch, hist);
+        kd<<<gd, FIX_NT, 0, st>>>(comp[s], idx[s], comp[s ^ 1], idx[s ^ 1], db, de, S, dmax, segbits, ps, (int)nch,
+                                 hist);
+    }
+    Bufs b;
+    memset(&b, 0, sizeof(b));
+    b.k[BUF_IN] = ki;
+    b.k[BUF_OUT] = ko;
+    b.v[BUF_IN] = vi;
+    b.v[BUF_OUT] = vo;
+    k_fix_gather<U, V, HAS_V><<<cus * 8, 256, 0, st>>>(b, idx[0], idx[1], db, de, S, dmax, segbits);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(e, "segsort_sort(fixpass): launch");
+    return ok();
+}
+
+template <typename U, typename V, bool HAS_V>
+int dispatch_algo(int algo, void* d_temp, size_t* temp_bytes, const void* ki, void* ko, const void* vi, void* vo,
+                  int N, int S, const int* db, const int* de, KeyMode km, hipStream_t st) {
+    switch (algo) {
+    case SEGSORT_ALGO_BATCHED:
+        return Launcher<U, V, HAS_V>::batched(d_temp, temp_bytes, ki, ko, vi, vo, N, S, db, de, km, st);
+    case SEGSORT_ALGO_PLAIN:
+        return Launcher<U, V, HAS_V>::plain(d_temp, temp_bytes, ki, ko, vi, vo, N, S, db, de, km, st);
+    case SEGSORT_ALGO_FIXPASS:
+        if constexpr (sizeof(U) == 4) return fixpass<U, V, HAS_V>(d_temp, temp_bytes, ki, ko, vi, vo, N, S, db, de, km, st);
+        return fail(hipErrorInvalidValue, "segsort_sort(fixpass): 32-bit keys only");
+    }
+    return fail(hipErrorInvalidValue, "segsort_sort: unknown algo");
+}
+
+}  // namespace
+
+extern "C" {
+
+int segsort_version(void) { return 100; }
+
+const char* segsort_last_error(void) { return g_last_error.c_str(); }
+
+int segsort_sort(void* d_temp, size_t* temp_bytes, const void* d_keys_in, void* d_keys_out, const void* d_vals_in,
+                 void* d_vals_out, int num_items, int num_segments, const int* d_begin_offsets,
+                 const int* d_end_offsets, int begin_bit, int end_bit, int key_type, int value_bytes, int descending,
+                 int algo, void* stream) {
+    if (!temp_bytes) return fail(hipErrorInvalidValue, "segsort_sort: temp_bytes is NULL");
+    if (num_items < 0 || num_segments < 0) return fail(hipErrorInvalidValue, "segsort_sort: negative size");
+    if (key_type < SEGSORT_KEY_U32 || key_type > SEGSORT_KEY_F64)
+        return fail(hipErrorInvalidValue, "segsort_sort: bad key_type");
+    if (value_bytes != 0 && value_bytes != 4 && value_bytes != 8)
+        return fail(hipErrorInvalidValue, "segsort_sort: value_bytes must be 0, 4 or 8");
+    const int kw = key_type >= SEGSORT_KEY_U64 ? 8 : 4;
+    if (begin_bit < 0 || end_bit > kw * 8 || begin_bit > end_bit)
+        return fail(hipErrorInvalidValue, "segsort_sort: bad bit range");
+    if (d_temp && num_items > 0 && num_segments > 0) {
+        if (!d_keys_in || !d_keys_out || !d_begin_offsets || !d_end_offsets)
+            return fail(hipErrorInvalidValue, "segsort_sort: NULL buffer");
+        if (value_bytes && (!d_vals_in || !d_vals_out)) return fail(hipErrorInvalidValue, "segsort_sort: NULL values");
+    }
+    KeyMode km;
+    km.kind = (key_type % 3 == 0) ? KIND_UNSIGNED : (key_type % 3 == 1) ? KIND_SIGNED : KIND_FLOAT;
+    km.desc = descending ? 1 : 0;
+    km.begin_bit = begin_bit;
+    km.end_bit = end_bit;
+    hipStream_t st = (hipStream_t)stream;
+#define SEGSORT_ARGS algo, d_temp, temp_bytes, d_keys_in, d_keys_out, d_vals_in, d_vals_out, num_items, num_segments, \
+                     d_begin_offsets, d_end_offsets, km, st
+    if (kw == 4) {
+        if (value_bytes == 0) return dispatch_algo<uint32_t, uint32_t, false>(SEGSORT_ARGS);
+        if (value_bytes == 4) return dispatch_algo<uint32_t, uint32_t, true>(SEGSORT_ARGS);
+        return dispatch_algo<uint32_t, uint64_t, true>(SEGSORT_ARGS);
+    }
+    if (value_bytes == 0) return dispatch_algo<uint64_t, uint32_t, false>(SEGSORT_ARGS);
+    if (value_bytes == 4) return dispatch_algo<uint64_t, uint32_t, true>(SEGSORT_ARGS);
+    return dispatch_algo<uint64_t, uint64_t, true>(SEGSORT_ARGS);
+#undef SEGSORT_ARGS
+}
+
+int segsort_sort_pairs_u32(void* d_temp, size_t* temp_bytes, const uint32_t* d_keys_in, uint32_t* d_keys_out,
+                           const uint32_t* d_vals_in, uint32_t* d_vals_out, int num_items, int num_segments,
+                           const int* d_begin_offsets, const int* d_end_offsets, int algo, void* stream) {
+    return segsort_sort(d_temp, temp_bytes, d_keys_in, d_keys_out, d_vals_in, d_vals_out, num_items, num_segments,
+                        d_begin_offsets, d_end_offsets, 0, 32, SEGSORT_KEY_U32, 4, 0, algo, stream);
+}
+
+int segsort_sort_keys_u32(void* d_temp, size_t* temp_bytes, const uint32_t* d_keys_in, uint32_t* d_keys_out,
+                          int num_items, int num_segments, const int* d_begin_offsets, const int* d_end_offsets,
+                          int algo, void* stream) {
+    return segsort_sort(d_temp, temp_bytes, d_keys_in, d_keys_out, nullptr, nullptr, num_items, num_segments,
+                        d_begin_offsets, d_end_offsets, 0, 32, SEGSORT_KEY_U32, 0, 0, algo, stream);
+}
+
+int segsort_row_iota(const int* d_begin_offsets, const int* d_end_offsets, int rows, int* d_perm, void* stream) {
+    if (rows < 0) return fail(hipErrorInvalidValue, "segsort_row_iota: negative rows");
+    if (rows == 0) return ok();
+    k_row_iota<<<std::min(rows, 65535), 256, 0, (hipStream_t)stream>>>(d_begin_offsets, d_end_offsets, rows, d_perm);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? ok() : fail(e, "segsort_row_iota");
+}
+
+int segsort_perm_scatter_f32(const float* d_grad_out, const int* d_perm, const int* d_begin_offsets,
+                             const int* d_end_offsets, int rows, float* d_grad_in, void* stream) {
+    if (rows < 0) return fail(hipErrorInvalidValue, "segsort_perm_scatter_f32: negative rows");
+    if (rows == 0) return ok();
+    k_perm_scatter_f32<<<std::min(rows, 65535), 256, 0, (hipStream_t)stream>>>(d_grad_out, d_perm, d_begin_offsets,
+                                                                               d_end_offsets, rows, d_grad_in);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? ok() : fail(e, "segsort_perm_scatter_f32");
+}
+
+int segsort_set_tuning(int key_width, int value_bytes, int wave_cap, int tile_enable, int grid_mult) {
+    const int s = tuning_slot(key_width, value_bytes);
+    if (s < 0 || wave_cap < 2 || wave_cap > 64 || tile_enable < 0 || tile_enable > 0xF || grid_mult < 1 ||
+        grid_mult > 8)
+        return fail(hipErrorInvalidValue, "segsort_set_tuning: out of range");
+    std::lock_guard<std::mutex> lk(g_tuning_mu);
+    g_tuning[s] = {wave_cap, tile_enable, grid_mult};
+    return ok();
+}
+
+int segsort_get_tuning(int key_width, int value_bytes, int* wave_cap, int* tile_enable, int* grid_mult) {
+    const int s = tuning_slot(key_width, value_bytes);
+    if (s < 0) return fail(hipErrorInvalidValue, "segsort_get_tuning: bad slot");
+    std::lock_guard<std::mutex> lk(g_tuning_mu);
+    if (wave_cap) *wave_cap = g_tuning[s].wave_cap;
+    if (tile_enable) *tile_enable = g_tuning[s].tile_enable;
+    if (grid_mult) *grid_mult = g_tuning[s].grid_mult;
+    return ok();
+}
+
+int segsort_tile_caps(int key_width, int value_bytes, int* caps) {
+    const int s = tuning_slot(key_width, value_bytes);
+    if (s < 0 || !caps) return fail(hipErrorInvalidValue, "segsort_tile_caps: bad arguments");
+    const int* src = nullptr;
+    if (key_width == 4)
+        src = value_bytes == 0 ? Cfg<uint32_t, uint32_t, false>::CAP
+              : value_bytes == 4 ? Cfg<uint32_t, uint32_t, true>::CAP
+                                 : Cfg<uint32_t, uint64_t, true>::CAP;
+    else
+        src = value_bytes == 0 ? Cfg<uint64_t, uint32_t, false>::CAP
+              : value_bytes == 4 ? Cfg<uint64_t, uint32_t, true>::CAP
+                                 : Cfg<uint64_t, uint64_t, true>::CAP;
+    for (int i = 0; i < 4; ++i) caps[i] = src[i];
+    return 4;
+}
+
+// In-place adapters: sort into a scratch copy, then copy back.
+static int inplace_sort(void* d_keys, void* d_vals, int n, const int* d_segs, int S, int key_type, int value_bytes,
+                        hipStream_t st) {
+    if (n <= 0 || S <= 0) return ok();
+    const int kw = key_type >= SEGSORT_KEY_U64 ? 8 : 4;
+    const int vb = d_vals ? value_bytes : 0;
+    // end offsets: d_segs[1..S-1] then n
+    int* d_end = nullptr;
+    void *kout = nullptr, *vout = nullptr, *temp = nullptr;
+    size_t tb = 0;
+    int rc = 0;
+    hipError_t e;
+    if ((e = hipMalloc(&d_end, sizeof(int) * (size_t)S)) != hipSuccess) return fail(e, "adapter: alloc");
+    if (S > 1 && (e = hipMemcpyAsync(d_end, d_segs + 1, sizeof(int) * (size_t)(S - 1), hipMemcpyDeviceToDevice, st)) !=
+                     hipSuccess) {
+        rc = fail(e, "adapter: copy offsets");
+        goto done;
+    }
+    if ((e = hipMemcpyAsync(d_end + (S - 1), &n, sizeof(int), hipMemcpyHostToDevice, st)) != hipSuccess) {
+        rc = fail(e, "adapter: copy n");
+        goto done;
+    }
+    if ((e = hipMalloc(&kout, (size_t)n * kw)) != hipSuccess) {
+        rc = fail(e, "adapter: alloc keys");
+        goto done;
+    }
+    if (vb && (e = hipMalloc(&vout, (size_t)n * vb)) != hipSuccess) {
+        rc = fail(e, "adapter: alloc values");
+        goto done;
+    }
+    rc = segsort_sort(nullptr, &tb, d_keys, kout, d_vals, vout, n, S, d_segs, d_end, 0, kw * 8, key_type, vb, 0,
+                      SEGSORT_ALGO_BATCHED, st);
+    if (rc) goto done;
+    if ((e = hipMalloc(&temp, tb)) != hipSuccess) {
+        rc = fail(e, "adapter: alloc temp");
+        goto done;
+    }
+    // positions before d_segs[0] are not in any segment: keep them
+    if ((e = hipMemcpyAsync(kout, d_keys, (size_t)n * kw, hipMemcpyDeviceToDevice, st)) != hipSuccess) {
+        rc = fail(e, "adapter: prefill keys");
+        goto done;
+    }
+    if (vb && (e = hipMemcpyAsync(vout, d_vals, (size_t)n * vb, hipMemcpyDeviceToDevice, st)) != hipSuccess) {
+        rc = fail(e, "adapter: prefill values");
+        goto done;
+    }
+    rc = segsort_sort(temp, &tb, d_keys, kout, d_vals, vout, n, S, d_segs, d_end, 0, kw * 8, key_type, vb, 0,
+                      SEGSORT_ALGO_BATCHED, st);
+    if (rc) goto done;
+    if ((e = hipMemcpyAsync(d_keys, kout, (size_t)n * kw, hipMemcpyDeviceToDevice, st)) != hipSuccess) {
+        rc = fail(e, "adapter: copy back keys");
+        goto done;
+    }
+    if (vb && (e = hipMemcpyAsync(d_vals, vout, (size_t)n * vb, hipMemcpyDeviceToDevice, st)) != hipSuccess) {
+        rc = fail(e, "adapter: copy back values");
+        goto done;
+    }
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) rc = fail(e, "adapter: sync");
+done:
+    hipStreamSynchronize(st);
+    if (temp) hipFree(temp);
+    if (vout) hipFree(vout);
+    if (kout) hipFree(kout);
+    if (d_end) hipFree(d_end);
+    return rc;
+}
+
+int segsort_bb_segsort(void* d_keys, void* d_vals, int num_items, const int* d_segs, int num_segments, int key_type,
+                       int value_bytes, void* stream) {
+    return inplace_sort(d_keys, d_vals, num_items, d_segs, num_segments, key_type, value_bytes,
+                        (hipStream_t)stream) == 0
+               ? 1
+               : 0;
+}
+
+int segsort_mgpu_segmented_sort(void* d_keys, void* d_vals, int count, const int* d_segments, int num_segments,
+                                int key_type, int value_bytes, void* stream) {
+    return inplace_sort(d_keys, d_vals, count, d_segments, num_segments, key_type, value_bytes, (hipStream_t)stream);
+}
+
+}  // extern "C"

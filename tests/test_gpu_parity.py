@@ -1,0 +1,196 @@
+"""GPU parity: the HIP library (through its C-ABI) against the CPU oracle, bit-exact.
+
+Small and mid sizes compare every output element with the oracle; the full-size
+workloads compare against the oracle too (it finishes in seconds with OpenMP).
+Every case runs the three algorithms (batched, plain per-class, fix-pass).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ALGOS = ("batched", "plain", "fixpass")
+KEY_TYPES = {np.dtype(np.uint32): 0, np.dtype(np.int32): 1, np.dtype(np.float32): 2,
+             np.dtype(np.uint64): 3, np.dtype(np.int64): 4, np.dtype(np.float64): 5}
+FILL = 0xA5
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a).view(np.uint8).copy()).cuda()
+
+
+def host(t, dtype):
+    return t.cpu().numpy().view(dtype)
+
+
+def gpu_sort(pkg, torch, keys, vals=None, offsets=None, begin=None, end=None, algo="batched", begin_bit=0,
+             end_bit=None, descending=False):
+    """Run segsort_sort through the C-ABI; outputs start filled with FILL bytes so that
+    any write outside the segments shows."""
+    if begin is None:
+        offsets = np.asarray(offsets, dtype=np.int32)
+        begin, end = offsets[:-1], offsets[1:]
+    begin = np.ascontiguousarray(begin, dtype=np.int32)
+    end = np.ascontiguousarray(end, dtype=np.int32)
+    kt = KEY_TYPES[keys.dtype]
+    vb = 0 if vals is None else vals.dtype.itemsize
+    n, s = keys.size, begin.size
+    kin, bd, ed = dev(torch, keys), dev(torch, begin), dev(torch, end)
+    kout = torch.full_like(kin, FILL)
+    vin = dev(torch, vals) if vals is not None else None
+    vout = torch.full_like(vin, FILL) if vals is not None else None
+    tb = [0]
+    D = pkg.DeviceSegmentedRadixSort
+    eb = end_bit if end_bit is not None else keys.dtype.itemsize * 8
+    rc = D._call(None, tb, None, None, None, None, n, s, None, None, begin_bit, eb, None, int(descending), kt, vb,
+                 algo)
+    assert rc == 0, pkg.last_error()
+    temp = torch.empty(max(tb[0], 1), dtype=torch.uint8, device="cuda")
+    tb = [temp.numel()]
+    rc = D._call(temp, tb, kin, kout, vin, vout, n, s, bd, ed, begin_bit, eb, None, int(descending), kt, vb, algo)
+    assert rc == 0, pkg.last_error()
+    torch.cuda.synchronize()
+    ko = host(kout, keys.dtype)
+    vo = host(vout, vals.dtype) if vals is not None else None
+    return ko, vo
+
+
+def oracle_sort(orc, keys, vals=None, offsets=None, begin=None, end=None, begin_bit=0, end_bit=None,
+                descending=False):
+    fill_k = np.full(keys.nbytes, FILL, dtype=np.uint8).view(keys.dtype)
+    fill_v = np.full(vals.nbytes, FILL, dtype=np.uint8).view(vals.dtype) if vals is not None else None
+    return orc.seg_sort(keys, vals, offsets=offsets, begin=begin, end=end, begin_bit=begin_bit, end_bit=end_bit,
+                        descending=descending, out_keys=fill_k, out_vals=fill_v)
+
+
+def check(pkg, torch, orc, keys, vals, offsets=None, begin=None, end=None, algos=ALGOS, **kw):
+    ek, ev = oracle_sort(orc, keys, vals, offsets, begin, end, **kw)
+    for algo in algos:
+        if algo == "fixpass" and (keys.dtype.itemsize == 8 or begin is not None):
+            continue
+        print(f"[case] n={keys.size} algo={algo} keys={keys.dtype} vals={None if vals is None else vals.dtype} {kw}",
+              flush=True)
+        gk, gv = gpu_sort(pkg, torch, keys, vals, offsets, begin, end, algo=algo, **kw)
+        bad = np.flatnonzero(gk.view(np.uint8) != ek.view(np.uint8))
+        assert np.array_equal(gk.view(np.uint8), ek.view(np.uint8)), (algo, bad[:10])
+        if vals is not None:
+            assert np.array_equal(gv.view(np.uint8), ev.view(np.uint8)), (algo, np.flatnonzero(gv != ev)[:10])
+
+
+def test_cub_doc_vectors_gpu(pkg, torch_cuda, orc):
+    import json
+    from pathlib import Path
+    vec = json.loads((Path(__file__).parent / "golden" / "cub_doc_vectors.json").read_text())
+    for name, v in vec.items():
+        keys = np.array(v["keys"], dtype=np.int32)
+        vals = np.array(v["values"], dtype=np.int32) if "values" in v else None
+        desc = "descending" in name
+        for algo in ALGOS:
+            ko, vo = gpu_sort(pkg, torch_cuda, keys, vals, v["offsets"], algo=algo, descending=desc)
+            assert ko.tolist() == v["keys_out"], (name, algo)
+            if vals is not None:
+                assert vo.tolist() == v["values_out"], (name, algo)
+
+
+def test_keys_only_2e20_1024_segments(pkg, torch_cuda, orc):
+    """2^20 u32 keys uniform in [0, 2^31) in 1024 equal segments, keys only."""
+    keys, _, off = pkg.gen.equal_segments(1 << 20, 1024, seed=0x5E650A7 + 0, key_bits=31, pairs=False)
+    check(pkg, torch_cuda, orc, keys, None, off)
+
+
+@pytest.mark.parametrize("k", [0, 4, 8, 12, 16, 20])
+def test_equal_segments_pairs_reduced(pkg, torch_cuda, orc, k):
+    """Equal-segment sweep at 2^20 pairs (segment length 2^(20-k))."""
+    n = 1 << 20
+    keys, vals, off = pkg.gen.equal_segments(n, 1 << k, seed=0x5E650A7 + 1 + k)
+    check(pkg, torch_cuda, orc, keys, vals, off)
+
+
+@pytest.mark.parametrize("k", [0, 2, 6, 10, 14, 18, 20])
+def test_equal_segments_pairs_2e24(pkg, torch_cuda, orc, k):
+    """2^24 u32 pairs (values = index) in 2^k equal segments, against the oracle."""
+    keys, vals, off = pkg.gen.equal_segments(1 << 24, 1 << k, seed=0x5E650A7 + 1)
+    check(pkg, torch_cuda, orc, keys, vals, off, algos=("batched", "plain") if k != 10 else ALGOS)
+
+
+@pytest.mark.parametrize("dup", [False, True])
+def test_ragged_lognormal_2e24(pkg, torch_cuda, orc, dup):
+    """2^24 pairs in 2^16 lognormal segments (5% empty, 5% length 1), uniform and [0,256) keys."""
+    keys, vals, off = pkg.gen.ragged_lognormal(0x5E650A7 + 2, dup_heavy=dup)
+    check(pkg, torch_cuda, orc, keys, vals, off)
+
+
+@pytest.mark.parametrize("key", ["f32", "u64"])
+def test_tiny_segments_reduced(pkg, torch_cuda, orc, key):
+    """2^16 segments of length U[1,32]: f32 N(0,1) with +-0.0 and denormals, or u64."""
+    keys, vals, off = pkg.gen.tiny_segments(0x5E650A7 + 3, key=key, s=1 << 16)
+    check(pkg, torch_cuda, orc, keys, vals, off)
+
+
+@pytest.mark.parametrize("key", ["f32", "u64"])
+def test_tiny_segments_2e20(pkg, torch_cuda, orc, key):
+    """2^20 segments of length U[1,32] (about 17.3M items)."""
+    keys, vals, off = pkg.gen.tiny_segments(0x5E650A7 + 3, key=key)
+    check(pkg, torch_cuda, orc, keys, vals, off, algos=("batched",))
+
+
+def test_key_types_and_value_widths(pkg, torch_cuda, orc):
+    rng = np.random.default_rng(11)
+    lengths = np.concatenate([rng.integers(0, 70, 200), [0, 1, 2, 300, 5000, 20000, 70000]])
+    off = pkg.gen.offsets_from_lengths(lengths)
+    n = int(off[-1])
+    raw = pkg.gen.u64(11, 1, n)
+    for dt in (np.uint32, np.int32, np.float32, np.uint64, np.int64, np.float64):
+        if np.dtype(dt).itemsize == 4:
+            keys = (raw >> np.uint64(32)).astype(np.uint32).view(dt)
+        else:
+            keys = raw.view(dt)
+        if np.dtype(dt).kind == "f":
+            keys = keys.copy()
+            keys[np.isnan(keys)] = 0.5  # NaN order is covered by the twiddle closed form
+        keys = keys.copy()
+        keys[n // 2:] = keys[: n - n // 2]  # duplicates across and within segments
+        for vdt in (None, np.uint32, np.uint64):
+            vals = None if vdt is None else np.arange(n, dtype=vdt)
+            check(pkg, torch_cuda, orc, keys, vals, off)
+
+
+def test_bit_ranges_descending(pkg, torch_cuda, orc):
+    keys, vals, off = pkg.gen.equal_segments(1 << 18, 1 << 6, seed=21)
+    lengths = np.diff(off)
+    for bb, eb in ((0, 32), (4, 12), (8, 32), (0, 1), (31, 32), (3, 3), (0, 16)):
+        for desc in (False, True):
+            check(pkg, torch_cuda, orc, keys, vals, off, begin_bit=bb, end_bit=eb, descending=desc)
+    assert lengths.sum() == keys.size
+
+
+def test_gaps_unordered_and_empty(pkg, torch_cuda, orc):
+    """Non-contiguous, unordered segments with gaps and end <= begin: positions outside
+    the segments are never written."""
+    n = 300000
+    keys = pkg.gen.keys_u32(5, 1, n)
+    vals = np.arange(n, dtype=np.uint32)
+    begin = np.array([250000, 10, 100, 100, 5000, 90000, 200, 299990, 7], dtype=np.int32)
+    end = np.array([299000, 90, 100, 50, 85000, 91000, 201, 300000, 9], dtype=np.int32)
+    check(pkg, torch_cuda, orc, keys, vals, begin=begin, end=end, algos=("batched", "plain"))
+
+
+def test_duplicate_heavy_tiles(pkg, torch_cuda, orc):
+    rng = np.random.default_rng(4)
+    lengths = rng.integers(1, 40000, 300)
+    off = pkg.gen.offsets_from_lengths(lengths)
+    n = int(off[-1])
+    for bits in (1, 4, 8, 20):
+        keys = pkg.gen.keys_u32(bits, 2, n, bits)
+        check(pkg, torch_cuda, orc, keys, np.arange(n, dtype=np.uint32), off)
+    keys = np.full(n, 7, dtype=np.uint32)
+    check(pkg, torch_cuda, orc, keys, np.arange(n, dtype=np.uint32), off)
+
+
+def test_edge_sizes(pkg, torch_cuda, orc):
+    for lengths in ([0], [1], [2], [64], [65], [256], [257], [1024], [1025], [4096], [4097], [16384], [16385],
+                    [8192], [8193], [0, 0, 0], [1, 0, 1], [33, 31, 32]):
+        off = pkg.gen.offsets_from_lengths(np.array(lengths))
+        n = int(off[-1])
+        keys = pkg.gen.keys_u32(len(lengths) + n, 1, n)
+        check(pkg, torch_cuda, orc, keys, np.arange(n, dtype=np.uint32), off)

@@ -1,0 +1,266 @@
+"""Benchmark of the MI355X segmented sort (one GPU).
+
+Default workload: SortPairs of 2^24 uint32 key/value pairs (values = index) in 2^16
+segments whose lengths are lognormal(mu=5, sigma=2), clipped to [0, 2^20], rescaled
+to sum to 2^24, with at least 5% empty and 5% length-1 segments; keys uniform in
+[0, 2^32). A step is one SortPairs call over that whole batch, inputs resident in HBM.
+
+Prints ONE JSON line: the batched form's Gkeys/s as `value`, the plain per-class and
+fix-pass forms beside it, the roofline of the dominant kernel (its algorithmic bytes
+per launch over its average launch time, timed with HIP events on its stream), the
+whole call against the algorithmic-bytes roofline, and the CPU baseline (the oracle's
+per-segment std::stable_sort, OpenMP over the host cores).
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+PKG_DIR = ROOT / "segmented-sorting_amd"
+HBM_PEAK_GBS = 8000.0
+METRIC = "Gkeys/s SortPairs 2^24 u32 ragged segs, % HBM roofline, batched vs plain"
+SEED = 0x5E650A7 + 2
+
+
+def load_pkg():
+    name = "segsort_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, PKG_DIR / "__init__.py",
+                                                  submodule_search_locations=[str(PKG_DIR)])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+WORKLOADS = {
+    "ragged": "2^24 u32 pairs, 2^16 lognormal(5,2) segments, keys uniform u32",
+    "ragged_dup": "2^24 u32 pairs, 2^16 lognormal(5,2) segments, keys uniform in [0,256)",
+    "tiny_f32": "2^20 segments of U[1,32] f32 N(0,1) keys (+-0, denormals), u32 values",
+    "tiny_u64": "2^20 segments of U[1,32] u64 keys, u32 values",
+}
+
+
+def make_workload(pkg, name):
+    g = pkg.gen
+    if name == "ragged":
+        return g.ragged_lognormal(SEED, dup_heavy=False)
+    if name == "ragged_dup":
+        return g.ragged_lognormal(SEED, dup_heavy=True)
+    if name.startswith("equal"):
+        k = int(name.split(":")[1])
+        return g.equal_segments(1 << 24, 1 << k, seed=0x5E650A7 + 1)
+    if name == "tiny_f32":
+        return g.tiny_segments(0x5E650A7 + 3, key="f32")
+    if name == "tiny_u64":
+        return g.tiny_segments(0x5E650A7 + 3, key="u64")
+    raise SystemExit(f"unknown workload {name}")
+
+
+def to_dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a).view(np.uint8).copy()).cuda()
+
+
+class Runner:
+    def __init__(self, pkg, torch, keys, vals, off, algo):
+        self.pkg, self.torch = pkg, torch
+        self.n, self.s = keys.size, off.size - 1
+        self.kt = {np.dtype(np.uint32): 0, np.dtype(np.float32): 2, np.dtype(np.uint64): 3}[keys.dtype]
+        self.vb = vals.dtype.itemsize
+        self.kin, self.vin = to_dev(torch, keys), to_dev(torch, vals)
+        self.kout, self.vout = torch.empty_like(self.kin), torch.empty_like(self.vin)
+        self.off = to_dev(torch, off.astype(np.int32))
+        self.begin = self.off.data_ptr()
+        self.end = self.off.data_ptr() + 4
+        self.algo = algo
+        self.eb = keys.dtype.itemsize * 8
+        tb = [0]
+        rc = pkg.DeviceSegmentedRadixSort._call(None, tb, None, None, None, None, self.n, self.s, None, None, 0,
+                                                self.eb, None, 0, self.kt, self.vb, algo)
+        assert rc == 0, pkg.last_error()
+        self.temp = torch.empty(tb[0], dtype=torch.uint8, device="cuda")
+        self.stream = torch.cuda.current_stream().cuda_stream
+
+    def step(self):
+        tb = [self.temp.numel()]
+        rc = self.pkg.DeviceSegmentedRadixSort._call(self.temp, tb, self.kin, self.kout, self.vin, self.vout, self.n,
+                                                     self.s, self.begin, self.end, 0, self.eb, self.stream, 0,
+                                                     self.kt, self.vb, self.algo)
+        if rc != 0:
+            raise RuntimeError(f"{self.algo}: hipError {rc}: {self.pkg.last_error()}")
+
+    def time(self, steps, warmup):
+        torch = self.torch
+        for _ in range(warmup):
+            self.step()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record()
+        for _ in range(steps):
+            self.step()
+        e1.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return e0.elapsed_time(e1) / steps, wall / steps * 1e3
+
+
+def sorted_within_segments(torch, r, key_dtype):
+    """Cheap output property check on the GPU: keys non-decreasing inside every segment."""
+    if key_dtype != np.uint32:
+        return True
+    k = r.kout.view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+    off = r.off.view(torch.int32).to(torch.int64)
+    head = torch.zeros(r.n, dtype=torch.bool, device="cuda")
+    starts = off[:-1][(off[1:] - off[:-1]) > 0]
+    head[starts] = True
+    ok = (k[1:] >= k[:-1]) | head[1:]
+    return bool(ok.all().item())
+
+
+def cpu_baseline(keys, vals, off, budget_s=12.0):
+    """The oracle's per-segment std::stable_sort on the host cores (OpenMP, all threads),
+    repeated over the full workload until ~budget_s of CPU wall time is spent."""
+    sys.path.insert(0, str(ROOT))
+    from oracle import oracle as orc  # checker / baseline only
+
+    threads = orc.max_threads()
+    reps, total = 0, 0.0
+    while total < budget_s and reps < 20:
+        total += orc.time_seg_sort(keys, vals, off, threads)
+        reps += 1
+    t = total / reps
+    cpu_model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": keys.size / t / 1e9, "unit": "Gkeys/s", "cores": threads, "kind": "port",
+            "sample": f"full workload ({keys.size} pairs), {reps} reps, OpenMP dynamic over segments, "
+                      f"std::stable_sort per segment, {cpu_model}"}
+
+
+def read_traffic(workload, kernel):
+    p = ROOT / "profiles" / "traffic.json"
+    if not p.exists():
+        return None
+    try:
+        d = json.loads(p.read_text())
+        return d.get(workload, {}).get(kernel)
+    except (ValueError, OSError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="ragged", help="ragged | ragged_dup | equal:<k> | tiny_f32 | tiny_u64")
+    ap.add_argument("--algos", default="batched,plain,fixpass")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--only", default=None, help="time one algorithm and print its ms (profiling runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(segments shard with no exchange, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    pkg = load_pkg()
+    pkg.load_library()
+    keys, vals, off = make_workload(pkg, args.workload)
+    n, s = keys.size, off.size - 1
+    item_bytes = keys.dtype.itemsize + vals.dtype.itemsize
+    algo_bytes = n * 2 * item_bytes + 8 * s  # one read + one write of every pair, begin+end offsets
+
+    if args.only:
+        r = Runner(pkg, torch, keys, vals, off, args.only)
+        ms, wall = r.time(args.steps, args.warmup)
+        print(json.dumps({"algo": args.only, "workload": args.workload, "ms": ms, "wall_ms": wall}), flush=True)
+        return
+
+    algos = [a for a in args.algos.split(",") if a]
+    if keys.dtype.itemsize == 8 and "fixpass" in algos:
+        algos.remove("fixpass")
+    res = {}
+    runners = {}
+    for a in algos:
+        r = Runner(pkg, torch, keys, vals, off, a)
+        ms, wall = r.time(args.steps, args.warmup)
+        ok = sorted_within_segments(torch, r, keys.dtype)
+        res[a] = {"ms": ms, "wall_ms_per_step": wall, "gkeys_s": n / (ms * 1e-3) / 1e9,
+                  "hbm_frac": algo_bytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), "sorted": ok}
+        runners[a] = r
+        print(f"[bench] {a}: {ms:.3f} ms/step, {res[a]['gkeys_s']:.1f} Gkeys/s, sorted={ok}", file=sys.stderr,
+              flush=True)
+
+    # per-kernel HIP-event timing of the batched form over the same number of steps
+    rb = runners["batched"]
+    pkg.profile(True)
+    for _ in range(args.steps):
+        rb.step()
+    torch.cuda.synchronize()
+    rep = pkg.profile_report()
+    pkg.profile(False)
+    dom = max(rep.items(), key=lambda kv: kv[1]["ms"])
+    dname, d = dom
+    avg_ms = d["ms"] / d["launches"]
+    bytes_per_launch = d["bytes"] / d["launches"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    kernels = {k: {"launches": v["launches"], "avg_us": v["ms"] / v["launches"] * 1e3,
+                   "share": v["ms"] / sum(x["ms"] for x in rep.values()),
+                   "gb_s": (v["bytes"] / v["launches"]) / (v["ms"] / v["launches"] * 1e-3) / 1e9 if v["ms"] else 0}
+               for k, v in rep.items()}
+
+    ms = res["batched"]["ms"]
+    out = {
+        "metric": METRIC,
+        "value": round(res["batched"]["gkeys_s"], 3),
+        "unit": "Gkeys/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32" if keys.dtype == np.uint32 else str(keys.dtype),
+        "data": "synthetic (seeded splitmix64)",
+        "config": {"workload": WORKLOADS.get(args.workload, args.workload), "num_items": n, "num_segments": s,
+                   "algorithm": "batched", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(args.workload, dname),
+                     "kernel": dname, "avg_launch_us": round(avg_ms * 1e3, 2),
+                     "algorithmic_bytes_per_launch": int(bytes_per_launch)},
+        "end_to_end": {"algorithmic_bytes": algo_bytes,
+                       "achieved_gb_s": round(algo_bytes / (ms * 1e-3) / 1e9, 1),
+                       "frac": round(res["batched"]["hbm_frac"], 4)},
+        "forms": {a: {"ms": round(v["ms"], 4), "wall_ms": round(v["wall_ms_per_step"], 4),
+                      "gkeys_s": round(v["gkeys_s"], 3), "hbm_frac": round(v["hbm_frac"], 4),
+                      "sorted": v["sorted"]} for a, v in res.items()},
+        "batched_over_plain": round(res["plain"]["ms"] / ms, 3) if "plain" in res else None,
+        "kernels": {k: {kk: round(vv, 4) if isinstance(vv, float) else vv for kk, vv in v.items()}
+                    for k, v in kernels.items()},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(keys, vals, off)
+        out["cpu_baseline"]["value"] = round(out["cpu_baseline"]["value"], 4)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

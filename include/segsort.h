@@ -115,17 +115,34 @@ int segsort_perm_scatter_f32(const float* d_grad_out, const int* d_perm,
                              const int* d_begin_offsets, const int* d_end_offsets, int rows,
                              float* d_grad_in, void* stream);
 
-/* Tuning of the batched and plain forms (read by the autotuner).
- * wave_cap: ranges of 2..wave_cap items use the wave tier (2 <= wave_cap <= 64).
+/* Tuning of the batched and plain forms (written by the autotuner).
+ * wave_cap: ranges of 2..wave_cap items use the wave tier (2 <= wave_cap <= 512).
  * tile_enable: bit c enables LDS tile class c (classes 0..3, capacities given
- * by segsort_tile_caps); a disabled class's ranges go to the next enabled one.
+ * by segsort_tile_caps); a disabled class's ranges go to the next enabled one,
+ * and ranges above the largest enabled class go to the multi-workgroup MSD
+ * tier (batched) or the one-workgroup-per-segment LSD (plain).
  * grid_mult: persistent workgroups per CU per resident slot (1..8).
+ * concurrent: 1 runs the tile classes on forked streams (parallel graph
+ * branches), 0 one after another.
  * key_width: 4 or 8; value_bytes: 0, 4 or 8. Returns a hipError_t value. */
-int segsort_set_tuning(int key_width, int value_bytes, int wave_cap, int tile_enable, int grid_mult);
-int segsort_get_tuning(int key_width, int value_bytes, int* wave_cap, int* tile_enable, int* grid_mult);
+int segsort_set_tuning(int key_width, int value_bytes, int wave_cap, int tile_enable, int grid_mult,
+                       int concurrent);
+int segsort_get_tuning(int key_width, int value_bytes, int* wave_cap, int* tile_enable, int* grid_mult,
+                       int* concurrent);
 /* Capacities (items) of the LDS tile classes for a key width / value size;
  * writes up to 4 values and returns how many. */
 int segsort_tile_caps(int key_width, int value_bytes, int* caps);
+
+/* Per-launch timing (no counterpart in the reference, whose drivers time the
+ * whole call with cudaEvents, src/radixseg.cu:121-139). While enabled, every
+ * kernel launch is bracketed by HIP events on its stream and, after each
+ * batched/plain call, the call's per-class element counts are read back (a
+ * host sync), so it is for measurement runs only. Enabling clears the log. */
+int segsort_profile_enable(int on);
+/* Writes "name<TAB>launches<TAB>total_ms<TAB>algorithmic_bytes" lines, one per
+ * kernel name, into buf (NUL-terminated, truncated to buflen) after waiting
+ * for the logged events; returns the full report size including the NUL. */
+size_t segsort_profile_report(char* buf, size_t buflen);
 
 /* Human-readable text of the last error this thread's calls returned. */
 const char* segsort_last_error(void);

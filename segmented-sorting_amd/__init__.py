@@ -32,7 +32,8 @@ ALGOS = {"batched": ALGO_BATCHED, "plain": ALGO_PLAIN, "fixpass": ALGO_FIXPASS}
 EXPORTS = (
     "segsort_sort", "segsort_sort_pairs_u32", "segsort_sort_keys_u32", "segsort_bb_segsort",
     "segsort_mgpu_segmented_sort", "segsort_row_iota", "segsort_perm_scatter_f32", "segsort_set_tuning",
-    "segsort_get_tuning", "segsort_tile_caps", "segsort_last_error", "segsort_version",
+    "segsort_get_tuning", "segsort_tile_caps", "segsort_last_error", "segsort_version", "segsort_profile_enable",
+    "segsort_profile_report",
 )
 
 _lib = None
@@ -67,9 +68,10 @@ def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
     lib.segsort_row_iota.restype = i32
     lib.segsort_perm_scatter_f32.argtypes = [vp, vp, vp, vp, i32, vp, vp]
     lib.segsort_perm_scatter_f32.restype = i32
-    lib.segsort_set_tuning.argtypes = [i32, i32, i32, i32, i32]
+    lib.segsort_set_tuning.argtypes = [i32, i32, i32, i32, i32, i32]
     lib.segsort_set_tuning.restype = i32
-    lib.segsort_get_tuning.argtypes = [i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]
+    lib.segsort_get_tuning.argtypes = [i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32),
+                                       ctypes.POINTER(i32)]
     lib.segsort_get_tuning.restype = i32
     lib.segsort_tile_caps.argtypes = [i32, i32, ctypes.POINTER(i32)]
     lib.segsort_tile_caps.restype = i32
@@ -77,6 +79,10 @@ def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
     lib.segsort_last_error.restype = ctypes.c_char_p
     lib.segsort_version.argtypes = []
     lib.segsort_version.restype = i32
+    lib.segsort_profile_enable.argtypes = [i32]
+    lib.segsort_profile_enable.restype = i32
+    lib.segsort_profile_report.argtypes = [ctypes.c_char_p, sz]
+    lib.segsort_profile_report.restype = sz
     if path is None:
         _lib = lib
     return lib
@@ -231,15 +237,34 @@ def segmented_sort(keys, offsets=None, values=None, *, begin=None, end=None, des
     return keys_out, values_out
 
 
-def set_tuning(key_width, value_bytes, wave_cap, tile_enable, grid_mult):
-    _check(load_library().segsort_set_tuning(key_width, value_bytes, wave_cap, tile_enable, grid_mult), "set_tuning")
+def set_tuning(key_width, value_bytes, wave_cap, tile_enable, grid_mult, concurrent=1):
+    _check(load_library().segsort_set_tuning(key_width, value_bytes, wave_cap, tile_enable, grid_mult, concurrent),
+           "set_tuning")
 
 
 def get_tuning(key_width, value_bytes):
-    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    a, b, c, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     _check(load_library().segsort_get_tuning(key_width, value_bytes, ctypes.byref(a), ctypes.byref(b),
-                                             ctypes.byref(c)), "get_tuning")
-    return {"wave_cap": a.value, "tile_enable": b.value, "grid_mult": c.value}
+                                             ctypes.byref(c), ctypes.byref(d)), "get_tuning")
+    return {"wave_cap": a.value, "tile_enable": b.value, "grid_mult": c.value, "concurrent": d.value}
+
+
+def profile(on: bool) -> None:
+    """Turn per-launch HIP-event timing on (clearing the log) or off."""
+    _check(load_library().segsort_profile_enable(1 if on else 0), "profile_enable")
+
+
+def profile_report() -> dict:
+    """{kernel name: {"launches", "ms", "bytes"}} of the launches logged since enabling."""
+    lib = load_library()
+    need = lib.segsort_profile_report(None, 0)
+    buf = ctypes.create_string_buffer(int(need))
+    lib.segsort_profile_report(buf, need)
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, n, ms, b = line.split("\t")
+        out[name] = {"launches": int(n), "ms": float(ms), "bytes": float(b)}
+    return out
 
 
 def tile_caps(key_width, value_bytes):

@@ -155,9 +155,15 @@ struct LargeItem {
     int src;         // buffer holding the range now
     int hi_bit;      // bits at and above hi_bit are equal within the range
     int chunk_base;  // first chunk id of this range in its level
-    int shift;       // chosen digit position      (written by the scan kernel)
-    int bits;        // chosen digit width         (written by the scan kernel)
-    int mode;        // LMODE_*                    (written by the scan kernel)
+    int shift;       // chosen digit position      (written by the scan)
+    int bits;        // chosen digit width         (written by the scan)
+    int mode;        // LMODE_*                    (written by the scan)
+    int parent;      // index of the parent range in the previous level, -1 at level 0
+    uint32_t hist_rem;  // chunks not yet histogrammed
+    uint32_t scat_rem;  // chunks not yet scattered
+    uint32_t flags;     // LFLAG_*
 };
+
+enum : uint32_t { LFLAG_SCANNED = 1u, LFLAG_SCATTERED = 2u };
 
 }  // namespace ssort

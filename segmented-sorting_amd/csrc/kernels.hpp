@@ -15,6 +15,11 @@ struct Counters {
     uint32_t tile_count[MAX_TILE_CLASSES];
     uint32_t pad0[2];
     unsigned long long level[MAX_LEVELS + 1];  // (items << 32) | chunks, per MSD level
+    unsigned long long tile_items[MAX_TILE_CLASSES];  // elements queued per class
+    unsigned long long level_items[MAX_LEVELS + 1];   // elements queued per MSD level
+    uint32_t claim;                                   // next task of the large tier
+    uint32_t scans_done[MAX_LEVELS + 1];              // ranges scanned, per MSD level
+    uint32_t error;                                   // a bounded wait gave up
 };
 
 // Length caps of the classes of one call: class c holds ranges with
@@ -74,12 +79,18 @@ __device__ __forceinline__ void push_tile(const Sched& sc, int cls, bool act, Ti
         base = (uint32_t)__shfl((int)base, (int)leader, WAVE);
         const uint32_t slot = base + lanes_below(m);
         if (act && cls == c && slot < sc.tq_cap[c]) sc.tq[c][slot] = it;
+        uint32_t items = (act && cls == c) ? (uint32_t)ti_n(it) : 0u;
+#pragma unroll
+        for (int o = 1; o < WAVE; o <<= 1) items += __shfl_xor(items, o, WAVE);
+        if (lane_id() == leader) atomicAdd(&sc.ctr->tile_items[c], (unsigned long long)items);
     }
 }
 
-__device__ __forceinline__ void push_large(const Sched& sc, int level, int begin, int n, int src, int hi_bit) {
+__device__ __forceinline__ void push_large(const Sched& sc, int level, int begin, int n, int src, int hi_bit,
+                                           int parent = -1) {
     const unsigned long long nch = (unsigned long long)((n + sc.chunk - 1) / sc.chunk);
     const unsigned long long old = atomicAdd(&sc.ctr->level[level], (1ull << 32) | nch);
+    atomicAdd(&sc.ctr->level_items[level], (unsigned long long)n);
     LargeItem li;
     li.begin = begin;
     li.n = n;
@@ -89,29 +100,82 @@ __device__ __forceinline__ void push_large(const Sched& sc, int level, int begin
     li.shift = 0;
     li.bits = 0;
     li.mode = 0;
+    li.parent = parent;
+    li.hist_rem = (uint32_t)nch;
+    li.scat_rem = (uint32_t)nch;
+    li.flags = 0;
     if ((old >> 32) < sc.lq_cap && (old & 0xFFFFFFFFull) + nch <= sc.chunk_cap) sc.lq[level][old >> 32] = li;
 }
 
 // Bin every segment by length: length 1 is copied here, the rest go to the
 // queue of their class (batched and plain forms) or to MSD level 0 (large).
+// Each block bins BIN_SPT * BIN_NT segments; slots are taken from LDS counters
+// (one LDS atomic per wave and class) and then from the global counters with
+// one atomic per block and class, so the hot global counters see few atomics.
+constexpr int BIN_NT = 256;
+constexpr int BIN_SPT = 1;
+
 template <typename U, typename V, bool HAS_V>
-__global__ void k_bin(const int* __restrict__ d_begin, const int* __restrict__ d_end, int num_segments,
-                      Bufs b, Sched sc, ClassCaps cc, int end_bit, int large_to_level0) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    int beg = 0, n = 0;
-    if (s < num_segments) {
-        beg = d_begin[s];
-        n = d_end[s] - beg;
-        if (n < 0) n = 0;
+__global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin, const int* __restrict__ d_end,
+                                                int num_segments, Bufs b, Sched sc, ClassCaps cc, int end_bit,
+                                                int large_to_level0) {
+    __shared__ uint32_t lcount[MAX_TILE_CLASSES];
+    __shared__ uint32_t lbase[MAX_TILE_CLASSES];
+    __shared__ unsigned long long litems[MAX_TILE_CLASSES];
+    if (threadIdx.x < MAX_TILE_CLASSES) {
+        lcount[threadIdx.x] = 0;
+        litems[threadIdx.x] = 0;
     }
-    if (n == 1) {
-        buf_ptr<U>(b.k[BUF_OUT])[beg] = ((const U*)b.k[BUF_IN])[beg];
-        if constexpr (HAS_V) buf_ptr<V>(b.v[BUF_OUT])[beg] = ((const V*)b.v[BUF_IN])[beg];
+    __syncthreads();
+    int beg[BIN_SPT], n[BIN_SPT], cls[BIN_SPT];
+    uint32_t slot[BIN_SPT];
+    const int s0 = blockIdx.x * (BIN_NT * BIN_SPT) + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < BIN_SPT; ++j) {
+        const int s = s0 + j * BIN_NT;
+        beg[j] = 0;
+        n[j] = 0;
+        if (s < num_segments) {
+            beg[j] = d_begin[s];
+            n[j] = d_end[s] - beg[j];
+            if (n[j] < 0) n[j] = 0;
+        }
+        if (n[j] == 1) {
+            buf_ptr<U>(b.k[BUF_OUT])[beg[j]] = ((const U*)b.k[BUF_IN])[beg[j]];
+            if constexpr (HAS_V) buf_ptr<V>(b.v[BUF_OUT])[beg[j]] = ((const V*)b.v[BUF_IN])[beg[j]];
+        }
+        cls[j] = n[j] >= 2 ? class_of(cc, n[j]) : -2;
+        slot[j] = 0;
+        for (int c = 0; c < cc.nclass; ++c) {
+            const uint64_t m = __ballot(cls[j] == c);
+            if (m == 0) continue;
+            const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
+            uint32_t items = cls[j] == c ? (uint32_t)n[j] : 0u;
+#pragma unroll
+            for (int o = 1; o < WAVE; o <<= 1) items += __shfl_xor(items, o, WAVE);
+            uint32_t base = 0;
+            if (lane_id() == leader) {
+                base = atomicAdd(&lcount[c], (uint32_t)__popcll(m));
+                atomicAdd(&litems[c], (unsigned long long)items);
+            }
+            base = (uint32_t)__shfl((int)base, (int)leader, WAVE);
+            if (cls[j] == c) slot[j] = base + lanes_below(m);
+        }
+        if (cls[j] == -1 && large_to_level0) push_large(sc, 0, beg[j], n[j], BUF_IN, end_bit);
     }
-    const int cls = n >= 2 ? class_of(cc, n) : -2;
-    push_tile(sc, cls, cls >= 0, make_ti(beg, cls >= 0 ? n : 0, BUF_IN));
-    if (cls == -1) {
-        if (large_to_level0) push_large(sc, 0, beg, n, BUF_IN, end_bit);
+    __syncthreads();
+    if (threadIdx.x < (unsigned)cc.nclass) {
+        const int c = threadIdx.x;
+        lbase[c] = lcount[c] ? atomicAdd(&sc.ctr->tile_count[c], lcount[c]) : 0u;
+        if (litems[c]) atomicAdd(&sc.ctr->tile_items[c], litems[c]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < BIN_SPT; ++j) {
+        if (cls[j] >= 0) {
+            const uint32_t g = lbase[cls[j]] + slot[j];
+            if (g < sc.tq_cap[cls[j]]) sc.tq[cls[j]][g] = make_ti(beg[j], n[j], BUF_IN);
+        }
     }
 }
 
@@ -132,6 +196,7 @@ __global__ void k_bin_large_ids(const int* __restrict__ d_begin, const int* __re
     uint32_t base = 0;
     if (lane_id() == leader) base = (uint32_t)atomicAdd(&sc.ctr->level[MAX_LEVELS], (unsigned long long)__popcll(m));
     base = (uint32_t)__shfl((int)base, (int)leader, WAVE);
+    if (big) atomicAdd(&sc.ctr->level_items[MAX_LEVELS], (unsigned long long)n);
     if (big) {
         LargeItem li;
         li.begin = beg;
@@ -140,6 +205,8 @@ __global__ void k_bin_large_ids(const int* __restrict__ d_begin, const int* __re
         li.hi_bit = 0;
         li.chunk_base = 0;
         li.shift = li.bits = li.mode = 0;
+        li.parent = -1;
+        li.hist_rem = li.scat_rem = li.flags = 0;
         const uint32_t slot = base + lanes_below(m);
         if (slot < sc.lq_cap) sc.lq[MAX_LEVELS][slot] = li;
     }
@@ -147,9 +214,11 @@ __global__ void k_bin_large_ids(const int* __restrict__ d_begin, const int* __re
 
 // ---- tile tiers --------------------------------------------------------------
 
-// Wave tier: each wave sorts one queued range of n <= 64 items.
+// Wave tier: each wave sorts one queued range of n <= 512 items (compare-count
+// up to rank_max items, a register bitonic network above).
 template <typename U, typename V, bool HAS_V>
-__global__ __launch_bounds__(256) void k_wave_tier(Bufs b, Sched sc, int cls, KeyMode km, int persistent) {
+__global__ __launch_bounds__(256) void k_wave_tier(Bufs b, Sched sc, int cls, KeyMode km, int persistent,
+                                                   int rank_max) {
     const int waves_per_block = blockDim.x / WAVE;
     const int wid = blockIdx.x * waves_per_block + (threadIdx.x >> 6);
     const int stride = persistent ? gridDim.x * waves_per_block : 0x7FFFFFFF;
@@ -157,15 +226,17 @@ __global__ __launch_bounds__(256) void k_wave_tier(Bufs b, Sched sc, int cls, Ke
     for (int i = wid; i < count; i += stride) {
         const TileItem it = sc.tq[cls][i];
         const int src = ti_src(it);
-        wave_rank_sort<U, V, HAS_V>((const U*)b.k[src], (const V*)b.v[src], buf_ptr<U>(b.k[BUF_OUT]),
-                                    buf_ptr<V>(b.v[BUF_OUT]), it.begin, ti_n(it), km);
+        wave_sort<U, V, HAS_V>((const U*)b.k[src], (const V*)b.v[src], buf_ptr<U>(b.k[BUF_OUT]),
+                               buf_ptr<V>(b.v[BUF_OUT]), it.begin, ti_n(it), km, rank_max);
         if (!persistent) break;
     }
 }
 
 // LDS tier: each workgroup sorts one queued range of n <= NT*IPT items.
+// At least 4 waves per SIMD (16 per CU) so that LDS and memory latency can be
+// hidden; this caps the tile kernels at 128 VGPRs.
 template <int NT, int IPT, typename U, typename V, bool HAS_V>
-__global__ __launch_bounds__(NT) void k_tile_tier(Bufs b, Sched sc, int cls, KeyMode km, int persistent) {
+__global__ __launch_bounds__(NT, 1024 / NT) void k_tile_tier(Bufs b, Sched sc, int cls, KeyMode km, int persistent) {
     using T = RadixTile<NT, IPT, U, V, HAS_V>;
     __shared__ typename T::Smem sm;
     const int count = tile_count(sc, cls);

@@ -1,13 +1,17 @@
 // C-ABI of libsegsort.so: argument checks, temp-storage layout, tuning table
 // and the launch sequences of the batched, plain and fix-pass forms.
 #include "kernels.hpp"
+#include "profile.hpp"
 #include "../../include/segsort.h"
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
+#include <vector>
 
 using namespace ssort;
 
@@ -27,16 +31,23 @@ int ok() {
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
+int bin_grid(int S) { return (S + BIN_NT * BIN_SPT - 1) / (BIN_NT * BIN_SPT); }
+
+// Ranges up to this length use the wave's compare-count rank (about 4n VALU
+// operations per item); longer ones the register bitonic network.
+constexpr int WAVE_RANK_MAX = 24;
+
 // ---- tuning ------------------------------------------------------------------
 
 struct Tuning {
-    int wave_cap;
-    int tile_enable;
-    int grid_mult;
+    int wave_cap;     // ranges of 2..wave_cap items go to the wave tier
+    int tile_enable;  // bit i: LDS tile class i on; ranges above the largest enabled class use the MSD tier
+    int grid_mult;    // persistent workgroups per resident slot
+    int concurrent;   // tile classes run on forked streams (graph branches) instead of one after another
 };
 
 // index: (key_width == 8) * 3 + value-bytes slot (0, 4, 8 -> 0, 1, 2)
-Tuning g_tuning[6] = {{32, 0xF, 1}, {32, 0xF, 1}, {32, 0xF, 1}, {32, 0xF, 1}, {32, 0xF, 1}, {32, 0xF, 1}};
+Tuning g_tuning[6] = {{256, 0xF, 1, 0}, {256, 0xF, 1, 0}, {256, 0xF, 1, 0}, {256, 0xF, 1, 0}, {256, 0xF, 1, 0}, {256, 0xF, 1, 0}};
 std::mutex g_tuning_mu;
 
 int tuning_slot(int key_width, int value_bytes) {
@@ -50,20 +61,149 @@ Tuning get_tuning(int key_width, int value_bytes) {
     return g_tuning[tuning_slot(key_width, value_bytes)];
 }
 
-int device_cus() {
+// Per-device CU count and per-kernel residency, queried once and cached.
+std::mutex g_occ_mu;
+std::map<std::pair<int, const void*>, int> g_occ;
+
+int current_device() {
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    return dev;
+}
+
+int device_cus() {
+    const int dev = current_device();
+    std::lock_guard<std::mutex> lk(g_occ_mu);
+    auto it = g_occ.find({dev, nullptr});
+    if (it != g_occ.end()) return it->second;
     int cus = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) return 256;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    g_occ[{dev, nullptr}] = cus;
     return cus;
 }
 
 template <typename K>
 int resident_blocks(K kernel, int threads) {
+    const int cus = device_cus();
+    const int dev = current_device();
+    const void* key = (const void*)kernel;
+    {
+        std::lock_guard<std::mutex> lk(g_occ_mu);
+        auto it = g_occ.find({dev, key});
+        if (it != g_occ.end()) return it->second * cus;
+    }
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess || per_cu <= 0)
         per_cu = 1;
-    return per_cu * device_cus();
+    std::lock_guard<std::mutex> lk(g_occ_mu);
+    g_occ[{dev, key}] = per_cu;
+    return per_cu * cus;
+}
+
+// ---- hipGraph capture of the batched sequence ---------------------------------
+
+// Everything a captured sequence depends on; an identical call replays the graph.
+struct GraphKey {
+    const void* p[8];  // temp, keys in/out, values in/out, begin, end, stream
+    int n, s, kw, vb, dev;
+    ssort::KeyMode km;
+    Tuning tu;
+};
+
+struct GraphEntry {
+    GraphKey key;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    uint64_t last_use;
+};
+
+std::mutex g_graph_mu;
+std::vector<GraphEntry> g_graphs;
+uint64_t g_graph_clock = 0;
+int g_graph_mode = -1;  // -1: read SEGSORT_GRAPH once (default on), 0 off, 1 on
+constexpr size_t GRAPH_CACHE = 16;
+
+bool graphs_enabled() {
+    if (g_graph_mode < 0) {
+        const char* e = getenv("SEGSORT_GRAPH");
+        g_graph_mode = (e && e[0] == '0') ? 0 : 1;
+    }
+    return g_graph_mode == 1;
+}
+
+bool same_key(const GraphKey& a, const GraphKey& b) { return memcmp(&a, &b, sizeof(GraphKey)) == 0; }
+
+// Run `enqueue` on st: replayed from a cached graph when possible, captured on a
+// private stream on first use, launched directly while profiling or when the
+// caller's stream is itself being captured.
+template <typename F>
+hipError_t run_graph(const GraphKey& key, hipStream_t st, F&& enqueue) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (st) {
+        hipError_t e = hipStreamIsCapturing(st, &cs);
+        if (e != hipSuccess) return e;
+    }
+    if (!graphs_enabled() || Profiler::get().on() || cs != hipStreamCaptureStatusNone) return enqueue(st);
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    for (auto& g : g_graphs) {
+        if (same_key(g.key, key)) {
+            g.last_use = ++g_graph_clock;
+            return hipGraphLaunch(g.exec, st);
+        }
+    }
+    static thread_local hipStream_t cap = nullptr;
+    if (!cap) {
+        hipError_t e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+        if (e != hipSuccess) return e;
+    }
+    hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    hipError_t le = enqueue(cap);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(cap, &graph);
+    if (le != hipSuccess || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return le != hipSuccess ? le : e;
+    }
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return e;
+    }
+    if (g_graphs.size() >= GRAPH_CACHE) {
+        auto victim = std::min_element(g_graphs.begin(), g_graphs.end(),
+                                       [](const GraphEntry& a, const GraphEntry& b) { return a.last_use < b.last_use; });
+        (void)hipGraphExecDestroy(victim->exec);
+        (void)hipGraphDestroy(victim->graph);
+        g_graphs.erase(victim);
+    }
+    g_graphs.push_back({key, graph, exec, ++g_graph_clock});
+    return hipGraphLaunch(exec, st);
+}
+
+// Per-thread, per-device streams and events for forking the tile classes.
+// All streams keep the default priority.
+struct SideStreams {
+    bool ok = false;
+    hipStream_t s[ssort::MAX_TILE_CLASSES];
+    hipEvent_t fork;
+    hipEvent_t join[ssort::MAX_TILE_CLASSES];
+};
+
+SideStreams& side_streams() {
+    static thread_local SideStreams per_dev[16];
+    const int dev = current_device() & 15;
+    SideStreams& ss = per_dev[dev];
+    if (!ss.ok) {
+        bool good = hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming) == hipSuccess;
+        for (int c = 0; c < ssort::MAX_TILE_CLASSES && good; ++c) {
+            good = hipStreamCreateWithFlags(&ss.s[c], hipStreamNonBlocking) == hipSuccess &&
+                   hipEventCreateWithFlags(&ss.join[c], hipEventDisableTiming) == hipSuccess;
+        }
+        ss.ok = good;
+    }
+    return ss;
 }
 
 // ---- per-(key, value) configuration -------------------------------------------
@@ -93,10 +233,10 @@ template <typename C>
 Plan make_plan(const Tuning& tu) {
     Plan p;
     p.nclass = 0;
-    int wave_cap = std::min(64, std::max(2, tu.wave_cap));
+    int wave_cap = std::min(512, std::max(2, tu.wave_cap));
     p.cc.cap[p.nclass] = wave_cap;
     p.kind[p.nclass++] = 0;
-    int enable = tu.tile_enable | 0x8;  // the largest tile is always on
+    const int enable = tu.tile_enable & 0xF;
     for (int i = 0; i < 4; ++i) {
         if (!(enable & (1 << i))) continue;
         if (C::CAP[i] <= wave_cap) continue;
@@ -105,7 +245,7 @@ Plan make_plan(const Tuning& tu) {
     }
     p.cc.nclass = p.nclass;
     for (int c = p.nclass; c < MAX_TILE_CLASSES; ++c) p.cc.cap[c] = 0;
-    p.tile_max = C::TILE_MAX;
+    p.tile_max = p.cc.cap[p.nclass - 1];  // ranges above this go to the MSD (or per-segment LSD) tier
     return p;
 }
 
@@ -128,7 +268,7 @@ Layout make_layout(const Plan& p, int num_items, int num_segments, int levels, b
         return o;
     };
     const size_t N = (size_t)num_items, S = (size_t)num_segments;
-    L.lq_cap = N / (size_t)(C::TILE_MAX + 1) + 1;
+    L.lq_cap = N / (size_t)(p.tile_max + 1) + 1;
     L.chunks = N / (size_t)C::TILE_MAX + L.lq_cap + 1;
     const size_t bucket_bound = plain ? 0 : (size_t)RADIX * L.lq_cap * (size_t)std::max(levels, 1);
     L.ctr = take(sizeof(Counters));
@@ -146,7 +286,7 @@ Layout make_layout(const Plan& p, int num_items, int num_segments, int levels, b
         L.hist = take(L.chunks * C::NPOS * RADIX * sizeof(uint32_t));
         L.off = take(L.chunks * RADIX * sizeof(uint32_t));
     }
-    const bool need_tmp = plain ? (N > (size_t)C::TILE_MAX) : (levels > 0);
+    const bool need_tmp = plain ? (N > (size_t)p.tile_max) : (levels > 0);
     for (int i = 0; i < 2; ++i) {
         L.tk[i] = need_tmp ? take(N * sizeof(U)) : 0;
         L.tv[i] = (need_tmp && HAS_V) ? take(N * sizeof(V)) : 0;
@@ -159,41 +299,43 @@ template <typename U, typename V, bool HAS_V>
 struct Launcher {
     using C = Cfg<U, V, HAS_V>;
 
+    static constexpr double PAIR_BYTES = 2.0 * C::ITEM_BYTES;  // one read + one write per element
+
+    template <int NT, int IPT>
+    static void launch_tile(Bufs b, Sched sc, int cls, KeyMode km, hipStream_t st, int grid, bool persistent,
+                            int grid_mult) {
+        auto k = k_tile_tier<NT, IPT, U, V, HAS_V>;
+        const int g = persistent ? resident_blocks(k, NT) * grid_mult : grid;
+        char name[32];
+        snprintf(name, sizeof(name), "tile_%dx%d", NT, IPT);
+        if (g > 0) SEGSORT_LAUNCH(name, st, cls, PAIR_BYTES, 0.0, (k<<<g, NT, 0, st>>>(b, sc, cls, km, persistent)));
+    }
+
     static void launch_class(int kind, Bufs b, Sched sc, int cls, KeyMode km, hipStream_t st, int grid, bool persistent,
                              int grid_mult) {
-        const int pers = persistent ? 1 : 0;
         switch (kind) {
         case 0: {
             auto k = k_wave_tier<U, V, HAS_V>;
-            int g = persistent ? resident_blocks(k, 256) * grid_mult : (grid + 3) / 4;
-            if (g > 0) k<<<g, 256, 0, st>>>(b, sc, cls, km, pers);
+            const int g = persistent ? resident_blocks(k, 256) * grid_mult : (grid + 3) / 4;
+            if (g > 0)
+                SEGSORT_LAUNCH("wave_tier", st, cls, PAIR_BYTES, 0.0,
+                               (k<<<g, 256, 0, st>>>(b, sc, cls, km, persistent ? 1 : 0, WAVE_RANK_MAX)));
             break;
         }
-        case 1: {
-            auto k = k_tile_tier<C::NT0, C::IPT0, U, V, HAS_V>;
-            int g = persistent ? resident_blocks(k, C::NT0) * grid_mult : grid;
-            if (g > 0) k<<<g, C::NT0, 0, st>>>(b, sc, cls, km, pers);
-            break;
+        case 1: launch_tile<C::NT0, C::IPT0>(b, sc, cls, km, st, grid, persistent, grid_mult); break;
+        case 2: launch_tile<C::NT1, C::IPT1>(b, sc, cls, km, st, grid, persistent, grid_mult); break;
+        case 3: launch_tile<C::NT2, C::IPT2>(b, sc, cls, km, st, grid, persistent, grid_mult); break;
+        case 4: launch_tile<C::NT3, C::IPT3>(b, sc, cls, km, st, grid, persistent, grid_mult); break;
         }
-        case 2: {
-            auto k = k_tile_tier<C::NT1, C::IPT1, U, V, HAS_V>;
-            int g = persistent ? resident_blocks(k, C::NT1) * grid_mult : grid;
-            if (g > 0) k<<<g, C::NT1, 0, st>>>(b, sc, cls, km, pers);
-            break;
-        }
-        case 3: {
-            auto k = k_tile_tier<C::NT2, C::IPT2, U, V, HAS_V>;
-            int g = persistent ? resident_blocks(k, C::NT2) * grid_mult : grid;
-            if (g > 0) k<<<g, C::NT2, 0, st>>>(b, sc, cls, km, pers);
-            break;
-        }
-        case 4: {
-            auto k = k_tile_tier<C::NT3, C::IPT3, U, V, HAS_V>;
-            int g = persistent ? resident_blocks(k, C::NT3) * grid_mult : grid;
-            if (g > 0) k<<<g, C::NT3, 0, st>>>(b, sc, cls, km, pers);
-            break;
-        }
-        }
+    }
+
+    // In profiling mode: read the call's element counts back and attribute bytes.
+    static void profile_attribute(size_t from, const Sched& sc, int nclass, hipStream_t st) {
+        if (!Profiler::get().on()) return;
+        Counters h;
+        if (hipMemcpyAsync(&h, sc.ctr, sizeof(Counters), hipMemcpyDeviceToHost, st) != hipSuccess) return;
+        if (hipStreamSynchronize(st) != hipSuccess) return;
+        Profiler::get().attribute(from, h.tile_items, nclass, h.level_items, MAX_LEVELS + 1);
     }
 
     static Sched make_sched(char* base, const Layout& L, const Plan& p, bool plain) {
@@ -229,16 +371,56 @@ struct Launcher {
         return b;
     }
 
-    static int levels_for(const KeyMode& km, int num_items) {
-        if (num_items <= C::TILE_MAX) return 0;
+    static int levels_for(const KeyMode& km, int num_items, const Plan& p) {
+        if (num_items <= p.tile_max) return 0;
         return (km.end_bit - km.begin_bit + RADIX_BITS - 1) / RADIX_BITS;
+    }
+
+    // Launch the batched sequence (bin, MSD levels, tile tiers) on st.
+    static hipError_t enqueue_batched(char* base, const Layout& L, const Plan& p, const Tuning& tu, int levels,
+                                      const void* ki, void* ko, const void* vi, void* vo, int S, const int* db,
+                                      const int* de, KeyMode km, hipStream_t st) {
+        const Sched sc = make_sched(base, L, p, false);
+        const Bufs b = make_bufs(base, L, ki, ko, vi, vo);
+        const size_t mark = Profiler::get().mark();
+        SEGSORT_LAUNCH("init", st, -1, 0.0, 0.0, (k_init<<<1, 64, 0, st>>>(sc.ctr)));
+        SEGSORT_LAUNCH("bin", st, -1, 0.0, 8.0 * S,
+                       (k_bin<U, V, HAS_V><<<bin_grid(S), BIN_NT, 0, st>>>(db, de, S, b, sc, p.cc, km.end_bit, 1)));
+        for (int l = 0; l < levels; ++l) {
+            auto kh = k_large_hist<C::NT3, C::IPT3, U, V, HAS_V>;
+            auto ks = k_large_scatter<C::NT3, C::IPT3, U, V, HAS_V>;
+            const int gh = resident_blocks(kh, C::NT3) * 2;
+            const int gs = resident_blocks(ks, C::NT3) * tu.grid_mult;
+            SEGSORT_LAUNCH("large_hist", st, 100 + l, (double)sizeof(U), 0.0, (kh<<<gh, C::NT3, 0, st>>>(b, sc, l, km)));
+            SEGSORT_LAUNCH("large_scan", st, -1, 0.0, 0.0,
+                           (k_large_scan<<<device_cus() * 4, RADIX, 0, st>>>(sc, l, km, p.cc, p.tile_max)));
+            SEGSORT_LAUNCH("large_scatter", st, 100 + l, PAIR_BYTES, 0.0, (ks<<<gs, C::NT3, 0, st>>>(b, sc, l, km)));
+        }
+        if (tu.concurrent && p.nclass > 1) {
+            // fork: every tile class on its own stream, joined back into st
+            SideStreams& ss = side_streams();
+            if (!ss.ok) return hipErrorOutOfMemory;
+            hipError_t e = hipEventRecord(ss.fork, st);
+            if (e != hipSuccess) return e;
+            for (int c = 0; c < p.nclass; ++c) {
+                if ((e = hipStreamWaitEvent(ss.s[c], ss.fork, 0)) != hipSuccess) return e;
+                launch_class(p.kind[c], b, sc, c, km, ss.s[c], 0, true, tu.grid_mult);
+                if ((e = hipEventRecord(ss.join[c], ss.s[c])) != hipSuccess) return e;
+            }
+            for (int c = 0; c < p.nclass; ++c)
+                if ((e = hipStreamWaitEvent(st, ss.join[c], 0)) != hipSuccess) return e;
+        } else {
+            for (int c = 0; c < p.nclass; ++c) launch_class(p.kind[c], b, sc, c, km, st, 0, true, tu.grid_mult);
+        }
+        profile_attribute(mark, sc, p.nclass, st);
+        return hipGetLastError();
     }
 
     static int batched(void* d_temp, size_t* temp_bytes, const void* ki, void* ko, const void* vi, void* vo, int N,
                        int S, const int* db, const int* de, KeyMode km, hipStream_t st) {
         const Tuning tu = get_tuning(sizeof(U), HAS_V ? sizeof(V) : 0);
         const Plan p = make_plan<C>(tu);
-        const int levels = levels_for(km, N);
+        const int levels = levels_for(km, N, p);
         const Layout L = make_layout<C, U, V, HAS_V>(p, N, S, levels, false);
         if (!d_temp) {
             *temp_bytes = L.total;
@@ -247,21 +429,20 @@ struct Launcher {
         if (*temp_bytes < L.total) return fail(hipErrorInvalidValue, "segsort_sort: temp storage too small");
         if (N == 0 || S == 0) return ok();
         char* base = (char*)((((uintptr_t)d_temp) + 255) & ~uintptr_t(255));
-        const Sched sc = make_sched(base, L, p, false);
-        const Bufs b = make_bufs(base, L, ki, ko, vi, vo);
-        k_init<<<1, 64, 0, st>>>(sc.ctr);
-        k_bin<U, V, HAS_V><<<(S + 255) / 256, 256, 0, st>>>(db, de, S, b, sc, p.cc, km.end_bit, 1);
-        for (int l = 0; l < levels; ++l) {
-            auto kh = k_large_hist<C::NT3, C::IPT3, U, V, HAS_V>;
-            auto ks = k_large_scatter<C::NT3, C::IPT3, U, V, HAS_V>;
-            const int gh = resident_blocks(kh, C::NT3) * 2;
-            const int gs = resident_blocks(ks, C::NT3) * tu.grid_mult;
-            kh<<<gh, C::NT3, 0, st>>>(b, sc, l, km);
-            k_large_scan<<<device_cus() * 4, RADIX, 0, st>>>(sc, l, km, p.cc, C::TILE_MAX);
-            ks<<<gs, C::NT3, 0, st>>>(b, sc, l, km);
-        }
-        for (int c = 0; c < p.nclass; ++c) launch_class(p.kind[c], b, sc, c, km, st, 0, true, tu.grid_mult);
-        hipError_t e = hipGetLastError();
+        GraphKey key;
+        memset(&key, 0, sizeof(key));
+        const void* ptrs[] = {d_temp, ki, ko, vi, vo, db, de, (const void*)st};
+        for (int i = 0; i < 8; ++i) key.p[i] = ptrs[i];
+        key.n = N;
+        key.s = S;
+        key.km = km;
+        key.kw = (int)sizeof(U);
+        key.vb = HAS_V ? (int)sizeof(V) : 0;
+        key.tu = tu;
+        key.dev = current_device();
+        hipError_t e = run_graph(key, st, [&](hipStream_t cs) {
+            return enqueue_batched(base, L, p, tu, levels, ki, ko, vi, vo, S, db, de, km, cs);
+        });
         if (e != hipSuccess) return fail(e, "segsort_sort(batched): launch");
         return ok();
     }
@@ -280,9 +461,12 @@ struct Launcher {
         char* base = (char*)((((uintptr_t)d_temp) + 255) & ~uintptr_t(255));
         const Sched sc = make_sched(base, L, p, true);
         const Bufs b = make_bufs(base, L, ki, ko, vi, vo);
-        k_init<<<1, 64, 0, st>>>(sc.ctr);
-        k_bin<U, V, HAS_V><<<(S + 255) / 256, 256, 0, st>>>(db, de, S, b, sc, p.cc, km.end_bit, 0);
-        k_bin_large_ids<<<(S + 255) / 256, 256, 0, st>>>(db, de, S, sc, p.cc);
+        const size_t mark = Profiler::get().mark();
+        SEGSORT_LAUNCH("init", st, -1, 0.0, 0.0, (k_init<<<1, 64, 0, st>>>(sc.ctr)));
+        SEGSORT_LAUNCH("bin", st, -1, 0.0, 8.0 * S,
+                       (k_bin<U, V, HAS_V><<<bin_grid(S), BIN_NT, 0, st>>>(db, de, S, b, sc, p.cc, km.end_bit, 0)));
+        SEGSORT_LAUNCH("bin_large", st, -1, 0.0, 8.0 * S,
+                       (k_bin_large_ids<<<(S + 255) / 256, 256, 0, st>>>(db, de, S, sc, p.cc)));
         Counters h;
         hipError_t e = hipMemcpyAsync(&h, sc.ctr, sizeof(Counters), hipMemcpyDeviceToHost, st);
         if (e != hipSuccess) return fail(e, "segsort_sort(plain): count read-back");
@@ -293,8 +477,10 @@ struct Launcher {
         const int nlarge = (int)h.level[MAX_LEVELS];
         if (nlarge > 0) {
             if (km.end_bit > km.begin_bit)
-                k_seg_lsd<C::NT3, C::IPT3, U, V, HAS_V><<<nlarge, C::NT3, 0, st>>>(b, sc, km);
+                SEGSORT_LAUNCH("seg_lsd", st, 100 + MAX_LEVELS, PAIR_BYTES, 0.0,
+                               (k_seg_lsd<C::NT3, C::IPT3, U, V, HAS_V><<<nlarge, C::NT3, 0, st>>>(b, sc, km)));
         }
+        profile_attribute(mark, sc, p.nclass, st);
         e = hipGetLastError();
         if (e != hipSuccess) return fail(e, "segsort_sort(plain): launch");
         return ok();
@@ -335,19 +521,25 @@ int fixpass(void* d_temp, size_t* temp_bytes, const void* ki, void* ko, const vo
     int segbits = 0;
     while (segbits < 31 && (1ll << segbits) < (long long)S) ++segbits;  // bit length of S-1
     const int cus = device_cus();
-    k_init<<<1, 64, 0, st>>>(ctr);
-    k_fix_max<U><<<cus * 4, 256, 0, st>>>((const U*)ki, db, de, S, km, dmax);
-    k_fix_composite<U><<<cus * 8, 256, 0, st>>>((const U*)ki, db, de, S, km, dmax, comp[0], idx[0]);
+    SEGSORT_LAUNCH("init", st, -1, 0.0, 0.0, (k_init<<<1, 64, 0, st>>>(ctr)));
+    SEGSORT_LAUNCH("fix_max", st, -1, 0.0, (double)N * sizeof(U),
+                   (k_fix_max<U><<<cus * 4, 256, 0, st>>>((const U*)ki, db, de, S, km, dmax)));
+    SEGSORT_LAUNCH("fix_composite", st, -1, 0.0, (double)N * (sizeof(U) + 12),
+                   (k_fix_composite<U><<<cus * 8, 256, 0, st>>>((const U*)ki, db, de, S, km, dmax, comp[0], idx[0])));
     const int kb = km.end_bit - km.begin_bit;
     const int max_passes = (kb + segbits + RADIX_BITS - 1) / RADIX_BITS;
     auto kd = k_fix_downsweep<FIX_NT, FIX_IPT>;
     const int gd = resident_blocks(kd, FIX_NT);
     for (int ps = 0; ps < max_passes; ++ps) {
         const int s = ps & 1;
-        k_fix_upsweep<256><<<cus * 4, 256, 0, st>>>(comp[s], db, de, S, dmax, segbits, ps, FIX_TILE, (int)nch, hist);
-        k_fix_spine<1024><<<1, 1024, 0, st>>>(db, de, S, dmax, segbits, ps, FIX_TILE, (int)nch, hist);
-        kd<<<gd, FIX_NT, 0, st>>>(comp[s], idx[s], comp[s ^ 1], idx[s ^ 1], db, de, S, dmax, segbits, ps, (int)nch,
-                                 hist);
+        SEGSORT_LAUNCH("fix_upsweep", st, -1, 0.0, (double)N * 8,
+                       (k_fix_upsweep<256><<<cus * 4, 256, 0, st>>>(comp[s], db, de, S, dmax, segbits, ps, FIX_TILE,
+                                                                    (int)nch, hist)));
+        SEGSORT_LAUNCH("fix_spine", st, -1, 0.0, 0.0,
+                       (k_fix_spine<1024><<<1, 1024, 0, st>>>(db, de, S, dmax, segbits, ps, FIX_TILE, (int)nch, hist)));
+        SEGSORT_LAUNCH("fix_downsweep", st, -1, 0.0, (double)N * 24,
+                       (kd<<<gd, FIX_NT, 0, st>>>(comp[s], idx[s], comp[s ^ 1], idx[s ^ 1], db, de, S, dmax, segbits,
+                                                 ps, (int)nch, hist)));
     }
     Bufs b;
     memset(&b, 0, sizeof(b));
@@ -355,7 +547,8 @@ int fixpass(void* d_temp, size_t* temp_bytes, const void* ki, void* ko, const vo
     b.k[BUF_OUT] = ko;
     b.v[BUF_IN] = vi;
     b.v[BUF_OUT] = vo;
-    k_fix_gather<U, V, HAS_V><<<cus * 8, 256, 0, st>>>(b, idx[0], idx[1], db, de, S, dmax, segbits);
+    SEGSORT_LAUNCH("fix_gather", st, -1, 0.0, (double)N * (4 + 2 * (sizeof(U) + (HAS_V ? sizeof(V) : 0))),
+                   (k_fix_gather<U, V, HAS_V><<<cus * 8, 256, 0, st>>>(b, idx[0], idx[1], db, de, S, dmax, segbits)));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(e, "segsort_sort(fixpass): launch");
     return ok();
@@ -381,6 +574,21 @@ int dispatch_algo(int algo, void* d_temp, size_t* temp_bytes, const void* ki, vo
 extern "C" {
 
 int segsort_version(void) { return 100; }
+
+int segsort_profile_enable(int on) {
+    Profiler::get().enable(on != 0);
+    return ok();
+}
+
+size_t segsort_profile_report(char* buf, size_t buflen) {
+    const std::string r = Profiler::get().report();
+    if (buf && buflen) {
+        const size_t n = std::min(buflen - 1, r.size());
+        memcpy(buf, r.data(), n);
+        buf[n] = 0;
+    }
+    return r.size() + 1;
+}
 
 const char* segsort_last_error(void) { return g_last_error.c_str(); }
 
@@ -453,23 +661,25 @@ int segsort_perm_scatter_f32(const float* d_grad_out, const int* d_perm, const i
     return e == hipSuccess ? ok() : fail(e, "segsort_perm_scatter_f32");
 }
 
-int segsort_set_tuning(int key_width, int value_bytes, int wave_cap, int tile_enable, int grid_mult) {
+int segsort_set_tuning(int key_width, int value_bytes, int wave_cap, int tile_enable, int grid_mult, int concurrent) {
     const int s = tuning_slot(key_width, value_bytes);
-    if (s < 0 || wave_cap < 2 || wave_cap > 64 || tile_enable < 0 || tile_enable > 0xF || grid_mult < 1 ||
-        grid_mult > 8)
+    if (s < 0 || wave_cap < 2 || wave_cap > 512 || tile_enable < 0 || tile_enable > 0xF || grid_mult < 1 ||
+        grid_mult > 8 || concurrent < 0 || concurrent > 1)
         return fail(hipErrorInvalidValue, "segsort_set_tuning: out of range");
     std::lock_guard<std::mutex> lk(g_tuning_mu);
-    g_tuning[s] = {wave_cap, tile_enable, grid_mult};
+    g_tuning[s] = {wave_cap, tile_enable, grid_mult, concurrent};
     return ok();
 }
 
-int segsort_get_tuning(int key_width, int value_bytes, int* wave_cap, int* tile_enable, int* grid_mult) {
+int segsort_get_tuning(int key_width, int value_bytes, int* wave_cap, int* tile_enable, int* grid_mult,
+                       int* concurrent) {
     const int s = tuning_slot(key_width, value_bytes);
     if (s < 0) return fail(hipErrorInvalidValue, "segsort_get_tuning: bad slot");
     std::lock_guard<std::mutex> lk(g_tuning_mu);
     if (wave_cap) *wave_cap = g_tuning[s].wave_cap;
     if (tile_enable) *tile_enable = g_tuning[s].tile_enable;
     if (grid_mult) *grid_mult = g_tuning[s].grid_mult;
+    if (concurrent) *concurrent = g_tuning[s].concurrent;
     return ok();
 }
 

@@ -1,29 +1,39 @@
-// Workgroup-level building blocks: stable digit ranking with 64-bit ballots,
-// the LDS-resident LSD radix sort of one tile, the stable partition of one
-// chunk by one digit (used by the large-range tier, the per-segment global
-// LSD of the plain form and the fix-pass LSD), and the wave sort of tiny ranges.
+// Workgroup- and wave-level building blocks:
+//  * RadixTile: stable LSD radix sort of one range in LDS (digit ranking with
+//    64-bit ballots), the stable one-digit partition of one chunk (used by the
+//    multi-workgroup MSD tier, the plain form's per-segment LSD and the
+//    fix-pass LSD) and chunk histograms;
+//  * wave sorts of short ranges: a bitonic network over packed sort words held
+//    in registers, and a compare-count rank for the shortest ranges.
 #pragma once
 
 #include "common.hpp"
 
 namespace ssort {
 
-// NT threads hold IPT items each, in "wave-striped" order: item j of lane l of
-// wave w is tile position w*64*IPT + j*64 + l. This order is the input order,
-// so ranking items in (w, j, l) order is stable.
+// NT threads hold IPT items each in "wave-striped" order: item j of lane l of
+// wave w is tile position w*64*IPT + j*64 + l, which is the input order, so
+// ranking items in (w, j, l) order is stable. Only the first
+// wact = ceil(n / (64*IPT)) waves hold items; the others skip the per-item
+// work, so the cost follows n with a granularity of 64*IPT items.
 template <int NT, int IPT, typename U, typename V, bool HAS_V>
 struct RadixTile {
     static constexpr int NW = NT / WAVE;
     static constexpr int TILE = NT * IPT;
+    static constexpr int WITEMS = WAVE * IPT;
     static constexpr int NDT = NT < RADIX ? NT : RADIX;  // threads in the digit scan
     static constexpr int DPT = RADIX / NDT;               // digits per scanning thread
     static constexpr int NDW = (NDT + WAVE - 1) / WAVE;   // waves in the digit scan
     static_assert(NT % WAVE == 0, "whole waves");
     static_assert(RADIX % NDT == 0, "digit split");
+    static_assert(TILE <= 65536, "16-bit source indices");
 
     struct Smem {
         U keys[TILE];
-        V vals[HAS_V ? TILE : 1];
+        union {
+            V vals[HAS_V ? TILE : 1];                       // partition(): values ride along
+            uint16_t idx[HAS_V ? 2 : 1][HAS_V ? TILE : 1];  // sort(): source index per item, ping-pong
+        };
         uint32_t wcnt[NW * RADIX];  // per-wave digit counts, then scatter bases
         uint32_t dstart[RADIX];     // tile-exclusive start of each digit
         uint32_t goff[RADIX];       // global destination of each digit (partition)
@@ -32,9 +42,51 @@ struct RadixTile {
         U red_and[NW];
     };
 
-    __device__ static __forceinline__ int pos(int j) {
-        const int t = threadIdx.x;
-        return (t >> 6) * (WAVE * IPT) + j * WAVE + (t & 63);
+    __device__ static __forceinline__ int wave() { return threadIdx.x >> 6; }
+    __device__ static __forceinline__ int pos(int j) { return wave() * WITEMS + j * WAVE + (threadIdx.x & 63); }
+    __device__ static __forceinline__ int active_waves(int n) { return (n + WITEMS - 1) / WITEMS; }
+
+    // Lanes of this wave whose item has the same NBITS-bit digit d as the
+    // calling lane: for every bit, OR in (ballot of the bit) XOR (own bit
+    // replicated), then invert. v_bitop3 does the OR-XOR in one instruction
+    // per 32-lane half (truth table 0xF6 = src0 | (src1 ^ src2)).
+    template <int NBITS>
+    __device__ static __forceinline__ uint64_t match_digit(uint32_t d) {
+        uint32_t acc_lo = 0, acc_hi = 0;
+#pragma unroll
+        for (int b = 0; b < NBITS; ++b) {
+            const uint32_t t = (uint32_t)__builtin_amdgcn_sbfe((int)d, b, 1);
+            const uint64_t bal = __ballot(t != 0);
+            acc_lo = __builtin_amdgcn_bitop3_b32(acc_lo, (uint32_t)bal, t, 0xF6);
+            acc_hi = __builtin_amdgcn_bitop3_b32(acc_hi, (uint32_t)(bal >> 32), t, 0xF6);
+        }
+        return ~((uint64_t(acc_hi) << 32) | uint64_t(acc_lo));
+    }
+
+    __device__ static __forceinline__ uint64_t match_digit_rt(uint32_t d, int bits) {
+        uint32_t acc_lo = 0, acc_hi = 0;
+        for (int b = 0; b < bits; ++b) {
+            const uint32_t t = (uint32_t)__builtin_amdgcn_sbfe((int)d, b, 1);
+            const uint64_t bal = __ballot(t != 0);
+            acc_lo = __builtin_amdgcn_bitop3_b32(acc_lo, (uint32_t)bal, t, 0xF6);
+            acc_hi = __builtin_amdgcn_bitop3_b32(acc_hi, (uint32_t)(bal >> 32), t, 0xF6);
+        }
+        return ~((uint64_t(acc_hi) << 32) | uint64_t(acc_lo));
+    }
+
+    // Per-wave counting of one item per lane: returns the running count of the
+    // digit in this wave plus the lanes below with the same digit. The lowest
+    // lane of each digit group advances the wave's counter with one LDS atomic
+    // (returning the old count) and the group reads it with a cross-lane
+    // permute, so successive items do not wait on each other's update.
+    __device__ static __forceinline__ uint32_t count_one(uint32_t* wc, uint32_t d, bool valid, uint64_t peers) {
+        peers &= __ballot(valid);
+        const uint32_t below = lanes_below(peers);
+        const int leader = __ffsll((long long)peers) - 1;
+        uint32_t old = 0;
+        if (valid && below == 0) old = atomicAdd(&wc[d], (uint32_t)__popcll(peers));
+        const uint32_t base = (uint32_t)__builtin_amdgcn_ds_bpermute(leader << 2, (int)old);
+        return base + below;
     }
 
     // Stable rank of every valid item (position < n) by digit
@@ -43,29 +95,27 @@ struct RadixTile {
     __device__ static void rank_digits(Smem& sm, const U (&k)[IPT], int n, int shift, int bits,
                                        uint32_t (&dst)[IPT]) {
         const int t = threadIdx.x;
-        const int w = t >> 6;
+        const int w = wave();
         const uint32_t l = t & 63;
+        const int wact = active_waves(n);
         uint32_t* wc = sm.wcnt + w * RADIX;
-#pragma unroll
-        for (int q = 0; q < RADIX / WAVE; ++q) wc[q * WAVE + l] = 0;
         const U dmask = U((U(1) << bits) - 1);
+        if (w < wact) {
 #pragma unroll
-        for (int j = 0; j < IPT; ++j) {
-            const bool valid = pos(j) < n;
-            const uint32_t d = (uint32_t)((k[j] >> shift) & dmask);
-            uint64_t peers = __ballot(valid);
+            for (int q = 0; q < RADIX / WAVE; ++q) wc[q * WAVE + l] = 0;
+            if (bits == RADIX_BITS) {
 #pragma unroll
-            for (int b = 0; b < RADIX_BITS; ++b) {
-                if (b < bits) {
-                    const bool bit = (d >> b) & 1u;
-                    const uint64_t bb = __ballot(bit);
-                    peers &= bit ? bb : ~bb;
+                for (int j = 0; j < IPT; ++j) {
+                    const uint32_t d = (uint32_t)((k[j] >> shift) & U(RADIX - 1));
+                    dst[j] = count_one(wc, d, pos(j) < n, match_digit<RADIX_BITS>(d));
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < IPT; ++j) {
+                    const uint32_t d = (uint32_t)((k[j] >> shift) & dmask);
+                    dst[j] = count_one(wc, d, pos(j) < n, match_digit_rt(d, bits));
                 }
             }
-            const uint32_t below = lanes_below(peers);
-            const uint32_t base = wc[d];
-            dst[j] = base + below;
-            if (valid && below == 0) wc[d] = base + (uint32_t)__popcll(peers);
         }
         __syncthreads();
         uint32_t tot[DPT];
@@ -75,7 +125,7 @@ struct RadixTile {
             for (int i = 0; i < DPT; ++i) {
                 const int d = t * DPT + i;
                 uint32_t s = 0;
-                for (int ww = 0; ww < NW; ++ww) {
+                for (int ww = 0; ww < wact; ++ww) {
                     const uint32_t c = sm.wcnt[ww * RADIX + d];
                     sm.wcnt[ww * RADIX + d] = s;
                     s += c;
@@ -95,45 +145,45 @@ struct RadixTile {
                 const int d = t * DPT + i;
                 const uint32_t base = excl + tot[i];
                 sm.dstart[d] = base;
-                for (int ww = 0; ww < NW; ++ww) sm.wcnt[ww * RADIX + d] += base;
+                for (int ww = 0; ww < wact; ++ww) sm.wcnt[ww * RADIX + d] += base;
             }
         }
         __syncthreads();
+        if (w < wact) {
 #pragma unroll
-        for (int j = 0; j < IPT; ++j) {
-            const uint32_t d = (uint32_t)((k[j] >> shift) & dmask);
-            dst[j] += wc[d];
+            for (int j = 0; j < IPT; ++j) {
+                const uint32_t d = (uint32_t)((k[j] >> shift) & dmask);
+                dst[j] += wc[d];
+            }
         }
     }
 
-    // Load n <= TILE items from src + begin (twiddled keys) into registers.
-    __device__ static __forceinline__ void load(const U* ks, const V* vs, int64_t begin, int n, const KeyMode& km,
-                                                U (&k)[IPT], V (&v)[IPT]) {
+    // Twiddled keys of the wave's items; slots at or past n read the last
+    // element (always in bounds) and are then marked with the all-ones key.
+    __device__ static __forceinline__ void load_keys(const U* ks, int64_t begin, int n, const KeyMode& km,
+                                                     U (&k)[IPT]) {
 #pragma unroll
         for (int j = 0; j < IPT; ++j) {
             const int p = pos(j);
-            if (p < n) {
-                k[j] = twiddle_in<U>(ks[begin + p], km.kind, km.desc);
-                if constexpr (HAS_V) v[j] = vs[begin + p];
-            } else {
-                k[j] = Bits<U>::ALL;
-                if constexpr (HAS_V) v[j] = V(0);
-            }
+            const U raw = ks[begin + (p < n ? p : n - 1)];
+            k[j] = p < n ? twiddle_in<U>(raw, km.kind, km.desc) : Bits<U>::ALL;
         }
     }
 
     // OR and AND of the valid keys over the whole tile (every thread gets them).
     __device__ static void reduce_or_and(Smem& sm, const U (&k)[IPT], int n, U& o, U& a) {
         U vo = 0, va = Bits<U>::ALL;
+        if (wave() < active_waves(n)) {
 #pragma unroll
-        for (int j = 0; j < IPT; ++j)
-            if (pos(j) < n) {
-                vo |= k[j];
-                va &= k[j];
+            for (int j = 0; j < IPT; ++j) {
+                const bool v = pos(j) < n;
+                vo |= v ? k[j] : U(0);
+                va &= v ? k[j] : Bits<U>::ALL;
             }
+        }
         vo = wave_or(vo);
         va = wave_and(va);
-        const int w = threadIdx.x >> 6;
+        const int w = wave();
         if ((threadIdx.x & 63) == 0) {
             sm.red_or[w] = vo;
             sm.red_and[w] = va;
@@ -151,22 +201,26 @@ struct RadixTile {
 
     // Stable sort of n <= TILE items at [begin, begin+n) of (ks, vs) into
     // (ko, vo) at the same offsets. Only digits whose bits vary inside the tile
-    // are ranked, so a low-entropy tile costs fewer passes.
+    // are ranked, so a low-entropy tile costs fewer passes. Values do not move
+    // through the passes: a 16-bit source index does, and each value is read
+    // once from its source at the end.
     __device__ static void sort(Smem& sm, const U* ks, const V* vs, U* ko, V* vo, int64_t begin, int n,
                                 const KeyMode& km) {
+        const bool act = wave() < active_waves(n);
         U k[IPT];
-        V v[IPT];
-        load(ks, vs, begin, n, km, k, v);
+        if (act) load_keys(ks, begin, n, km, k);
         U o, a;
         reduce_or_and(sm, k, n, o, a);
         const U varying = U((o ^ a) & range_mask<U>(km.begin_bit, km.end_bit));
         if (varying == 0) {
+            if (act) {
 #pragma unroll
-            for (int j = 0; j < IPT; ++j) {
-                const int p = pos(j);
-                if (p < n) {
-                    ko[begin + p] = twiddle_out<U>(k[j], km.kind, km.desc);
-                    if constexpr (HAS_V) vo[begin + p] = v[j];
+                for (int j = 0; j < IPT; ++j) {
+                    const int p = pos(j);
+                    if (p < n) {
+                        ko[begin + p] = twiddle_out<U>(k[j], km.kind, km.desc);
+                        if constexpr (HAS_V) vo[begin + p] = vs[begin + p];
+                    }
                 }
             }
             __syncthreads();
@@ -174,34 +228,39 @@ struct RadixTile {
         }
         const int lo = lowest_bit<U>(varying);
         const int hi = bit_length<U>(varying);
+        int cur = 0;  // idx buffer holding the source index of each tile position
         for (int shift = lo; shift < hi; shift += RADIX_BITS) {
             const int bits = (hi - shift) < RADIX_BITS ? (hi - shift) : RADIX_BITS;
             uint32_t dst[IPT];
             rank_digits(sm, k, n, shift, bits, dst);
-#pragma unroll
-            for (int j = 0; j < IPT; ++j)
-                if (pos(j) < n) {
-                    sm.keys[dst[j]] = k[j];
-                    if constexpr (HAS_V) sm.vals[dst[j]] = v[j];
-                }
-            __syncthreads();
-            if (shift + RADIX_BITS < hi) {
+            if (act) {
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
                     const int p = pos(j);
                     if (p < n) {
-                        k[j] = sm.keys[p];
-                        if constexpr (HAS_V) v[j] = sm.vals[p];
+                        sm.keys[dst[j]] = k[j];
+                        if constexpr (HAS_V) sm.idx[cur ^ 1][dst[j]] = shift == lo ? (uint16_t)p : sm.idx[cur][p];
                     }
                 }
             }
-        }
+            cur ^= 1;
+            __syncthreads();
+            if (act && shift + RADIX_BITS < hi) {
 #pragma unroll
-        for (int j = 0; j < IPT; ++j) {
-            const int p = pos(j);
-            if (p < n) {
-                ko[begin + p] = twiddle_out<U>(sm.keys[p], km.kind, km.desc);
-                if constexpr (HAS_V) vo[begin + p] = sm.vals[p];
+                for (int j = 0; j < IPT; ++j) {
+                    const int p = pos(j);
+                    k[j] = sm.keys[p < n ? p : 0];
+                }
+            }
+        }
+        if (act) {
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int p = pos(j);
+                if (p < n) {
+                    ko[begin + p] = twiddle_out<U>(sm.keys[p], km.kind, km.desc);
+                    if constexpr (HAS_V) vo[begin + p] = vs[begin + sm.idx[cur][p]];
+                }
             }
         }
         __syncthreads();
@@ -213,29 +272,43 @@ struct RadixTile {
     // ordered by digit in LDS first, so each digit run is written contiguously.
     __device__ static void partition(Smem& sm, const U* ks, const V* vs, U* kd, V* vd, int64_t begin, int n,
                                      const KeyMode& km, int shift, int bits, const uint32_t* goff_global) {
+        const bool act = wave() < active_waves(n);
         U k[IPT];
         V v[IPT];
-        load(ks, vs, begin, n, km, k, v);
+        if (act) {
+            load_keys(ks, begin, n, km, k);
+            if constexpr (HAS_V) {
+#pragma unroll
+                for (int j = 0; j < IPT; ++j) {
+                    const int p = pos(j);
+                    v[j] = vs[begin + (p < n ? p : n - 1)];
+                }
+            }
+        }
         for (int d = threadIdx.x; d < RADIX; d += NT) sm.goff[d] = goff_global[d];
         uint32_t dst[IPT];
         rank_digits(sm, k, n, shift, bits, dst);  // contains barriers: goff visible after
+        if (act) {
 #pragma unroll
-        for (int j = 0; j < IPT; ++j)
-            if (pos(j) < n) {
-                sm.keys[dst[j]] = k[j];
-                if constexpr (HAS_V) sm.vals[dst[j]] = v[j];
-            }
+            for (int j = 0; j < IPT; ++j)
+                if (pos(j) < n) {
+                    sm.keys[dst[j]] = k[j];
+                    if constexpr (HAS_V) sm.vals[dst[j]] = v[j];
+                }
+        }
         __syncthreads();
         const U dmask = U((U(1) << bits) - 1);
+        if (act) {
 #pragma unroll
-        for (int j = 0; j < IPT; ++j) {
-            const int p = pos(j);
-            if (p < n) {
-                const U key = sm.keys[p];
-                const uint32_t d = (uint32_t)((key >> shift) & dmask);
-                const uint32_t g = sm.goff[d] + ((uint32_t)p - sm.dstart[d]);
-                kd[g] = twiddle_out<U>(key, km.kind, km.desc);
-                if constexpr (HAS_V) vd[g] = sm.vals[p];
+            for (int j = 0; j < IPT; ++j) {
+                const int p = pos(j);
+                if (p < n) {
+                    const U key = sm.keys[p];
+                    const uint32_t d = (uint32_t)((key >> shift) & dmask);
+                    const uint32_t g = sm.goff[d] + ((uint32_t)p - sm.dstart[d]);
+                    kd[g] = twiddle_out<U>(key, km.kind, km.desc);
+                    if constexpr (HAS_V) vd[g] = sm.vals[p];
+                }
             }
         }
         __syncthreads();
@@ -258,6 +331,142 @@ struct RadixTile {
     }
 };
 
+// ---- wave sorts of short ranges ------------------------------------------------
+
+// A sort word: the masked order key with the item's position in the range as
+// tie-break, so that all words differ and any sorting network sorts stably.
+// 32-bit keys pack into one u64; 64-bit keys use a (key, position) pair.
+template <typename U>
+struct SortWord;
+
+template <>
+struct SortWord<uint32_t> {
+    uint64_t w;
+    __device__ static __forceinline__ SortWord make(uint32_t okey, uint32_t p) {
+        return {(uint64_t(okey) << 32) | p};
+    }
+    __device__ static __forceinline__ SortWord inf() { return {~0ull}; }
+    __device__ __forceinline__ bool lt(const SortWord& o) const { return w < o.w; }
+    __device__ __forceinline__ uint32_t pos() const { return (uint32_t)w; }
+    __device__ __forceinline__ uint32_t key() const { return (uint32_t)(w >> 32); }
+    template <typename F>
+    __device__ __forceinline__ SortWord map(F f) const {
+        return {(uint64_t(f((uint32_t)(w >> 32))) << 32) | uint64_t(f((uint32_t)w))};
+    }
+};
+
+template <>
+struct SortWord<uint64_t> {
+    uint64_t k;
+    uint32_t p;
+    __device__ static __forceinline__ SortWord make(uint64_t okey, uint32_t pp) { return {okey, pp}; }
+    __device__ static __forceinline__ SortWord inf() { return {~0ull, ~0u}; }
+    __device__ __forceinline__ bool lt(const SortWord& o) const { return k < o.k || (k == o.k && p < o.p); }
+    __device__ __forceinline__ uint32_t pos() const { return p; }
+    __device__ __forceinline__ uint64_t key() const { return k; }
+    template <typename F>
+    __device__ __forceinline__ SortWord map(F f) const {
+        return {(uint64_t(f((uint32_t)(k >> 32))) << 32) | uint64_t(f((uint32_t)k)), f(p)};
+    }
+};
+
+// Value of lane (lane ^ M) for a compile-time M: DPP quad permutes for 1 and
+// 2, ds_swizzle (bitmask mode) for 4, 8 and 16, ds_bpermute for 32.
+template <int M>
+__device__ __forceinline__ uint32_t xor_lane(uint32_t v) {
+    if constexpr (M == 1) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
+    else if constexpr (M == 2) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);
+    else if constexpr (M < 32) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (M << 10));
+    else return (uint32_t)__builtin_amdgcn_ds_bpermute((int)((lane_id() ^ 32u) << 2), (int)v);
+}
+
+template <typename W>
+__device__ __forceinline__ void compare_exchange(W& a, W& b, bool ascending) {
+    const bool swap = ascending ? b.lt(a) : a.lt(b);
+    const W x = swap ? b : a;
+    const W y = swap ? a : b;
+    a = x;
+    b = y;
+}
+
+// One cross-lane stage of the bitonic network at lane distance M (element
+// distance M*R): the lower partner keeps the smaller word when the run is
+// ascending, the upper partner the larger one.
+template <int R, int M, typename W>
+__device__ __forceinline__ void bitonic_cross(W (&w)[R], int k) {
+    const uint32_t l = lane_id();
+    const bool lower = (l & M) == 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const W o = w[r].map([](uint32_t x) { return xor_lane<M>(x); });
+        const bool asc = (((int)l * R + r) & k) == 0;
+        const bool keep_small = lower == asc;
+        const bool o_smaller = o.lt(w[r]);
+        w[r] = (keep_small == o_smaller) ? o : w[r];
+    }
+}
+
+// Bitonic sort of 64*R words held R per lane; afterwards lane l holds the
+// words of ranks l*R .. l*R+R-1 in order.
+template <int R, typename W>
+__device__ __forceinline__ void bitonic_wave(W (&w)[R]) {
+    const uint32_t l = lane_id();
+#pragma unroll
+    for (int k = 2; k <= WAVE * R; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j < R) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if ((r & j) == 0) {
+                        const bool asc = (((int)l * R + r) & k) == 0;
+                        compare_exchange(w[r], w[r | j], asc);
+                    }
+                }
+            } else {
+                const int m = j / R;
+                if (m == 1) bitonic_cross<R, 1>(w, k);
+                else if (m == 2) bitonic_cross<R, 2>(w, k);
+                else if (m == 4) bitonic_cross<R, 4>(w, k);
+                else if (m == 8) bitonic_cross<R, 8>(w, k);
+                else if (m == 16) bitonic_cross<R, 16>(w, k);
+                else bitonic_cross<R, 32>(w, k);
+            }
+        }
+    }
+}
+
+// One wave sorts a range of n <= 64*R items: sort words are loaded with
+// coalesced reads (slot (lane, r) takes item r*64 + lane), sorted by the
+// network, and written in rank order; keys are rebuilt from the words when
+// the whole key takes part in the order, else read back by position, and
+// values are read by position.
+template <int R, typename U, typename V, bool HAS_V>
+__device__ __forceinline__ void wave_bitonic_sort(const U* ks, const V* vs, U* ko, V* vo, int64_t begin, int n,
+                                                  const KeyMode& km) {
+    using W = SortWord<U>;
+    const uint32_t l = lane_id();
+    const U rm = range_mask<U>(km.begin_bit, km.end_bit);
+    W w[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int p = r * WAVE + (int)l;
+        const U raw = ks[begin + (p < n ? p : n - 1)];
+        w[r] = p < n ? W::make(U(twiddle_in<U>(raw, km.kind, km.desc) & rm), (uint32_t)p) : W::inf();
+    }
+    bitonic_wave<R>(w);
+    const bool full = rm == Bits<U>::ALL;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int q = (int)l * R + r;
+        if (q < n) {
+            const uint32_t src = w[r].pos();
+            ko[begin + q] = full ? twiddle_out<U>(U(w[r].key()), km.kind, km.desc) : ks[begin + src];
+            if constexpr (HAS_V) vo[begin + q] = vs[begin + src];
+        }
+    }
+}
+
 // One wave sorts a range of n <= 64 items stably: the rank of item l is the
 // number of items whose masked order key is smaller, or equal and earlier.
 template <typename U, typename V, bool HAS_V>
@@ -278,6 +487,19 @@ __device__ __forceinline__ void wave_rank_sort(const U* ks, const V* vs, U* ko, 
         ko[begin + rank] = raw;
         if constexpr (HAS_V) vo[begin + rank] = val;
     }
+}
+
+// Sort one range of n <= 512 items with one wave: compare-count for the
+// shortest ranges, else the bitonic network on the smallest power of two
+// of words per lane that holds n.
+template <typename U, typename V, bool HAS_V>
+__device__ __forceinline__ void wave_sort(const U* ks, const V* vs, U* ko, V* vo, int64_t begin, int n,
+                                          const KeyMode& km, int rank_max) {
+    if (n <= rank_max) wave_rank_sort<U, V, HAS_V>(ks, vs, ko, vo, begin, n, km);
+    else if (n <= 64) wave_bitonic_sort<1, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km);
+    else if (n <= 128) wave_bitonic_sort<2, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km);
+    else if (n <= 256) wave_bitonic_sort<4, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km);
+    else wave_bitonic_sort<8, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km);
 }
 
 }  // namespace ssort

@@ -79,10 +79,12 @@ def test_temp_too_small_is_invalid_value(pkg):
 
 def test_tuning_table(pkg):
     before = pkg.get_tuning(4, 4)
-    pkg.set_tuning(4, 4, 16, 0x5, 2)
-    assert pkg.get_tuning(4, 4) == {"wave_cap": 16, "tile_enable": 5, "grid_mult": 2}
+    pkg.set_tuning(4, 4, 16, 0x5, 2, 0)
+    assert pkg.get_tuning(4, 4) == {"wave_cap": 16, "tile_enable": 5, "grid_mult": 2, "concurrent": 0}
     with pytest.raises(pkg.SegsortError):
-        pkg.set_tuning(4, 4, 65, 0xF, 1)
+        pkg.set_tuning(4, 4, 513, 0xF, 1)
+    with pytest.raises(pkg.SegsortError):
+        pkg.set_tuning(4, 4, 32, 0xF, 1, 2)
     pkg.set_tuning(4, 4, **before)
     caps = pkg.tile_caps(4, 4)
     assert caps == sorted(caps) and caps[-1] >= 8192

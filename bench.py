@@ -17,6 +17,7 @@ import argparse
 import importlib.util
 import json
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -47,6 +48,7 @@ WORKLOADS = {
     "ragged_dup": "2^24 u32 pairs, 2^16 lognormal(5,2) segments, keys uniform in [0,256)",
     "tiny_f32": "2^20 segments of U[1,32] f32 N(0,1) keys (+-0, denormals), u32 values",
     "tiny_u64": "2^20 segments of U[1,32] u64 keys, u32 values",
+    "diff": "differentiable pair sort: 4096 rows of U[16,8192] f32 N(0,1), forward + backward",
 }
 
 
@@ -152,6 +154,47 @@ def cpu_baseline(keys, vals, off, budget_s=12.0):
                       f"std::stable_sort per segment, {cpu_model}"}
 
 
+def profiler_name(kernel_name):
+    """The library profiler's name of a kernel, from its demangled name in a PMC trace."""
+    base = kernel_name.split("(")[0].split("<")[0].split("::")[-1]
+    m = re.search(r"k_tile_tier<(\d+), (\d+)", kernel_name)
+    if m:
+        return f"tile_{m.group(1)}x{m.group(2)}"
+    return {"k_wave_tier": "wave_tier", "k_bin": "bin", "k_init": "init", "k_large_hist": "large_hist",
+            "k_large_scan": "large_scan", "k_large_scatter": "large_scatter"}.get(base)
+
+
+def traffic_from_pmc(fetch_csv, write_csv, copy_mib):
+    """HBM bytes per launch of each library kernel from one FETCH_SIZE and one WRITE_SIZE
+    rocprofv3 pass over `bench.py --only batched --copy-calibration MIB`. Each counter
+    is scaled so that the plain device copy of MIB MiB at the start of the run reads
+    and writes exactly MIB MiB; the averages run over every launch of a kernel (the
+    per-launch figure the in-bench profiler divides by)."""
+    import csv
+
+    def per_kernel(path, counter):
+        rows = [r for r in csv.DictReader(open(path)) if r["Counter_Name"] == counter]
+        copy = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]]
+        if len(copy) != 1:
+            raise SystemExit(f"{path}: expected one calibration copy dispatch, found {len(copy)}")
+        scale = copy_mib * 1024.0 / copy[0]  # counter reads KiB
+        acc = {}
+        for r in rows:
+            name = profiler_name(r["Kernel_Name"])
+            if name:
+                t = acc.setdefault(name, [0.0, 0])
+                t[0] += float(r["Counter_Value"]) * scale * 1024.0
+                t[1] += 1
+        return scale, {k: v[0] / v[1] for k, v in acc.items()}
+
+    fs, fetch = per_kernel(fetch_csv, "FETCH_SIZE")
+    ws, write = per_kernel(write_csv, "WRITE_SIZE")
+    return {"fetch_scale": fs, "write_scale": ws,
+            "bytes": {k: int(round(fetch[k] + write.get(k, 0.0))) for k in fetch},
+            "fetch": {k: int(round(v)) for k, v in fetch.items()},
+            "write": {k: int(round(v)) for k, v in write.items()}}
+
+
 def read_traffic(workload, kernel):
     p = ROOT / "profiles" / "traffic.json"
     if not p.exists():
@@ -163,6 +206,34 @@ def read_traffic(workload, kernel):
         return None
 
 
+def bench_diff(pkg, torch, steps, warmup):
+    """Forward + backward of the differentiable pair sort per step (4096 rows)."""
+    x, off = pkg.gen.diff_rows(0x5E650A7 + 4)
+    xd = torch.from_numpy(x).cuda().requires_grad_(True)
+    od = torch.from_numpy(off).cuda()
+    g = torch.from_numpy(pkg.gen.normal(98, 1, x.size).astype(np.float32)).cuda()
+
+    def step():
+        y, _ = pkg.segmented_pair_sort(xd, od)
+        return torch.autograd.grad(y, xd, g)[0]
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    n = x.size
+    algo_bytes = 2 * 12 * n  # forward: read x, write y + perm; backward: read g + perm, write grad
+    return {"workload": WORKLOADS["diff"], "n": n, "rows": off.size - 1, "ms_fwd_bwd": round(ms, 4),
+            "gkeys_s": round(n / (ms * 1e-3) / 1e9, 3),
+            "hbm_frac": round(algo_bytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,7 +243,20 @@ def main():
     ap.add_argument("--algos", default="batched,plain,fixpass")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only", default=None, help="time one algorithm and print its ms (profiling runs)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run one plain device copy of MIB MiB (HBM-counter calibration in profiling runs)")
+    ap.add_argument("--traffic-from", nargs=2, default=None, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="reduce two rocprofv3 --pmc counter CSVs (FETCH_SIZE, WRITE_SIZE) of a --copy-calibration "
+                         "run to HBM bytes per launch, store them for --workload in profiles/traffic.json (no GPU)")
     args = ap.parse_args()
+    if args.traffic_from:
+        t = traffic_from_pmc(*args.traffic_from, args.copy_calibration or 256)
+        p = ROOT / "profiles" / "traffic.json"
+        d = json.loads(p.read_text()) if p.exists() else {}
+        d[args.workload] = t["bytes"]
+        p.write_text(json.dumps(d, indent=1, sort_keys=True) + "\n")
+        print(json.dumps(t, indent=1, sort_keys=True))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(segments shard with no exchange, see DESIGN.md)", file=sys.stderr)
@@ -182,10 +266,21 @@ def main():
 
     pkg = load_pkg()
     pkg.load_library()
+    if args.workload == "diff":
+        print(json.dumps(bench_diff(pkg, torch, args.steps, args.warmup)), flush=True)
+        return
     keys, vals, off = make_workload(pkg, args.workload)
     n, s = keys.size, off.size - 1
     item_bytes = keys.dtype.itemsize + vals.dtype.itemsize
     algo_bytes = n * 2 * item_bytes + 8 * s  # one read + one write of every pair, begin+end offsets
+    tuning = pkg.autotune.apply(pkg, n, s, keys.dtype.itemsize, vals.dtype.itemsize)
+    if args.copy_calibration:
+        a = torch.empty(args.copy_calibration << 18, dtype=torch.float32, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     if args.only:
         r = Runner(pkg, torch, keys, vals, off, args.only)
@@ -241,7 +336,7 @@ def main():
         "dtype": "u32" if keys.dtype == np.uint32 else str(keys.dtype),
         "data": "synthetic (seeded splitmix64)",
         "config": {"workload": WORKLOADS.get(args.workload, args.workload), "num_items": n, "num_segments": s,
-                   "algorithm": "batched", "parallelism": "single GPU"},
+                   "algorithm": "batched", "parallelism": "single GPU", "tuning": tuning},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(args.workload, dname),
                      "kernel": dname, "avg_launch_us": round(avg_ms * 1e3, 2),

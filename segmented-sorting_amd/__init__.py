@@ -19,7 +19,7 @@ import ctypes
 import os
 from pathlib import Path
 
-from . import gen  # noqa: F401  (seeded workload generators)
+from . import autotune, gen  # noqa: F401  (autotuner, seeded workload generators)
 
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "lib" / "libsegsort.so"
@@ -209,11 +209,14 @@ def temp_bytes(num_items, num_segments, key_dtype, value_bytes=0, algo="batched"
 
 
 def segmented_sort(keys, offsets=None, values=None, *, begin=None, end=None, descending=False, begin_bit=0,
-                   end_bit=None, algo="batched", keys_out=None, values_out=None, temp=None, stream=None):
+                   end_bit=None, algo="batched", keys_out=None, values_out=None, temp=None, stream=None,
+                   full=False):
     """Stable per-segment sort of cuda tensors; returns (keys_out, values_out).
 
     Segments are [offsets[i], offsets[i+1]) or [begin[i], end[i]); positions outside
-    every segment are copied from the input (the kernels never write them).
+    every segment are copied from the input (the kernels never write them), unless
+    full=True, which promises that the segments cover every position and skips
+    that copy.
     """
     import torch
 
@@ -222,9 +225,9 @@ def segmented_sort(keys, offsets=None, values=None, *, begin=None, end=None, des
     n = keys.numel()
     s = begin.numel()
     if keys_out is None:
-        keys_out = keys.clone()
+        keys_out = torch.empty_like(keys) if full else keys.clone()
     if values is not None and values_out is None:
-        values_out = values.clone()
+        values_out = torch.empty_like(values) if full else values.clone()
     kt = key_type_of(keys.dtype)
     vb = 0 if values is None else values.element_size()
     need = temp_bytes(n, s, kt, vb, algo, begin_bit, end_bit)
@@ -295,7 +298,7 @@ def _autograd():
             st = _stream(None)
             iota = torch.empty(x.numel(), dtype=torch.int32, device=x.device)
             _check(lib.segsort_row_iota(begin.data_ptr(), end.data_ptr(), rows, iota.data_ptr(), st), "row_iota")
-            y, perm = segmented_sort(x, begin=begin, end=end, values=iota, algo=algo)
+            y, perm = segmented_sort(x, begin=begin, end=end, values=iota, algo=algo, full=True)
             ctx.save_for_backward(perm, offsets)
             ctx.mark_non_differentiable(perm)
             return y, perm
@@ -304,7 +307,7 @@ def _autograd():
         def backward(ctx, grad_y, grad_perm):
             perm, offsets = ctx.saved_tensors
             grad_y = grad_y.contiguous().float()
-            gin = torch.zeros_like(grad_y)
+            gin = torch.empty_like(grad_y)  # rows cover every position
             begin, end = offsets[:-1], offsets[1:]
             _check(load_library().segsort_perm_scatter_f32(grad_y.data_ptr(), perm.data_ptr(), begin.data_ptr(),
                                                            end.data_ptr(), begin.numel(), gin.data_ptr(),

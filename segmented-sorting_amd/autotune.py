@@ -1,0 +1,195 @@
+"""Autotuner of the batched segmented sort's tier cut-offs and tile classes.
+
+The library's tuning (per key width and value width) decides which tier sorts a
+range: the wave tier up to `wave_cap` items, then the enabled LDS tile classes
+(`tile_enable` bits, capacities from segsort_tile_caps), and the multi-workgroup
+MSD tier above the largest enabled class; `grid_mult` sizes the persistent grids.
+(`concurrent`, forking the tile classes onto parallel streams, is only honoured
+for direct launches and is not searched.)
+
+`tune()` times every candidate on a resident workload and keeps the fastest one
+whose output is bit-identical to the default tuning's. Results are stored in
+tuning.json keyed by a signature the host knows without reading device memory:
+key width, value width, round(log2 N) and round(log2 mean segment length).
+`apply()` looks up the nearest stored signature at plan time and installs it
+(one host-side table read, no device synchronisation).
+
+    python autotune.py --workload ragged   # tunes on the named bench workload (GPU)
+"""
+from __future__ import annotations
+
+import itertools
+import json
+import math
+import sys
+from pathlib import Path
+
+TABLE = Path(__file__).resolve().parent / "tuning.json"
+DEFAULT = {"wave_cap": 256, "tile_enable": 0xF, "grid_mult": 1, "concurrent": 0}
+
+
+def signature(num_items: int, num_segments: int, key_width: int, value_bytes: int) -> str:
+    n = max(num_items, 1)
+    mean = max(n / max(num_segments, 1), 1.0)
+    return f"k{key_width}v{value_bytes}-n{round(math.log2(n))}-m{round(math.log2(mean))}"
+
+
+def _parse(sig: str):
+    head, n, m = sig.split("-")
+    return head, int(n[1:]), int(m[1:])
+
+
+def candidates(quick: bool = False):
+    wcs, tes = ((128, 256, 512), (0xF, 0xE, 0xD, 0xC, 0xB))
+    if quick:
+        wcs, tes = (128, 512), (0xF, 0xC)
+    for wc, te, gm in itertools.product(wcs, tes, (1,) if quick else (1, 2)):
+        yield {"wave_cap": wc, "tile_enable": te, "grid_mult": gm, "concurrent": 0}
+
+
+def load_table(path: Path = TABLE) -> dict:
+    if path.exists():
+        return json.loads(path.read_text())
+    return {}
+
+
+def lookup(sig: str, table: dict | None = None) -> dict:
+    """The stored tuning of the nearest signature with the same key/value widths."""
+    table = load_table() if table is None else table
+    if sig in table:
+        return {k: table[sig][k] for k in DEFAULT}
+    head, n, m = _parse(sig)
+    best, dist = None, None
+    for s, t in table.items():
+        h2, n2, m2 = _parse(s)
+        if h2 != head:
+            continue
+        d = abs(n2 - n) + 2 * abs(m2 - m)
+        if dist is None or d < dist:
+            best, dist = t, d
+    return {k: best[k] for k in DEFAULT} if best else dict(DEFAULT)
+
+
+def apply(pkg, num_items: int, num_segments: int, key_width: int, value_bytes: int, table: dict | None = None) -> dict:
+    t = lookup(signature(num_items, num_segments, key_width, value_bytes), table)
+    pkg.set_tuning(key_width, value_bytes, t["wave_cap"], t["tile_enable"], t["grid_mult"], t["concurrent"])
+    return t
+
+
+def _time(pkg, torch, args, steps, warmup):
+    (kin, kout, vin, vout, n, s, begin, end, kt, vb) = args
+    tb = [0]
+    rc = pkg.DeviceSegmentedRadixSort._call(None, tb, None, None, None, None, n, s, None, None, 0, None, None, 0, kt,
+                                            vb, "batched")
+    if rc:
+        raise pkg.SegsortError(pkg.last_error())
+    temp = torch.empty(tb[0], dtype=torch.uint8, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+
+    def step():
+        rc = pkg.DeviceSegmentedRadixSort._call(temp, [temp.numel()], kin, kout, vin, vout, n, s, begin, end, 0, None,
+                                                st, 0, kt, vb, "batched")
+        if rc:
+            raise pkg.SegsortError(pkg.last_error())
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def _same(torch, a, b) -> bool:
+    return torch.equal(a.view(torch.uint8), b.view(torch.uint8))
+
+
+def tune(pkg, torch, keys, vals, offsets, steps=5, warmup=2, verbose=True, quick=False):
+    """Time every candidate on device-resident (keys, vals, offsets) tensors; return
+    (best tuning, {candidate: ms}). Candidates whose output differs from the default
+    tuning's are rejected (they would be a bug)."""
+    kt = pkg.key_type_of(keys.dtype)
+    kw = keys.element_size()
+    vb = 0 if vals is None else vals.element_size()
+    n, s = keys.numel(), offsets.numel() - 1
+    kout = torch.empty_like(keys)
+    vout = torch.empty_like(vals) if vals is not None else None
+    args = (keys, kout, vals, vout, n, s, offsets[:-1], offsets[1:], kt, vb)
+    before = pkg.get_tuning(kw, vb)
+    pkg.set_tuning(kw, vb, **DEFAULT)
+    _time(pkg, torch, args, 1, 0)
+    ref_k, ref_v = kout.clone(), (vout.clone() if vout is not None else None)
+    results, best = {}, None
+    try:
+        for c in candidates(quick):
+            pkg.set_tuning(kw, vb, **c)
+            ms = _time(pkg, torch, args, steps, warmup)
+            same = _same(torch, kout, ref_k) and (vout is None or _same(torch, vout, ref_v))
+            key = json.dumps(c, sort_keys=True)
+            results[key] = ms if same else None
+            if verbose:
+                print(f"[tune] {c} -> {ms:.4f} ms{'' if same else ' (output differs: rejected)'}", flush=True)
+            if same and (best is None or ms < best[1]):
+                best = (c, ms)
+    finally:
+        pkg.set_tuning(kw, vb, **before)
+    return best, results
+
+
+def store(sig: str, tuning: dict, ms: float, path: Path = TABLE, note: str = "") -> None:
+    table = load_table(path)
+    table[sig] = dict(tuning, ms=round(ms, 4), note=note)
+    path.write_text(json.dumps(table, indent=1, sort_keys=True) + "\n")
+
+
+def main(argv=None):
+    import argparse
+
+    import torch
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", action="append", default=None)
+    ap.add_argument("--steps", type=int, default=5)
+    args = ap.parse_args(argv)
+    root = Path(__file__).resolve().parent.parent
+    sys.path.insert(0, str(root))
+    import bench
+
+    pkg = bench.load_pkg()
+    pkg.load_library()
+    by_sig: dict[str, list] = {}
+    for wl in args.workload or ["ragged"]:
+        keys, vals, off = bench.make_workload(pkg, wl)
+        kdt = {"u": {4: torch.uint32, 8: torch.uint64}, "f": {4: torch.float32, 8: torch.float64},
+               "i": {4: torch.int32, 8: torch.int64}}[keys.dtype.kind][keys.dtype.itemsize]
+        dk = bench.to_dev(torch, keys).view(kdt)
+        dv = bench.to_dev(torch, vals).view(torch.uint32)
+        do = torch.from_numpy(off).cuda()
+        best, results = tune(pkg, torch, dk, dv, do, steps=args.steps)
+        sig = signature(keys.size, off.size - 1, keys.dtype.itemsize, vals.dtype.itemsize)
+        by_sig.setdefault(sig, []).append((wl, best[1], results))
+        print(json.dumps({"workload": wl, "signature": sig, "best": best[0], "ms": round(best[1], 4)}), flush=True)
+        del dk, dv, do
+    # workloads that share a signature: keep the candidate with the smallest sum of
+    # times relative to each workload's own best
+    for sig, runs in by_sig.items():
+        score = {}
+        for _, best_ms, results in runs:
+            for cand, ms in results.items():
+                if ms is None or score.get(cand, 0.0) is None:
+                    score[cand] = None
+                else:
+                    score[cand] = score.get(cand, 0.0) + ms / best_ms
+        cand = min((c for c, v in score.items() if v is not None), key=lambda c: score[c])
+        ms = sum(r[2][cand] for r in runs) / len(runs)
+        store(sig, json.loads(cand), ms, note="tuned on " + ", ".join(r[0] for r in runs))
+        print(json.dumps({"signature": sig, "chosen": json.loads(cand), "relative": round(score[cand], 4)}),
+              flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -15,7 +15,7 @@ COMMON = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-Xclang", "-target-fea
           "-packed-fp32-ops", "-Wall", "-Wno-unused-result"]
 SOURCES = [HERE / "csrc" / "segsort_api.hip"]
 LIB = HERE / "lib" / "libsegsort.so"
-DRIVERS = ["radixseg", "fixcub", "mergeseg", "bbsegsort"]
+DRIVERS = ["radixseg", "fixcub", "mergeseg", "bbsegsort", "facade_example"]
 
 
 def _run(cmd: list[str]) -> None:

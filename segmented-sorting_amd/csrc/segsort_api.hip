@@ -43,7 +43,7 @@ struct Tuning {
     int wave_cap;     // ranges of 2..wave_cap items go to the wave tier
     int tile_enable;  // bit i: LDS tile class i on; ranges above the largest enabled class use the MSD tier
     int grid_mult;    // persistent workgroups per resident slot
-    int concurrent;   // tile classes run on forked streams (graph branches) instead of one after another
+    int concurrent;   // tile classes run on forked streams instead of one after another (direct launches only)
 };
 
 // index: (key_width == 8) * 3 + value-bytes slot (0, 4, 8 -> 0, 1, 2)
@@ -114,8 +114,15 @@ struct GraphEntry {
     GraphKey key;
     hipGraph_t graph;
     hipGraphExec_t exec;
+    hipEvent_t done;  // recorded after every replay: eviction waits on it before destroying
     uint64_t last_use;
 };
+
+hipError_t replay(GraphEntry& g, hipStream_t st) {
+    hipError_t e = hipGraphLaunch(g.exec, st);
+    if (e != hipSuccess) return e;
+    return hipEventRecord(g.done, st);
+}
 
 std::mutex g_graph_mu;
 std::vector<GraphEntry> g_graphs;
@@ -143,12 +150,13 @@ hipError_t run_graph(const GraphKey& key, hipStream_t st, F&& enqueue) {
         hipError_t e = hipStreamIsCapturing(st, &cs);
         if (e != hipSuccess) return e;
     }
-    if (!graphs_enabled() || Profiler::get().on() || cs != hipStreamCaptureStatusNone) return enqueue(st);
+    if (!graphs_enabled() || Profiler::get().on() || cs != hipStreamCaptureStatusNone)
+        return enqueue(st, cs != hipStreamCaptureStatusNone);
     std::lock_guard<std::mutex> lk(g_graph_mu);
     for (auto& g : g_graphs) {
         if (same_key(g.key, key)) {
             g.last_use = ++g_graph_clock;
-            return hipGraphLaunch(g.exec, st);
+            return replay(g, st);
         }
     }
     static thread_local hipStream_t cap = nullptr;
@@ -158,7 +166,7 @@ hipError_t run_graph(const GraphKey& key, hipStream_t st, F&& enqueue) {
     }
     hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
-    hipError_t le = enqueue(cap);
+    hipError_t le = enqueue(cap, true);
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(cap, &graph);
     if (le != hipSuccess || e != hipSuccess) {
@@ -171,15 +179,24 @@ hipError_t run_graph(const GraphKey& key, hipStream_t st, F&& enqueue) {
         (void)hipGraphDestroy(graph);
         return e;
     }
+    hipEvent_t done = nullptr;
+    e = hipEventCreateWithFlags(&done, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        return e;
+    }
     if (g_graphs.size() >= GRAPH_CACHE) {
         auto victim = std::min_element(g_graphs.begin(), g_graphs.end(),
                                        [](const GraphEntry& a, const GraphEntry& b) { return a.last_use < b.last_use; });
+        (void)hipEventSynchronize(victim->done);  // its last replay may still be running
         (void)hipGraphExecDestroy(victim->exec);
         (void)hipGraphDestroy(victim->graph);
+        (void)hipEventDestroy(victim->done);
         g_graphs.erase(victim);
     }
-    g_graphs.push_back({key, graph, exec, ++g_graph_clock});
-    return hipGraphLaunch(exec, st);
+    g_graphs.push_back({key, graph, exec, done, ++g_graph_clock});
+    return replay(g_graphs.back(), st);
 }
 
 // Per-thread, per-device streams and events for forking the tile classes.
@@ -379,7 +396,7 @@ struct Launcher {
     // Launch the batched sequence (bin, MSD levels, tile tiers) on st.
     static hipError_t enqueue_batched(char* base, const Layout& L, const Plan& p, const Tuning& tu, int levels,
                                       const void* ki, void* ko, const void* vi, void* vo, int S, const int* db,
-                                      const int* de, KeyMode km, hipStream_t st) {
+                                      const int* de, KeyMode km, hipStream_t st, bool allow_fork) {
         const Sched sc = make_sched(base, L, p, false);
         const Bufs b = make_bufs(base, L, ki, ko, vi, vo);
         const size_t mark = Profiler::get().mark();
@@ -396,7 +413,7 @@ struct Launcher {
                            (k_large_scan<<<device_cus() * 4, RADIX, 0, st>>>(sc, l, km, p.cc, p.tile_max)));
             SEGSORT_LAUNCH("large_scatter", st, 100 + l, PAIR_BYTES, 0.0, (ks<<<gs, C::NT3, 0, st>>>(b, sc, l, km)));
         }
-        if (tu.concurrent && p.nclass > 1) {
+        if (allow_fork && tu.concurrent && p.nclass > 1) {
             // fork: every tile class on its own stream, joined back into st
             SideStreams& ss = side_streams();
             if (!ss.ok) return hipErrorOutOfMemory;
@@ -440,8 +457,8 @@ struct Launcher {
         key.vb = HAS_V ? (int)sizeof(V) : 0;
         key.tu = tu;
         key.dev = current_device();
-        hipError_t e = run_graph(key, st, [&](hipStream_t cs) {
-            return enqueue_batched(base, L, p, tu, levels, ki, ko, vi, vo, S, db, de, km, cs);
+        hipError_t e = run_graph(key, st, [&](hipStream_t cs, bool captured) {
+            return enqueue_batched(base, L, p, tu, levels, ki, ko, vi, vo, S, db, de, km, cs, !captured);
         });
         if (e != hipSuccess) return fail(e, "segsort_sort(batched): launch");
         return ok();

@@ -55,3 +55,9 @@ def test_driver_timing_mode(pkg):
                            capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stderr
         assert float(r.stdout.strip()) > 0
+
+
+def test_cpp_facade_example():
+    """Every overload of the cub-style C++ facade on cub's documented example."""
+    r = subprocess.run([str(BIN / "facade_example")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "facade ok" in r.stdout, r.stdout + r.stderr

@@ -149,9 +149,16 @@ def cpu_baseline(keys, vals, off, budget_s=12.0):
                 break
     except OSError:
         pass
+    # the sequential form of the same restatement (one core), a shorter sample
+    reps1, total1 = 0, 0.0
+    while total1 < budget_s / 3 and reps1 < 5:
+        total1 += orc.time_seg_sort(keys, vals, off, 1)
+        reps1 += 1
     return {"value": keys.size / t / 1e9, "unit": "Gkeys/s", "cores": threads, "kind": "port",
             "sample": f"full workload ({keys.size} pairs), {reps} reps, OpenMP dynamic over segments, "
-                      f"std::stable_sort per segment, {cpu_model}"}
+                      f"std::stable_sort per segment, {cpu_model}",
+            "single_core": {"value": round(keys.size / (total1 / reps1) / 1e9, 4), "cores": 1,
+                            "sample": f"full workload, {reps1} reps"}}
 
 
 def profiler_name(kernel_name):
@@ -234,6 +241,32 @@ def bench_diff(pkg, torch, steps, warmup):
             "hbm_frac": round(algo_bytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
 
 
+def sweep(pkg, torch, args):
+    """One JSON line per k: ms and Gkeys/s of every form on 2^24 u32 pairs (values =
+    index) in 2^k equal segments, keys uniform in [0, 2^32)."""
+    rows = []
+    for k in range(0, 21, 2):
+        keys, vals, off = make_workload(pkg, f"equal:{k}")
+        n, s = keys.size, off.size - 1
+        tuning = pkg.autotune.apply(pkg, n, s, 4, 4)
+        row = {"k": k, "segments": s, "segment_length": n // s, "tuning": tuning}
+        for a in [a for a in args.algos.split(",") if a]:
+            r = Runner(pkg, torch, keys, vals, off, a)
+            ms, _ = r.time(args.steps, args.warmup)
+            row[a] = {"ms": round(ms, 4), "gkeys_s": round(n / (ms * 1e-3) / 1e9, 3),
+                      "hbm_frac": round((n * 16 + 8 * s) / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+                      "sorted": sorted_within_segments(torch, r, keys.dtype)}
+            del r
+            torch.cuda.empty_cache()
+        if "batched" in row and "plain" in row:
+            row["batched_over_plain"] = round(row["plain"]["ms"] / row["batched"]["ms"], 3)
+        if "batched" in row and "fixpass" in row:
+            row["batched_over_fixpass"] = round(row["fixpass"]["ms"] / row["batched"]["ms"], 3)
+        print(json.dumps(row), flush=True)
+        rows.append(row)
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -248,6 +281,8 @@ def main():
     ap.add_argument("--traffic-from", nargs=2, default=None, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="reduce two rocprofv3 --pmc counter CSVs (FETCH_SIZE, WRITE_SIZE) of a --copy-calibration "
                          "run to HBM bytes per launch, store them for --workload in profiles/traffic.json (no GPU)")
+    ap.add_argument("--sweep", action="store_true",
+                    help="equal-segment sweep: 2^24 u32 pairs in 2^k segments, k = 0,2,...,20, every form")
     args = ap.parse_args()
     if args.traffic_from:
         t = traffic_from_pmc(*args.traffic_from, args.copy_calibration or 256)
@@ -266,6 +301,9 @@ def main():
 
     pkg = load_pkg()
     pkg.load_library()
+    if args.sweep:
+        sweep(pkg, torch, args)
+        return
     if args.workload == "diff":
         print(json.dumps(bench_diff(pkg, torch, args.steps, args.warmup)), flush=True)
         return

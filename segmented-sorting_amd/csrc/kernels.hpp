@@ -291,15 +291,23 @@ __global__ __launch_bounds__(NT) void k_large_hist(Bufs b, Sched sc, int level, 
     }
 }
 
-// Pass 2 of a level: one workgroup (256 threads, one per digit) per range.
-// Chooses the highest digit position whose digit takes more than one value,
-// writes every chunk's destination offsets, and queues the child ranges.
-__global__ __launch_bounds__(RADIX) void k_large_scan(Sched sc, int level, KeyMode km, ClassCaps cc,
-                                                      int tile_max) {
+// Pass 2 of a level: one workgroup per range. Its SCAN_G groups of 256 threads
+// (one thread per digit) each reduce a contiguous share of the range's chunk
+// histograms; the workgroup then chooses the highest digit position whose
+// digit takes more than one value, writes every chunk's destination offsets
+// (each group for its own share of chunks, from the prefix of the groups
+// before it), and queues the child ranges.
+constexpr int SCAN_NT = 1024;
+constexpr int SCAN_G = SCAN_NT / RADIX;
+
+__global__ __launch_bounds__(SCAN_NT) void k_large_scan(Sched sc, int level, KeyMode km, ClassCaps cc,
+                                                        int tile_max) {
+    __shared__ uint32_t part[SCAN_G][MAX_LEVELS][RADIX];
+    __shared__ uint32_t base_s[RADIX];
     __shared__ uint32_t nz[MAX_LEVELS];
     __shared__ uint32_t wsum[RADIX / WAVE];
-    const int d = threadIdx.x;
-    const int w = d >> 6;
+    const int d = threadIdx.x & (RADIX - 1);
+    const int g = threadIdx.x / RADIX;
     int count, chunks;
     level_counts(sc, level, count, chunks);
     LargeItem* items = sc.lq[level];
@@ -308,17 +316,35 @@ __global__ __launch_bounds__(RADIX) void k_large_scan(Sched sc, int level, KeyMo
         LargeItem it = items[i];
         const int nch = (it.n + sc.chunk - 1) / sc.chunk;
         const int npos = npos_of(it.hi_bit, km.begin_bit);
-        if (d < MAX_LEVELS) nz[d] = 0;
-        __syncthreads();
+        const int c0 = (nch * g) / SCAN_G, c1 = (nch * (g + 1)) / SCAN_G;
+        if (threadIdx.x < MAX_LEVELS) nz[threadIdx.x] = 0;
         uint32_t tot[MAX_LEVELS];
+#pragma unroll
         for (int p = 0; p < MAX_LEVELS; ++p) tot[p] = 0;
-        for (int c = 0; c < nch; ++c) {
-            const uint32_t* h = sc.hist + (size_t)(it.chunk_base + c) * sc.npos_max * RADIX;
-            for (int p = 0; p < npos; ++p) tot[p] += h[p * RADIX + d];
+#pragma unroll 4
+        for (int c = c0; c < c1; ++c) {
+            const uint32_t* h = sc.hist + (size_t)(it.chunk_base + c) * sc.npos_max * RADIX + d;
+#pragma unroll
+            for (int p = 0; p < MAX_LEVELS; ++p)
+                if (p < npos) tot[p] += h[p * RADIX];
         }
-        for (int p = 0; p < npos; ++p) {
-            const uint64_t m = __ballot(tot[p] > 0);
-            if (lane_id() == 0 && m) atomicAdd(&nz[p], (uint32_t)__popcll(m));
+#pragma unroll
+        for (int p = 0; p < MAX_LEVELS; ++p)
+            if (p < npos) part[g][p][d] = tot[p];
+        __syncthreads();
+        uint32_t all[MAX_LEVELS];
+#pragma unroll
+        for (int p = 0; p < MAX_LEVELS; ++p) {
+            all[p] = 0;
+            if (p < npos)
+                for (int gg = 0; gg < SCAN_G; ++gg) all[p] += part[gg][p][d];
+        }
+        if (g == 0) {
+#pragma unroll
+            for (int p = 0; p < MAX_LEVELS; ++p) {
+                const uint64_t m = __ballot(p < npos && all[p] > 0);
+                if (lane_id() == 0 && m) atomicAdd(&nz[p], (uint32_t)__popcll(m));
+            }
         }
         __syncthreads();
         int pstar = -1;
@@ -328,7 +354,7 @@ __global__ __launch_bounds__(RADIX) void k_large_scan(Sched sc, int level, KeyMo
                 break;
             }
         if (pstar < 0) {
-            if (d == 0) {
+            if (threadIdx.x == 0) {
                 it.mode = LMODE_COPY;
                 items[i] = it;
             }
@@ -340,26 +366,33 @@ __global__ __launch_bounds__(RADIX) void k_large_scan(Sched sc, int level, KeyMo
             if (nz[p] > 1) final_ = false;
         const int shift = km.begin_bit + pstar * RADIX_BITS;
         const int bits = (it.hi_bit - shift) < RADIX_BITS ? (it.hi_bit - shift) : RADIX_BITS;
-        const uint32_t mine = tot[pstar];
+        uint32_t mine = 0;
+#pragma unroll
+        for (int p = 0; p < MAX_LEVELS; ++p)
+            if (p == pstar) mine = all[p];
         const uint32_t incl = wave_incl_scan(mine);
-        if (lane_id() == 63) wsum[w] = incl;
+        if (g == 0 && lane_id() == 63) wsum[d >> 6] = incl;
         __syncthreads();
         uint32_t base = incl - mine;
-        for (int ww = 0; ww < w; ++ww) base += wsum[ww];
+        for (int ww = 0; ww < (d >> 6); ++ww) base += wsum[ww];
+        if (g == 0) base_s[d] = base;
+        // this group's chunks start after the digit-d items of the groups before it
         uint32_t run = (uint32_t)it.begin + base;
-        for (int c = 0; c < nch; ++c) {
-            const size_t g = (size_t)(it.chunk_base + c);
-            sc.off[g * RADIX + d] = run;
-            run += sc.hist[g * sc.npos_max * RADIX + pstar * RADIX + d];
+        for (int gg = 0; gg < g; ++gg) run += part[gg][pstar][d];
+#pragma unroll 4
+        for (int c = c0; c < c1; ++c) {
+            const size_t gc = (size_t)(it.chunk_base + c);
+            sc.off[gc * RADIX + d] = run;
+            run += sc.hist[gc * sc.npos_max * RADIX + pstar * RADIX + d];
         }
-        if (!final_) {
+        if (g == 0 && !final_) {
             const int cb = it.begin + (int)base;
             const int cn = (int)mine;
             const int cls = (cn >= 1 && cn <= tile_max) ? class_of(cc, cn) : -1;
             push_tile(sc, cls, cls >= 0, make_ti(cb, cls >= 0 ? cn : 0, child_buf));
             if (cn > tile_max) push_large(sc, level + 1, cb, cn, child_buf, shift);
         }
-        if (d == 0) {
+        if (threadIdx.x == 0) {
             it.shift = shift;
             it.bits = bits;
             it.mode = final_ ? LMODE_FINAL : LMODE_PARTITION;
@@ -399,6 +432,157 @@ __global__ __launch_bounds__(NT) void k_large_scatter(Bufs b, Sched sc, int leve
         const int dbuf = it.mode == LMODE_FINAL ? BUF_OUT : child_buf;
         T::partition(sm, ks, vs, buf_ptr<U>(b.k[dbuf]), buf_ptr<V>(b.v[dbuf]), start, n, km, it.shift, it.bits,
                      sc.off + (size_t)g * RADIX);
+    }
+}
+
+// Levels below the first: one workgroup takes a range that the level-0 scan
+// queued and partitions it, and every descendant still larger than a tile,
+// itself, depth first, so that no workgroup ever waits for another. Each
+// partition histograms the range for every digit position below its hi_bit,
+// picks the highest position whose digit takes more than one value, and
+// scatters the range chunk by chunk (running destinations in LDS) into the
+// other temporary buffer, or into the output when no lower digit varies.
+// Children that fit a tile go to the tile queues; a range whose keys are all
+// equal within the bit range is copied out as it is.
+template <int NT, int IPT, typename U, typename V, bool HAS_V>
+__global__ __launch_bounds__(NT) void k_large_local(Bufs b, Sched sc, int level, KeyMode km, ClassCaps cc,
+                                                    int tile_max) {
+    using T = RadixTile<NT, IPT, U, V, HAS_V>;
+    constexpr int NPOS = int(sizeof(U));  // 8-bit digit positions of a key
+    __shared__ union {
+        typename T::Smem sm;
+        uint32_t hist[NPOS * RADIX];
+    } u;
+    __shared__ uint32_t cstart[NPOS][RADIX + 1];  // child starts (range-relative) at each depth
+    __shared__ uint32_t run[RADIX];               // running destinations of a scatter
+    __shared__ uint32_t nz[NPOS];
+    __shared__ uint32_t wsum[RADIX / WAVE];
+    __shared__ int fb[NPOS + 1], fn[NPOS + 1], fsrc[NPOS + 1], fhi[NPOS + 1], fdst[NPOS + 1], fshift[NPOS + 1],
+        fnext[NPOS + 1];
+    __shared__ int sel;
+    const int t = threadIdx.x;
+    int count, chunks;
+    level_counts(sc, level, count, chunks);
+    const LargeItem* items = sc.lq[level];
+    for (;;) {
+        if (t == 0) sel = (int)atomicAdd(&sc.ctr->claim, 1u);
+        __syncthreads();
+        const int i = sel;
+        __syncthreads();
+        if (i >= count) break;
+        if (t == 0) {
+            const LargeItem it = items[i];
+            fb[0] = it.begin;
+            fn[0] = it.n;
+            fsrc[0] = it.src;
+            fhi[0] = it.hi_bit;
+        }
+        __syncthreads();
+        int depth = 0;
+        bool fresh = true;
+        while (depth >= 0) {
+            if (fresh) {
+                const int beg = fb[depth], n = fn[depth], src = fsrc[depth], hi = fhi[depth];
+                const int npos = npos_of(hi, km.begin_bit);
+                const U* ks = (const U*)b.k[src];
+                const V* vs = (const V*)b.v[src];
+                for (int x = t; x < NPOS * RADIX; x += NT) u.hist[x] = 0;
+                if (t < NPOS) nz[t] = 0;
+                __syncthreads();
+                T::histogram(u.hist, ks, beg, n, km, km.begin_bit, npos, hi);
+                __syncthreads();
+                if (t < RADIX) {
+                    for (int p = 0; p < npos; ++p) {
+                        const uint64_t m = __ballot(u.hist[p * RADIX + t] > 0);
+                        if (lane_id() == 0 && m) atomicAdd(&nz[p], (uint32_t)__popcll(m));
+                    }
+                }
+                __syncthreads();
+                int pstar = -1;
+                for (int p = npos - 1; p >= 0; --p)
+                    if (nz[p] > 1) {
+                        pstar = p;
+                        break;
+                    }
+                if (pstar < 0) {
+                    U* ko = buf_ptr<U>(b.k[BUF_OUT]);
+                    V* vo = buf_ptr<V>(b.v[BUF_OUT]);
+                    for (int x = t; x < n; x += NT) {
+                        ko[beg + x] = ks[beg + x];
+                        if constexpr (HAS_V) vo[beg + x] = vs[beg + x];
+                    }
+                    __syncthreads();
+                    --depth;
+                    fresh = false;
+                    continue;
+                }
+                bool final_ = true;
+                for (int p = 0; p < pstar; ++p)
+                    if (nz[p] > 1) final_ = false;
+                const int shift = km.begin_bit + pstar * RADIX_BITS;
+                const int bits = (hi - shift) < RADIX_BITS ? (hi - shift) : RADIX_BITS;
+                const int dst = final_ ? BUF_OUT : (src == BUF_TMP_A ? BUF_TMP_B : BUF_TMP_A);
+                uint32_t mine = 0, incl = 0;
+                if (t < RADIX) {
+                    mine = u.hist[pstar * RADIX + t];
+                    incl = wave_incl_scan(mine);
+                    if (lane_id() == 63) wsum[t >> 6] = incl;
+                }
+                __syncthreads();
+                if (t < RADIX) {
+                    uint32_t excl = incl - mine;
+                    for (int ww = 0; ww < (t >> 6); ++ww) excl += wsum[ww];
+                    cstart[depth][t] = excl;
+                    if (t == RADIX - 1) cstart[depth][RADIX] = excl + mine;
+                    run[t] = (uint32_t)beg + excl;
+                }
+                if (t == 0) {
+                    fshift[depth] = shift;
+                    fdst[depth] = dst;
+                    fnext[depth] = 0;
+                }
+                __syncthreads();  // the histogram (aliased with the tile) is dead from here
+                for (int c0 = 0; c0 < n; c0 += T::TILE) {
+                    const int cn = (n - c0) < T::TILE ? (n - c0) : T::TILE;
+                    T::partition_run(u.sm, ks, vs, buf_ptr<U>(b.k[dst]), buf_ptr<V>(b.v[dst]), (int64_t)beg + c0,
+                                     cn, km, shift, bits, run);
+                }
+                if (!final_ && t < RADIX) {
+                    const uint32_t cb = cstart[depth][t], cn = cstart[depth][t + 1] - cb;
+                    const int cls = (cn >= 1 && (int)cn <= tile_max) ? class_of(cc, (int)cn) : -1;
+                    push_tile(sc, cls, cls >= 0, make_ti(beg + (int)cb, cls >= 0 ? (int)cn : 0, dst));
+                }
+                __syncthreads();  // the scatter's writes are visible to the workgroup before a child is read
+                fresh = false;
+                if (final_) {
+                    --depth;
+                    continue;
+                }
+            }
+            // the next child of this depth's range that is larger than a tile
+            const int from = fnext[depth];
+            if (t == 0) sel = RADIX;
+            __syncthreads();
+            if (t < RADIX && t >= from && (int)(cstart[depth][t + 1] - cstart[depth][t]) > tile_max)
+                atomicMin(&sel, t);
+            __syncthreads();
+            const int dn = sel;
+            __syncthreads();
+            if (dn >= RADIX) {
+                --depth;
+                continue;
+            }
+            if (t == 0) {
+                fnext[depth] = dn + 1;
+                fb[depth + 1] = fb[depth] + (int)cstart[depth][dn];
+                fn[depth + 1] = (int)(cstart[depth][dn + 1] - cstart[depth][dn]);
+                fsrc[depth + 1] = fdst[depth];
+                fhi[depth + 1] = fshift[depth];
+            }
+            __syncthreads();
+            ++depth;
+            fresh = true;
+        }
     }
 }
 
@@ -446,14 +630,8 @@ __global__ __launch_bounds__(NT) void k_seg_lsd(Bufs b, Sched sc, KeyMode km) {
         __syncthreads();
         for (int t0 = 0; t0 < it.n; t0 += T::TILE) {
             const int n = (it.n - t0) < T::TILE ? (it.n - t0) : T::TILE;
-            T::partition(sm, ks, vs, buf_ptr<U>(b.k[dbuf]), buf_ptr<V>(b.v[dbuf]), (int64_t)it.begin + t0, n, km,
-                         shift, bits, run);
-            // advance every digit's running offset by this tile's count
-            for (int d = threadIdx.x; d < RADIX; d += NT) {
-                const uint32_t next = (d + 1 < RADIX) ? sm.dstart[d + 1] : (uint32_t)n;
-                run[d] += next - sm.dstart[d];
-            }
-            __syncthreads();
+            T::partition_run(sm, ks, vs, buf_ptr<U>(b.k[dbuf]), buf_ptr<V>(b.v[dbuf]), (int64_t)it.begin + t0, n,
+                             km, shift, bits, run);
         }
     }
 }

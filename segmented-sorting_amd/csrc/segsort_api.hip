@@ -287,7 +287,9 @@ Layout make_layout(const Plan& p, int num_items, int num_segments, int levels, b
     const size_t N = (size_t)num_items, S = (size_t)num_segments;
     L.lq_cap = N / (size_t)(p.tile_max + 1) + 1;
     L.chunks = N / (size_t)C::TILE_MAX + L.lq_cap + 1;
-    const size_t bucket_bound = plain ? 0 : (size_t)RADIX * L.lq_cap * (size_t)std::max(levels, 1);
+    // every partition (level 0 or deeper) queues at most RADIX tile children, and
+    // the ranges partitioned at one depth are disjoint and larger than a tile
+    const size_t bucket_bound = plain ? 0 : (size_t)RADIX * L.lq_cap * (size_t)C::NPOS;
     L.ctr = take(sizeof(Counters));
     for (int c = 0; c < p.nclass; ++c) {
         const size_t lo = c == 0 ? 1 : (size_t)p.cc.cap[c - 1] + 1;
@@ -299,7 +301,10 @@ Layout make_layout(const Plan& p, int num_items, int num_segments, int levels, b
         L.lq_cap = std::max<size_t>(std::min(S, L.lq_cap), 1);
         L.lq[MAX_LEVELS] = take(L.lq_cap * sizeof(LargeItem));
     } else {
-        for (int l = 0; l < levels; ++l) L.lq[l] = take(L.lq_cap * sizeof(LargeItem));
+        if (levels > 0) {
+            L.lq[0] = take(L.lq_cap * sizeof(LargeItem));  // multi-workgroup level 0
+            L.lq[1] = take(L.lq_cap * sizeof(LargeItem));  // ranges for the depth-first tier
+        }
         L.hist = take(L.chunks * C::NPOS * RADIX * sizeof(uint32_t));
         L.off = take(L.chunks * RADIX * sizeof(uint32_t));
     }
@@ -388,9 +393,11 @@ struct Launcher {
         return b;
     }
 
+    // 1 when a segment may exceed the largest tile: then the multi-workgroup
+    // level 0 and the depth-first tier below it are launched.
     static int levels_for(const KeyMode& km, int num_items, const Plan& p) {
-        if (num_items <= p.tile_max) return 0;
-        return (km.end_bit - km.begin_bit + RADIX_BITS - 1) / RADIX_BITS;
+        (void)km;
+        return num_items > p.tile_max ? 1 : 0;
     }
 
     // Launch the batched sequence (bin, MSD levels, tile tiers) on st.
@@ -403,15 +410,19 @@ struct Launcher {
         SEGSORT_LAUNCH("init", st, -1, 0.0, 0.0, (k_init<<<1, 64, 0, st>>>(sc.ctr)));
         SEGSORT_LAUNCH("bin", st, -1, 0.0, 8.0 * S,
                        (k_bin<U, V, HAS_V><<<bin_grid(S), BIN_NT, 0, st>>>(db, de, S, b, sc, p.cc, km.end_bit, 1)));
-        for (int l = 0; l < levels; ++l) {
+        if (levels > 0) {
             auto kh = k_large_hist<C::NT3, C::IPT3, U, V, HAS_V>;
             auto ks = k_large_scatter<C::NT3, C::IPT3, U, V, HAS_V>;
+            auto kl = k_large_local<C::NT3, C::IPT3, U, V, HAS_V>;
             const int gh = resident_blocks(kh, C::NT3) * 2;
             const int gs = resident_blocks(ks, C::NT3) * tu.grid_mult;
-            SEGSORT_LAUNCH("large_hist", st, 100 + l, (double)sizeof(U), 0.0, (kh<<<gh, C::NT3, 0, st>>>(b, sc, l, km)));
+            const int gl = resident_blocks(kl, C::NT3);
+            SEGSORT_LAUNCH("large_hist", st, 100, (double)sizeof(U), 0.0, (kh<<<gh, C::NT3, 0, st>>>(b, sc, 0, km)));
             SEGSORT_LAUNCH("large_scan", st, -1, 0.0, 0.0,
-                           (k_large_scan<<<device_cus() * 4, RADIX, 0, st>>>(sc, l, km, p.cc, p.tile_max)));
-            SEGSORT_LAUNCH("large_scatter", st, 100 + l, PAIR_BYTES, 0.0, (ks<<<gs, C::NT3, 0, st>>>(b, sc, l, km)));
+                           (k_large_scan<<<device_cus(), SCAN_NT, 0, st>>>(sc, 0, km, p.cc, p.tile_max)));
+            SEGSORT_LAUNCH("large_scatter", st, 100, PAIR_BYTES, 0.0, (ks<<<gs, C::NT3, 0, st>>>(b, sc, 0, km)));
+            SEGSORT_LAUNCH("large_local", st, 101, PAIR_BYTES + sizeof(U), 0.0,
+                           (kl<<<gl, C::NT3, 0, st>>>(b, sc, 1, km, p.cc, p.tile_max)));
         }
         if (allow_fork && tu.concurrent && p.nclass > 1) {
             // fork: every tile class on its own stream, joined back into st

@@ -270,8 +270,28 @@ struct RadixTile {
     // (twiddled >> shift) & (2^bits - 1). Item with digit d and rank r among
     // the chunk's digit-d items goes to global index goff[d] + r. The chunk is
     // ordered by digit in LDS first, so each digit run is written contiguously.
+    // goff comes from global memory (one offset table per chunk).
     __device__ static void partition(Smem& sm, const U* ks, const V* vs, U* kd, V* vd, int64_t begin, int n,
                                      const KeyMode& km, int shift, int bits, const uint32_t* goff_global) {
+        for (int d = threadIdx.x; d < RADIX; d += NT) sm.goff[d] = goff_global[d];
+        partition_body(sm, ks, vs, kd, vd, begin, n, km, shift, bits, sm.goff);  // barriers before goff is read
+    }
+
+    // The same with destinations from an LDS table run[] that the workgroup
+    // keeps across the chunks of one range: afterwards run[d] has advanced by
+    // the chunk's count of digit d.
+    __device__ static void partition_run(Smem& sm, const U* ks, const V* vs, U* kd, V* vd, int64_t begin, int n,
+                                         const KeyMode& km, int shift, int bits, uint32_t* run) {
+        partition_body(sm, ks, vs, kd, vd, begin, n, km, shift, bits, run);
+        for (int d = threadIdx.x; d < RADIX; d += NT) {
+            const uint32_t end = d + 1 < RADIX ? sm.dstart[d + 1] : (uint32_t)n;
+            run[d] += end - sm.dstart[d];
+        }
+        __syncthreads();
+    }
+
+    __device__ static void partition_body(Smem& sm, const U* ks, const V* vs, U* kd, V* vd, int64_t begin, int n,
+                                          const KeyMode& km, int shift, int bits, const uint32_t* goff) {
         const bool act = wave() < active_waves(n);
         U k[IPT];
         V v[IPT];
@@ -285,9 +305,8 @@ struct RadixTile {
                 }
             }
         }
-        for (int d = threadIdx.x; d < RADIX; d += NT) sm.goff[d] = goff_global[d];
         uint32_t dst[IPT];
-        rank_digits(sm, k, n, shift, bits, dst);  // contains barriers: goff visible after
+        rank_digits(sm, k, n, shift, bits, dst);  // contains barriers
         if (act) {
 #pragma unroll
             for (int j = 0; j < IPT; ++j)
@@ -305,7 +324,7 @@ struct RadixTile {
                 if (p < n) {
                     const U key = sm.keys[p];
                     const uint32_t d = (uint32_t)((key >> shift) & dmask);
-                    const uint32_t g = sm.goff[d] + ((uint32_t)p - sm.dstart[d]);
+                    const uint32_t g = goff[d] + ((uint32_t)p - sm.dstart[d]);
                     kd[g] = twiddle_out<U>(key, km.kind, km.desc);
                     if constexpr (HAS_V) vd[g] = sm.vals[p];
                 }

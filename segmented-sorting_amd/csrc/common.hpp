@@ -80,28 +80,57 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
-    const uint32_t l = lane_id();
-#pragma unroll
-    for (int o = 1; o < WAVE; o <<= 1) {
-        uint32_t y = __shfl_up(x, o, WAVE);
-        if (l >= (uint32_t)o) x += y;
-    }
+// DPP controls: row_shr:n shifts within rows of 16 lanes, row_bcast:15 and
+// row_bcast:31 pass lane 15 of each row (lane 31) on to the next row(s).
+constexpr int DPP_ROW_SHR = 0x110;
+constexpr int DPP_ROW_BCAST15 = 0x142;
+constexpr int DPP_ROW_BCAST31 = 0x143;
+
+// One step of a DPP scan: the value of the source lane, or `identity` where
+// the control has no source lane or the row is masked off.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_src(uint32_t x, uint32_t identity) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)identity, (int)x, CTRL, ROWS, 0xF, false);
+}
+
+// Inclusive scan over the 64 lanes with cross-lane DPP moves (no LDS):
+// Hillis-Steele inside each row of 16, then rows 1 and 3 add lane 15 of the
+// row before them, and rows 2 and 3 add lane 31.
+template <typename Op>
+__device__ __forceinline__ uint32_t wave_incl_scan_op(uint32_t x, uint32_t identity, Op op) {
+    x = op(x, dpp_src<DPP_ROW_SHR + 1, 0xF>(x, identity));
+    x = op(x, dpp_src<DPP_ROW_SHR + 2, 0xF>(x, identity));
+    x = op(x, dpp_src<DPP_ROW_SHR + 4, 0xF>(x, identity));
+    x = op(x, dpp_src<DPP_ROW_SHR + 8, 0xF>(x, identity));
+    x = op(x, dpp_src<DPP_ROW_BCAST15, 0xA>(x, identity));
+    x = op(x, dpp_src<DPP_ROW_BCAST31, 0xC>(x, identity));
     return x;
 }
 
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+    return wave_incl_scan_op(x, 0u, [](uint32_t a, uint32_t b) { return a + b; });
+}
+
+// OR / AND of x over the wave, returned to every lane (lane 63 of the
+// inclusive scan holds the whole wave's value).
 template <typename U>
 __device__ __forceinline__ U wave_or(U x) {
-#pragma unroll
-    for (int o = 1; o < WAVE; o <<= 1) x |= __shfl_xor(x, o, WAVE);
-    return x;
+    auto f = [](uint32_t v) {
+        v = wave_incl_scan_op(v, 0u, [](uint32_t a, uint32_t b) { return a | b; });
+        return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+    };
+    if constexpr (sizeof(U) == 8) return (U(f((uint32_t)(x >> 32))) << 32) | U(f((uint32_t)x));
+    else return (U)f((uint32_t)x);
 }
 
 template <typename U>
 __device__ __forceinline__ U wave_and(U x) {
-#pragma unroll
-    for (int o = 1; o < WAVE; o <<= 1) x &= __shfl_xor(x, o, WAVE);
-    return x;
+    auto f = [](uint32_t v) {
+        v = wave_incl_scan_op(v, ~0u, [](uint32_t a, uint32_t b) { return a & b; });
+        return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+    };
+    if constexpr (sizeof(U) == 8) return (U(f((uint32_t)(x >> 32))) << 32) | U(f((uint32_t)x));
+    else return (U)f((uint32_t)x);
 }
 
 template <typename U>

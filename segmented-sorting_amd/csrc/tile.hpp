@@ -35,6 +35,7 @@ struct RadixTile {
             uint16_t idx[HAS_V ? 2 : 1][HAS_V ? TILE : 1];  // sort(): source index per item, ping-pong
         };
         uint32_t wcnt[NW * RADIX];  // per-wave digit counts, then scatter bases
+        uint32_t dummy[NT];         // store target of the lanes that do not advance a count
         uint32_t dstart[RADIX];     // tile-exclusive start of each digit
         uint32_t goff[RADIX];       // global destination of each digit (partition)
         uint32_t wsum[NDW];
@@ -47,51 +48,66 @@ struct RadixTile {
     __device__ static __forceinline__ int active_waves(int n) { return (n + WITEMS - 1) / WITEMS; }
 
     // Lanes of this wave whose item has the same NBITS-bit digit d as the
-    // calling lane: for every bit, OR in (ballot of the bit) XOR (own bit
-    // replicated), then invert. v_bitop3 does the OR-XOR in one instruction
-    // per 32-lane half (truth table 0xF6 = src0 | (src1 ^ src2)).
+    // calling lane: for every bit, AND in NOT((ballot of the bit) XOR (own bit
+    // replicated)). v_bitop3 does that in one instruction per 32-lane half
+    // (truth table 0x90 = src0 & ~(src1 ^ src2)). The replicated bits are
+    // hidden from the optimiser so that each ballot compares one with zero
+    // (one instruction) instead of re-deriving the bit, and all bits are
+    // extracted and compared before they are combined, so that the compares'
+    // results are not read right after they are written.
     template <int NBITS>
     __device__ static __forceinline__ uint64_t match_digit(uint32_t d) {
-        uint32_t acc_lo = 0, acc_hi = 0;
+        uint32_t t[NBITS];
+        uint64_t bal[NBITS];
 #pragma unroll
         for (int b = 0; b < NBITS; ++b) {
-            const uint32_t t = (uint32_t)__builtin_amdgcn_sbfe((int)d, b, 1);
-            const uint64_t bal = __ballot(t != 0);
-            acc_lo = __builtin_amdgcn_bitop3_b32(acc_lo, (uint32_t)bal, t, 0xF6);
-            acc_hi = __builtin_amdgcn_bitop3_b32(acc_hi, (uint32_t)(bal >> 32), t, 0xF6);
+            t[b] = (uint32_t)__builtin_amdgcn_sbfe((int)d, b, 1);
+            __asm__("" : "+v"(t[b]));
         }
-        return ~((uint64_t(acc_hi) << 32) | uint64_t(acc_lo));
+#pragma unroll
+        for (int b = 0; b < NBITS; ++b) bal[b] = __ballot(t[b] != 0);
+        uint32_t acc_lo = ~0u, acc_hi = ~0u;
+#pragma unroll
+        for (int b = 0; b < NBITS; ++b) {
+            acc_lo = __builtin_amdgcn_bitop3_b32(acc_lo, (uint32_t)bal[b], t[b], 0x90);
+            acc_hi = __builtin_amdgcn_bitop3_b32(acc_hi, (uint32_t)(bal[b] >> 32), t[b], 0x90);
+        }
+        return (uint64_t(acc_hi) << 32) | uint64_t(acc_lo);
     }
 
     __device__ static __forceinline__ uint64_t match_digit_rt(uint32_t d, int bits) {
-        uint32_t acc_lo = 0, acc_hi = 0;
+        uint32_t acc_lo = ~0u, acc_hi = ~0u;
         for (int b = 0; b < bits; ++b) {
-            const uint32_t t = (uint32_t)__builtin_amdgcn_sbfe((int)d, b, 1);
+            uint32_t t = (uint32_t)__builtin_amdgcn_sbfe((int)d, b, 1);
+            __asm__("" : "+v"(t));
             const uint64_t bal = __ballot(t != 0);
-            acc_lo = __builtin_amdgcn_bitop3_b32(acc_lo, (uint32_t)bal, t, 0xF6);
-            acc_hi = __builtin_amdgcn_bitop3_b32(acc_hi, (uint32_t)(bal >> 32), t, 0xF6);
+            acc_lo = __builtin_amdgcn_bitop3_b32(acc_lo, (uint32_t)bal, t, 0x90);
+            acc_hi = __builtin_amdgcn_bitop3_b32(acc_hi, (uint32_t)(bal >> 32), t, 0x90);
         }
-        return ~((uint64_t(acc_hi) << 32) | uint64_t(acc_lo));
+        return (uint64_t(acc_hi) << 32) | uint64_t(acc_lo);
     }
 
-    // Per-wave counting of one item per lane: returns the running count of the
-    // digit in this wave plus the lanes below with the same digit. The lowest
-    // lane of each digit group advances the wave's counter with one LDS atomic
-    // (returning the old count) and the group reads it with a cross-lane
-    // permute, so successive items do not wait on each other's update.
-    __device__ static __forceinline__ uint32_t count_one(uint32_t* wc, uint32_t d, bool valid, uint64_t peers) {
-        peers &= __ballot(valid);
+    // Per-wave counting of one item per lane: every lane reads the wave's
+    // running count of its digit (lanes of one digit read the same word), and
+    // the lowest lane of each digit group stores the count advanced by the
+    // group's size; the other lanes store into a private dummy word, so the
+    // code has no branch. LDS operations of one wave complete in order, so the
+    // next item's read sees this item's store.
+    __device__ static __forceinline__ uint32_t count_one(uint32_t* wc, uint32_t* dummy, uint32_t d, uint64_t peers) {
         const uint32_t below = lanes_below(peers);
-        const int leader = __ffsll((long long)peers) - 1;
-        uint32_t old = 0;
-        if (valid && below == 0) old = atomicAdd(&wc[d], (uint32_t)__popcll(peers));
-        const uint32_t base = (uint32_t)__builtin_amdgcn_ds_bpermute(leader << 2, (int)old);
+        const uint32_t base = wc[d];
+        uint32_t* dst = below == 0 ? &wc[d] : dummy;
+        *dst = base + (uint32_t)__popcll(peers);
         return base + below;
     }
 
     // Stable rank of every valid item (position < n) by digit
     // (key >> shift) & (2^bits - 1): dst[j] is the item's tile position after a
-    // stable counting sort; sm.dstart[d] the first position of digit d.
+    // stable counting sort; sm.dstart[d] the first position of digit d. Slots
+    // at or past n hold the all-ones key and come after every valid item in
+    // rank order, so they are counted like any item without disturbing the
+    // ranks of valid items (they only add to the counts of the last digit, in
+    // the last active wave, after every valid item of that digit).
     __device__ static void rank_digits(Smem& sm, const U (&k)[IPT], int n, int shift, int bits,
                                        uint32_t (&dst)[IPT]) {
         const int t = threadIdx.x;
@@ -99,6 +115,7 @@ struct RadixTile {
         const uint32_t l = t & 63;
         const int wact = active_waves(n);
         uint32_t* wc = sm.wcnt + w * RADIX;
+        uint32_t* dummy = sm.dummy + t;
         const U dmask = U((U(1) << bits) - 1);
         if (w < wact) {
 #pragma unroll
@@ -107,13 +124,13 @@ struct RadixTile {
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
                     const uint32_t d = (uint32_t)((k[j] >> shift) & U(RADIX - 1));
-                    dst[j] = count_one(wc, d, pos(j) < n, match_digit<RADIX_BITS>(d));
+                    dst[j] = count_one(wc, dummy, d, match_digit<RADIX_BITS>(d));
                 }
             } else {
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
                     const uint32_t d = (uint32_t)((k[j] >> shift) & dmask);
-                    dst[j] = count_one(wc, d, pos(j) < n, match_digit_rt(d, bits));
+                    dst[j] = count_one(wc, dummy, d, match_digit_rt(d, bits));
                 }
             }
         }
@@ -249,7 +266,7 @@ struct RadixTile {
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
                     const int p = pos(j);
-                    k[j] = sm.keys[p < n ? p : 0];
+                    k[j] = p < n ? sm.keys[p] : Bits<U>::ALL;  // slots past n stay last
                 }
             }
         }

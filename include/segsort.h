@@ -117,10 +117,13 @@ int segsort_perm_scatter_f32(const float* d_grad_out, const int* d_perm,
 
 /* Tuning of the batched and plain forms (written by the autotuner).
  * wave_cap: ranges of 2..wave_cap items use the wave tier (2 <= wave_cap <= 512).
- * tile_enable: bit c enables LDS tile class c (classes 0..3, capacities given
- * by segsort_tile_caps); a disabled class's ranges go to the next enabled one,
- * and ranges above the largest enabled class go to the multi-workgroup MSD
- * tier (batched) or the one-workgroup-per-segment LSD (plain).
+ * tile_enable: bit c (c = 0..3) enables LDS tile class c (capacities given
+ * by segsort_tile_caps); bit 4 + g (g = 0..3) enables the wave-group class
+ * that sorts 64 / (4 << g) ranges of at most 4 << g items per wave. Ranges go
+ * to the smallest enabled class that holds them (groups, then the wave tier
+ * if wave_cap is larger, then the tiles); ranges above the largest enabled
+ * class go to the multi-workgroup MSD tier (batched) or the
+ * one-workgroup-per-segment LSD (plain). Default 0xEF.
  * grid_mult: persistent workgroups per CU per resident slot (1..8).
  * concurrent: 1 runs the tile classes on forked streams, 0 one after
  * another; honoured for direct launches (no graph replay, SEGSORT_GRAPH=0)

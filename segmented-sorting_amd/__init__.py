@@ -240,7 +240,7 @@ def segmented_sort(keys, offsets=None, values=None, *, begin=None, end=None, des
     return keys_out, values_out
 
 
-def set_tuning(key_width, value_bytes, wave_cap, tile_enable, grid_mult, concurrent=1):
+def set_tuning(key_width, value_bytes, wave_cap, tile_enable, grid_mult, concurrent=0):
     _check(load_library().segsort_set_tuning(key_width, value_bytes, wave_cap, tile_enable, grid_mult, concurrent),
            "set_tuning")
 

@@ -1,9 +1,11 @@
 """Autotuner of the batched segmented sort's tier cut-offs and tile classes.
 
 The library's tuning (per key width and value width) decides which tier sorts a
-range: the wave tier up to `wave_cap` items, then the enabled LDS tile classes
-(`tile_enable` bits, capacities from segsort_tile_caps), and the multi-workgroup
-MSD tier above the largest enabled class; `grid_mult` sizes the persistent grids.
+range: the enabled wave-group classes (`tile_enable` bits 4..7: 64/G ranges of at
+most G = 4, 8, 16, 32 items per wave), the wave tier up to `wave_cap` items, then
+the enabled LDS tile classes (`tile_enable` bits 0..3, capacities from
+segsort_tile_caps), and the multi-workgroup MSD tier above the largest enabled
+class; `grid_mult` sizes the persistent grids.
 (`concurrent`, forking the tile classes onto parallel streams, is only honoured
 for direct launches and is not searched.)
 
@@ -25,7 +27,7 @@ import sys
 from pathlib import Path
 
 TABLE = Path(__file__).resolve().parent / "tuning.json"
-DEFAULT = {"wave_cap": 256, "tile_enable": 0xF, "grid_mult": 1, "concurrent": 0}
+DEFAULT = {"wave_cap": 256, "tile_enable": 0xEF, "grid_mult": 1, "concurrent": 0}
 
 
 def signature(num_items: int, num_segments: int, key_width: int, value_bytes: int) -> str:
@@ -39,12 +41,17 @@ def _parse(sig: str):
     return head, int(n[1:]), int(m[1:])
 
 
+# LDS tile classes (bits 0..3) and wave-group classes (bits 4..7: groups of 4, 8, 16, 32 lanes)
+TILE_MASKS = (0xF, 0xE, 0xD, 0xC)
+GROUP_MASKS = (0x00, 0xE0, 0xA0, 0x60, 0xF0)
+
+
 def candidates(quick: bool = False):
-    wcs, tes = ((128, 256, 512), (0xF, 0xE, 0xD, 0xC, 0xB))
+    wcs, tms, gms = (128, 256, 512), TILE_MASKS, GROUP_MASKS
     if quick:
-        wcs, tes = (128, 512), (0xF, 0xC)
-    for wc, te, gm in itertools.product(wcs, tes, (1,) if quick else (1, 2)):
-        yield {"wave_cap": wc, "tile_enable": te, "grid_mult": gm, "concurrent": 0}
+        wcs, tms, gms = (128, 512), (0xF, 0xC), (0x00, 0xE0)
+    for wc, tm, gm_, mult in itertools.product(wcs, tms, gms, (1,) if quick else (1, 2)):
+        yield {"wave_cap": wc, "tile_enable": tm | gm_, "grid_mult": mult, "concurrent": 0}
 
 
 def load_table(path: Path = TABLE) -> dict:

@@ -111,6 +111,11 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
     return wave_incl_scan_op(x, 0u, [](uint32_t a, uint32_t b) { return a + b; });
 }
 
+// Sum of x over the wave, returned to every lane.
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan(x), 63);
+}
+
 // OR / AND of x over the wave, returned to every lane (lane 63 of the
 // inclusive scan holds the whole wave's value).
 template <typename U>

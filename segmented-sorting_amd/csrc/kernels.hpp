@@ -7,7 +7,7 @@
 
 namespace ssort {
 
-constexpr int MAX_TILE_CLASSES = 6;  // class 0 = wave tier, 1.. = LDS tiles
+constexpr int MAX_TILE_CLASSES = 9;  // wave-group classes, the wave tier, the LDS tiles
 constexpr int MAX_LEVELS = 8;        // MSD levels of the large tier (64-bit keys)
 
 // Device-resident counters of one call; cleared by k_init at the start.
@@ -68,21 +68,49 @@ __global__ void k_init(Counters* ctr) {
     for (int i = threadIdx.x; i < (int)(sizeof(Counters) / 4); i += blockDim.x) p[i] = 0;
 }
 
-// Append to a tile queue with one atomic per wave and class.
-__device__ __forceinline__ void push_tile(const Sched& sc, int cls, bool act, TileItem it) {
+// Queue slots taken by a whole workgroup at once: per-wave LDS counts, then
+// one global atomic per class for the slots (and one for the class's element
+// total), so that the few global queue counters see few atomics.
+struct BlockPush {
+    uint32_t cnt[MAX_TILE_CLASSES];
+    uint32_t base[MAX_TILE_CLASSES];
+    unsigned long long items[MAX_TILE_CLASSES];
+};
+
+// Every thread of the workgroup calls this (it holds barriers); threads with
+// cls < 0 queue nothing.
+__device__ __forceinline__ void push_tiles_block(const Sched& sc, BlockPush& bp, int cls, TileItem it) {
+    if (threadIdx.x < MAX_TILE_CLASSES) {
+        bp.cnt[threadIdx.x] = 0;
+        bp.items[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    uint32_t slot = 0;
+#pragma unroll
     for (int c = 0; c < MAX_TILE_CLASSES; ++c) {
-        const uint64_t m = __ballot(act && cls == c);
+        const uint64_t m = __ballot(cls == c);
         if (m == 0) continue;
+        const uint32_t items = wave_sum(cls == c ? (uint32_t)ti_n(it) : 0u);
         const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
         uint32_t base = 0;
-        if (lane_id() == leader) base = atomicAdd(&sc.ctr->tile_count[c], (uint32_t)__popcll(m));
-        base = (uint32_t)__shfl((int)base, (int)leader, WAVE);
-        const uint32_t slot = base + lanes_below(m);
-        if (act && cls == c && slot < sc.tq_cap[c]) sc.tq[c][slot] = it;
-        uint32_t items = (act && cls == c) ? (uint32_t)ti_n(it) : 0u;
-#pragma unroll
-        for (int o = 1; o < WAVE; o <<= 1) items += __shfl_xor(items, o, WAVE);
-        if (lane_id() == leader) atomicAdd(&sc.ctr->tile_items[c], (unsigned long long)items);
+        if (lane_id() == leader) {
+            base = atomicAdd(&bp.cnt[c], (uint32_t)__popcll(m));
+            atomicAdd(&bp.items[c], (unsigned long long)items);
+        }
+        base = (uint32_t)__builtin_amdgcn_readlane((int)base, (int)leader);
+        if (cls == c) slot = base + lanes_below(m);
+    }
+    __syncthreads();
+    if (threadIdx.x < MAX_TILE_CLASSES) {
+        const int c = threadIdx.x;
+        const uint32_t k = bp.cnt[c];
+        bp.base[c] = k ? atomicAdd(&sc.ctr->tile_count[c], k) : 0u;
+        if (k) atomicAdd(&sc.ctr->tile_items[c], bp.items[c]);
+    }
+    __syncthreads();
+    if (cls >= 0) {
+        const uint32_t g = bp.base[cls] + slot;
+        if (g < sc.tq_cap[cls]) sc.tq[cls][g] = it;
     }
 }
 
@@ -109,16 +137,17 @@ __device__ __forceinline__ void push_large(const Sched& sc, int level, int begin
 
 // Bin every segment by length: length 1 is copied here, the rest go to the
 // queue of their class (batched and plain forms) or to MSD level 0 (large).
-// Each block bins BIN_SPT * BIN_NT segments; slots are taken from LDS counters
+// Each block bins spt * BIN_NT segments (spt <= BIN_SPT, chosen by the host so
+// that the grid keeps a few hundred blocks); slots are taken from LDS counters
 // (one LDS atomic per wave and class) and then from the global counters with
 // one atomic per block and class, so the hot global counters see few atomics.
 constexpr int BIN_NT = 256;
-constexpr int BIN_SPT = 1;
+constexpr int BIN_SPT = 8;
 
 template <typename U, typename V, bool HAS_V>
 __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin, const int* __restrict__ d_end,
                                                 int num_segments, Bufs b, Sched sc, ClassCaps cc, int end_bit,
-                                                int large_to_level0) {
+                                                int large_to_level0, int spt) {
     __shared__ uint32_t lcount[MAX_TILE_CLASSES];
     __shared__ uint32_t lbase[MAX_TILE_CLASSES];
     __shared__ unsigned long long litems[MAX_TILE_CLASSES];
@@ -129,13 +158,13 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
     __syncthreads();
     int beg[BIN_SPT], n[BIN_SPT], cls[BIN_SPT];
     uint32_t slot[BIN_SPT];
-    const int s0 = blockIdx.x * (BIN_NT * BIN_SPT) + threadIdx.x;
+    const int s0 = blockIdx.x * (BIN_NT * spt) + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < BIN_SPT; ++j) {
         const int s = s0 + j * BIN_NT;
         beg[j] = 0;
         n[j] = 0;
-        if (s < num_segments) {
+        if (j < spt && s < num_segments) {
             beg[j] = d_begin[s];
             n[j] = d_end[s] - beg[j];
             if (n[j] < 0) n[j] = 0;
@@ -150,15 +179,13 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
             const uint64_t m = __ballot(cls[j] == c);
             if (m == 0) continue;
             const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
-            uint32_t items = cls[j] == c ? (uint32_t)n[j] : 0u;
-#pragma unroll
-            for (int o = 1; o < WAVE; o <<= 1) items += __shfl_xor(items, o, WAVE);
+            const uint32_t items = wave_sum(cls[j] == c ? (uint32_t)n[j] : 0u);
             uint32_t base = 0;
             if (lane_id() == leader) {
                 base = atomicAdd(&lcount[c], (uint32_t)__popcll(m));
                 atomicAdd(&litems[c], (unsigned long long)items);
             }
-            base = (uint32_t)__shfl((int)base, (int)leader, WAVE);
+            base = (uint32_t)__builtin_amdgcn_readlane((int)base, (int)leader);
             if (cls[j] == c) slot[j] = base + lanes_below(m);
         }
         if (cls[j] == -1 && large_to_level0) push_large(sc, 0, beg[j], n[j], BUF_IN, end_bit);
@@ -229,6 +256,24 @@ __global__ __launch_bounds__(256) void k_wave_tier(Bufs b, Sched sc, int cls, Ke
         wave_sort<U, V, HAS_V>((const U*)b.k[src], (const V*)b.v[src], buf_ptr<U>(b.k[BUF_OUT]),
                                buf_ptr<V>(b.v[BUF_OUT]), it.begin, ti_n(it), km, rank_max);
         if (!persistent) break;
+    }
+}
+
+// Wave-group tier: each wave sorts 64/G queued ranges of n <= G items at once,
+// one range per group of G lanes (a bitonic network confined to the group).
+template <int G, typename U, typename V, bool HAS_V>
+__global__ __launch_bounds__(256) void k_group_tier(Bufs b, Sched sc, int cls, KeyMode km) {
+    constexpr int PER_WAVE = WAVE / G;
+    const int waves_per_block = blockDim.x / WAVE;
+    const int wid = blockIdx.x * waves_per_block + (threadIdx.x >> 6);
+    const int nwaves = gridDim.x * waves_per_block;
+    const int count = tile_count(sc, cls);
+    const int g = (int)(lane_id() / G);
+    for (int base = wid * PER_WAVE; base < count; base += nwaves * PER_WAVE) {
+        const int i = base + g;
+        TileItem it = make_ti(0, 0, BUF_IN);
+        if (i < count) it = sc.tq[cls][i];
+        group_sort<G, U, V, HAS_V>(b, it.begin, i < count ? ti_n(it) : 0, ti_src(it), km);
     }
 }
 
@@ -304,6 +349,7 @@ __global__ __launch_bounds__(SCAN_NT) void k_large_scan(Sched sc, int level, Key
                                                         int tile_max) {
     __shared__ uint32_t part[SCAN_G][MAX_LEVELS][RADIX];
     __shared__ uint32_t base_s[RADIX];
+    __shared__ BlockPush bp;
     __shared__ uint32_t nz[MAX_LEVELS];
     __shared__ uint32_t wsum[RADIX / WAVE];
     const int d = threadIdx.x & (RADIX - 1);
@@ -385,11 +431,11 @@ __global__ __launch_bounds__(SCAN_NT) void k_large_scan(Sched sc, int level, Key
             sc.off[gc * RADIX + d] = run;
             run += sc.hist[gc * sc.npos_max * RADIX + pstar * RADIX + d];
         }
-        if (g == 0 && !final_) {
+        {
             const int cb = it.begin + (int)base;
-            const int cn = (int)mine;
+            const int cn = (g == 0 && !final_) ? (int)mine : 0;
             const int cls = (cn >= 1 && cn <= tile_max) ? class_of(cc, cn) : -1;
-            push_tile(sc, cls, cls >= 0, make_ti(cb, cls >= 0 ? cn : 0, child_buf));
+            push_tiles_block(sc, bp, cls, make_ti(cb, cls >= 0 ? cn : 0, child_buf));
             if (cn > tile_max) push_large(sc, level + 1, cb, cn, child_buf, shift);
         }
         if (threadIdx.x == 0) {
@@ -460,6 +506,7 @@ __global__ __launch_bounds__(NT) void k_large_local(Bufs b, Sched sc, int level,
     __shared__ int fb[NPOS + 1], fn[NPOS + 1], fsrc[NPOS + 1], fhi[NPOS + 1], fdst[NPOS + 1], fshift[NPOS + 1],
         fnext[NPOS + 1];
     __shared__ int sel;
+    __shared__ BlockPush bp;
     const int t = threadIdx.x;
     int count, chunks;
     level_counts(sc, level, count, chunks);
@@ -547,10 +594,11 @@ __global__ __launch_bounds__(NT) void k_large_local(Bufs b, Sched sc, int level,
                     T::partition_run(u.sm, ks, vs, buf_ptr<U>(b.k[dst]), buf_ptr<V>(b.v[dst]), (int64_t)beg + c0,
                                      cn, km, shift, bits, run);
                 }
-                if (!final_ && t < RADIX) {
-                    const uint32_t cb = cstart[depth][t], cn = cstart[depth][t + 1] - cb;
+                if (!final_) {
+                    const uint32_t cb = t < RADIX ? cstart[depth][t] : 0u;
+                    const uint32_t cn = t < RADIX ? cstart[depth][t + 1] - cb : 0u;
                     const int cls = (cn >= 1 && (int)cn <= tile_max) ? class_of(cc, (int)cn) : -1;
-                    push_tile(sc, cls, cls >= 0, make_ti(beg + (int)cb, cls >= 0 ? (int)cn : 0, dst));
+                    push_tiles_block(sc, bp, cls, make_ti(beg + (int)cb, cls >= 0 ? (int)cn : 0, dst));
                 }
                 __syncthreads();  // the scatter's writes are visible to the workgroup before a child is read
                 fresh = false;

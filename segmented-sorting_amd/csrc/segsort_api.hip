@@ -31,7 +31,9 @@ int ok() {
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-int bin_grid(int S) { return (S + BIN_NT * BIN_SPT - 1) / (BIN_NT * BIN_SPT); }
+// Segments per binning thread: one while that keeps >= 256 blocks, at most BIN_SPT.
+int bin_spt(int S) { return std::min(BIN_SPT, std::max(1, (S + BIN_NT * 256 - 1) / (BIN_NT * 256))); }
+int bin_grid(int S) { return (S + BIN_NT * bin_spt(S) - 1) / (BIN_NT * bin_spt(S)); }
 
 // Ranges up to this length use the wave's compare-count rank (about 4n VALU
 // operations per item); longer ones the register bitonic network.
@@ -41,13 +43,14 @@ constexpr int WAVE_RANK_MAX = 24;
 
 struct Tuning {
     int wave_cap;     // ranges of 2..wave_cap items go to the wave tier
-    int tile_enable;  // bit i: LDS tile class i on; ranges above the largest enabled class use the MSD tier
+    int tile_enable;  // bit i < 4: LDS tile class i on; bit 4 + g: wave-group class of 4 << g items on
     int grid_mult;    // persistent workgroups per resident slot
     int concurrent;   // tile classes run on forked streams instead of one after another (direct launches only)
 };
 
 // index: (key_width == 8) * 3 + value-bytes slot (0, 4, 8 -> 0, 1, 2)
-Tuning g_tuning[6] = {{256, 0xF, 1, 0}, {256, 0xF, 1, 0}, {256, 0xF, 1, 0}, {256, 0xF, 1, 0}, {256, 0xF, 1, 0}, {256, 0xF, 1, 0}};
+Tuning g_tuning[6] = {{256, 0xEF, 1, 0}, {256, 0xEF, 1, 0}, {256, 0xEF, 1, 0},
+                      {256, 0xEF, 1, 0}, {256, 0xEF, 1, 0}, {256, 0xEF, 1, 0}};
 std::mutex g_tuning_mu;
 
 int tuning_slot(int key_width, int value_bytes) {
@@ -239,26 +242,42 @@ struct Cfg {
     static constexpr int NPOS = int(sizeof(U));  // 8-bit digit positions of a key
 };
 
+constexpr int KIND_WAVE = 0;   // one range per wave
+constexpr int KIND_TILE = 1;   // + i: LDS tile class i (0..3)
+constexpr int KIND_GROUP = 5;  // + g: 64 / (4 << g) ranges of <= 4 << g items per wave (g = 0..3)
+
 struct Plan {
     ClassCaps cc;
-    int kind[MAX_TILE_CLASSES];  // 0 = wave tier, 1 + i = LDS tile class i
+    int kind[MAX_TILE_CLASSES];
     int nclass;
     int tile_max;
 };
 
+// Classes in increasing capacity: the enabled wave-group classes, the wave
+// tier (if its cap is above them), then the enabled LDS tile classes.
 template <typename C>
 Plan make_plan(const Tuning& tu) {
     Plan p;
     p.nclass = 0;
-    int wave_cap = std::min(512, std::max(2, tu.wave_cap));
-    p.cc.cap[p.nclass] = wave_cap;
-    p.kind[p.nclass++] = 0;
+    int last = 1;
+    for (int g = 0; g < 4; ++g) {
+        if (!(tu.tile_enable & (0x10 << g))) continue;
+        p.cc.cap[p.nclass] = 4 << g;
+        p.kind[p.nclass++] = KIND_GROUP + g;
+        last = 4 << g;
+    }
+    const int wave_cap = std::min(512, std::max(2, tu.wave_cap));
+    if (wave_cap > last) {
+        p.cc.cap[p.nclass] = wave_cap;
+        p.kind[p.nclass++] = KIND_WAVE;
+        last = wave_cap;
+    }
     const int enable = tu.tile_enable & 0xF;
     for (int i = 0; i < 4; ++i) {
         if (!(enable & (1 << i))) continue;
-        if (C::CAP[i] <= wave_cap) continue;
+        if (C::CAP[i] <= last) continue;
         p.cc.cap[p.nclass] = C::CAP[i];
-        p.kind[p.nclass++] = 1 + i;
+        p.kind[p.nclass++] = KIND_TILE + i;
     }
     p.cc.nclass = p.nclass;
     for (int c = p.nclass; c < MAX_TILE_CLASSES; ++c) p.cc.cap[c] = 0;
@@ -287,9 +306,11 @@ Layout make_layout(const Plan& p, int num_items, int num_segments, int levels, b
     const size_t N = (size_t)num_items, S = (size_t)num_segments;
     L.lq_cap = N / (size_t)(p.tile_max + 1) + 1;
     L.chunks = N / (size_t)C::TILE_MAX + L.lq_cap + 1;
-    // every partition (level 0 or deeper) queues at most RADIX tile children, and
-    // the ranges partitioned at one depth are disjoint and larger than a tile
-    const size_t bucket_bound = plain ? 0 : (size_t)RADIX * L.lq_cap * (size_t)C::NPOS;
+    // every partition (level 0 or deeper) queues at most RADIX tile children; the
+    // ranges partitioned at one depth are disjoint and larger than a tile, and
+    // their children are disjoint non-empty ranges (at most N of them)
+    const size_t bucket_bound =
+        plain ? 0 : std::min((size_t)RADIX * L.lq_cap, std::max<size_t>(N, 1)) * (size_t)C::NPOS;
     L.ctr = take(sizeof(Counters));
     for (int c = 0; c < p.nclass; ++c) {
         const size_t lo = c == 0 ? 1 : (size_t)p.cc.cap[c - 1] + 1;
@@ -333,6 +354,18 @@ struct Launcher {
         if (g > 0) SEGSORT_LAUNCH(name, st, cls, PAIR_BYTES, 0.0, (k<<<g, NT, 0, st>>>(b, sc, cls, km, persistent)));
     }
 
+    // Persistent grid, or (plain form) enough 256-thread blocks for `grid` ranges.
+    template <int G>
+    static void launch_group(Bufs b, Sched sc, int cls, KeyMode km, hipStream_t st, int grid, bool persistent,
+                             int grid_mult) {
+        auto k = k_group_tier<G, U, V, HAS_V>;
+        constexpr int PER_BLOCK = 4 * (WAVE / G);
+        const int g = persistent ? resident_blocks(k, 256) * grid_mult : (grid + PER_BLOCK - 1) / PER_BLOCK;
+        char name[32];
+        snprintf(name, sizeof(name), "group_%d", G);
+        if (g > 0) SEGSORT_LAUNCH(name, st, cls, PAIR_BYTES, 0.0, (k<<<g, 256, 0, st>>>(b, sc, cls, km)));
+    }
+
     static void launch_class(int kind, Bufs b, Sched sc, int cls, KeyMode km, hipStream_t st, int grid, bool persistent,
                              int grid_mult) {
         switch (kind) {
@@ -348,6 +381,10 @@ struct Launcher {
         case 2: launch_tile<C::NT1, C::IPT1>(b, sc, cls, km, st, grid, persistent, grid_mult); break;
         case 3: launch_tile<C::NT2, C::IPT2>(b, sc, cls, km, st, grid, persistent, grid_mult); break;
         case 4: launch_tile<C::NT3, C::IPT3>(b, sc, cls, km, st, grid, persistent, grid_mult); break;
+        case KIND_GROUP + 0: launch_group<4>(b, sc, cls, km, st, grid, persistent, grid_mult); break;
+        case KIND_GROUP + 1: launch_group<8>(b, sc, cls, km, st, grid, persistent, grid_mult); break;
+        case KIND_GROUP + 2: launch_group<16>(b, sc, cls, km, st, grid, persistent, grid_mult); break;
+        case KIND_GROUP + 3: launch_group<32>(b, sc, cls, km, st, grid, persistent, grid_mult); break;
         }
     }
 
@@ -409,7 +446,7 @@ struct Launcher {
         const size_t mark = Profiler::get().mark();
         SEGSORT_LAUNCH("init", st, -1, 0.0, 0.0, (k_init<<<1, 64, 0, st>>>(sc.ctr)));
         SEGSORT_LAUNCH("bin", st, -1, 0.0, 8.0 * S,
-                       (k_bin<U, V, HAS_V><<<bin_grid(S), BIN_NT, 0, st>>>(db, de, S, b, sc, p.cc, km.end_bit, 1)));
+                       (k_bin<U, V, HAS_V><<<bin_grid(S), BIN_NT, 0, st>>>(db, de, S, b, sc, p.cc, km.end_bit, 1, bin_spt(S))));
         if (levels > 0) {
             auto kh = k_large_hist<C::NT3, C::IPT3, U, V, HAS_V>;
             auto ks = k_large_scatter<C::NT3, C::IPT3, U, V, HAS_V>;
@@ -492,7 +529,7 @@ struct Launcher {
         const size_t mark = Profiler::get().mark();
         SEGSORT_LAUNCH("init", st, -1, 0.0, 0.0, (k_init<<<1, 64, 0, st>>>(sc.ctr)));
         SEGSORT_LAUNCH("bin", st, -1, 0.0, 8.0 * S,
-                       (k_bin<U, V, HAS_V><<<bin_grid(S), BIN_NT, 0, st>>>(db, de, S, b, sc, p.cc, km.end_bit, 0)));
+                       (k_bin<U, V, HAS_V><<<bin_grid(S), BIN_NT, 0, st>>>(db, de, S, b, sc, p.cc, km.end_bit, 0, bin_spt(S))));
         SEGSORT_LAUNCH("bin_large", st, -1, 0.0, 8.0 * S,
                        (k_bin_large_ids<<<(S + 255) / 256, 256, 0, st>>>(db, de, S, sc, p.cc)));
         Counters h;
@@ -691,7 +728,7 @@ int segsort_perm_scatter_f32(const float* d_grad_out, const int* d_perm, const i
 
 int segsort_set_tuning(int key_width, int value_bytes, int wave_cap, int tile_enable, int grid_mult, int concurrent) {
     const int s = tuning_slot(key_width, value_bytes);
-    if (s < 0 || wave_cap < 2 || wave_cap > 512 || tile_enable < 0 || tile_enable > 0xF || grid_mult < 1 ||
+    if (s < 0 || wave_cap < 2 || wave_cap > 512 || tile_enable < 0 || tile_enable > 0xFF || grid_mult < 1 ||
         grid_mult > 8 || concurrent < 0 || concurrent > 1)
         return fail(hipErrorInvalidValue, "segsort_set_tuning: out of range");
     std::lock_guard<std::mutex> lk(g_tuning_mu);

@@ -135,34 +135,61 @@ struct RadixTile {
             }
         }
         __syncthreads();
-        uint32_t tot[DPT];
-        uint32_t run = 0;
-        if (t < NDT) {
+        if constexpr (NW <= 4) {
+            // one wave scans the counts: lane l takes digits l, l+64, l+128, l+192
+            if (w == 0) {
+                uint32_t carry = 0;
 #pragma unroll
-            for (int i = 0; i < DPT; ++i) {
-                const int d = t * DPT + i;
-                uint32_t s = 0;
-                for (int ww = 0; ww < wact; ++ww) {
-                    const uint32_t c = sm.wcnt[ww * RADIX + d];
-                    sm.wcnt[ww * RADIX + d] = s;
-                    s += c;
+                for (int i = 0; i < RADIX / WAVE; ++i) {
+                    const int d = i * WAVE + (int)l;
+                    uint32_t c[NW];
+                    uint32_t s = 0;
+#pragma unroll
+                    for (int ww = 0; ww < NW; ++ww) {
+                        c[ww] = ww < wact ? sm.wcnt[ww * RADIX + d] : 0u;
+                        s += c[ww];
+                    }
+                    const uint32_t incl = wave_incl_scan(s);
+                    uint32_t base = carry + incl - s;
+                    carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+                    sm.dstart[d] = base;
+#pragma unroll
+                    for (int ww = 0; ww < NW; ++ww) {
+                        if (ww < wact) sm.wcnt[ww * RADIX + d] = base;
+                        base += c[ww];
+                    }
                 }
-                tot[i] = run;
-                run += s;
             }
-        }
-        const uint32_t incl = wave_incl_scan(run);
-        if (t < NDT && l == 63) sm.wsum[w] = incl;
-        __syncthreads();
-        if (t < NDT) {
-            uint32_t excl = incl - run;
-            for (int ww = 0; ww < w; ++ww) excl += sm.wsum[ww];
+        } else {
+            uint32_t tot[DPT];
+            uint32_t run = 0;
+            if (t < NDT) {
 #pragma unroll
-            for (int i = 0; i < DPT; ++i) {
-                const int d = t * DPT + i;
-                const uint32_t base = excl + tot[i];
-                sm.dstart[d] = base;
-                for (int ww = 0; ww < wact; ++ww) sm.wcnt[ww * RADIX + d] += base;
+                for (int i = 0; i < DPT; ++i) {
+                    const int d = t * DPT + i;
+                    uint32_t s = 0;
+                    for (int ww = 0; ww < wact; ++ww) {
+                        const uint32_t c = sm.wcnt[ww * RADIX + d];
+                        sm.wcnt[ww * RADIX + d] = s;
+                        s += c;
+                    }
+                    tot[i] = run;
+                    run += s;
+                }
+            }
+            const uint32_t incl = wave_incl_scan(run);
+            if (t < NDT && l == 63) sm.wsum[w] = incl;
+            __syncthreads();
+            if (t < NDT) {
+                uint32_t excl = incl - run;
+                for (int ww = 0; ww < w; ++ww) excl += sm.wsum[ww];
+#pragma unroll
+                for (int i = 0; i < DPT; ++i) {
+                    const int d = t * DPT + i;
+                    const uint32_t base = excl + tot[i];
+                    sm.dstart[d] = base;
+                    for (int ww = 0; ww < wact; ++ww) sm.wcnt[ww * RADIX + d] += base;
+                }
             }
         }
         __syncthreads();
@@ -270,13 +297,52 @@ struct RadixTile {
                 }
             }
         }
-        if (act) {
+        if constexpr (HAS_V && sizeof(V) <= sizeof(U)) {
+            // Values: one coalesced read of the range into the key area of LDS
+            // (free once the sorted keys are out), then the permutation is
+            // applied from LDS. A gather straight from global memory would pull
+            // a whole cache line for every value.
+            V v[IPT];
+            uint32_t srcs[IPT];
+            if (act) {
 #pragma unroll
-            for (int j = 0; j < IPT; ++j) {
-                const int p = pos(j);
-                if (p < n) {
-                    ko[begin + p] = twiddle_out<U>(sm.keys[p], km.kind, km.desc);
-                    if constexpr (HAS_V) vo[begin + p] = vs[begin + sm.idx[cur][p]];
+                for (int j = 0; j < IPT; ++j) {
+                    const int p = pos(j);
+                    if (p < n) v[j] = vs[begin + p];
+                }
+#pragma unroll
+                for (int j = 0; j < IPT; ++j) {
+                    const int p = pos(j);
+                    if (p < n) {
+                        srcs[j] = sm.idx[cur][p];
+                        ko[begin + p] = twiddle_out<U>(sm.keys[p], km.kind, km.desc);
+                    }
+                }
+            }
+            __syncthreads();
+            V* stage = reinterpret_cast<V*>(sm.keys);
+            if (act) {
+#pragma unroll
+                for (int j = 0; j < IPT; ++j)
+                    if (pos(j) < n) stage[pos(j)] = v[j];
+            }
+            __syncthreads();
+            if (act) {
+#pragma unroll
+                for (int j = 0; j < IPT; ++j) {
+                    const int p = pos(j);
+                    if (p < n) vo[begin + p] = stage[srcs[j]];
+                }
+            }
+        } else {
+            if (act) {
+#pragma unroll
+                for (int j = 0; j < IPT; ++j) {
+                    const int p = pos(j);
+                    if (p < n) {
+                        ko[begin + p] = twiddle_out<U>(sm.keys[p], km.kind, km.desc);
+                        if constexpr (HAS_V) vo[begin + p] = vs[begin + sm.idx[cur][p]];
+                    }
                 }
             }
         }
@@ -353,15 +419,38 @@ struct RadixTile {
     // Digit histograms of the n <= TILE items at ks[begin..) for `npos` digit
     // positions pos_p = pos0 + 8p (the last may be narrower than 8 bits, up to
     // hi_bit); counts are accumulated into the LDS array hist[npos][RADIX].
+    // Keys are read a tile at a time, all IPT loads of a thread issued before
+    // any is counted. A digit that is the same in every active lane (a
+    // constant high digit of small or duplicate-heavy keys) is added once by
+    // one lane instead of by every lane into the same LDS word.
     __device__ static void histogram(uint32_t* hist, const U* ks, int64_t begin, int n, const KeyMode& km,
                                      int pos0, int npos, int hi_bit) {
-        for (int i = threadIdx.x; i < n; i += NT) {
-            const U key = twiddle_in<U>(ks[begin + i], km.kind, km.desc);
-            for (int p = 0; p < npos; ++p) {
-                const int sh = pos0 + p * RADIX_BITS;
-                const int nb = (hi_bit - sh) < RADIX_BITS ? (hi_bit - sh) : RADIX_BITS;
-                const uint32_t d = (uint32_t)((key >> sh) & U((U(1) << nb) - 1));
-                atomicAdd(&hist[p * RADIX + d], 1u);
+        const int t = threadIdx.x;
+        for (int c0 = 0; c0 < n; c0 += TILE) {
+            U key[IPT];
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int i = c0 + j * NT + t;
+                key[j] = i < n ? ks[begin + i] : U(0);
+            }
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int i = c0 + j * NT + t;
+                if (c0 + j * NT + (t & ~63) >= n) break;  // this and later rows: whole wave past the end
+                const bool v = i < n;
+                const U k = twiddle_in<U>(key[j], km.kind, km.desc);
+                const uint64_t act = __ballot(v);
+                for (int p = 0; p < npos; ++p) {
+                    const int sh = pos0 + p * RADIX_BITS;
+                    const int nb = (hi_bit - sh) < RADIX_BITS ? (hi_bit - sh) : RADIX_BITS;
+                    const uint32_t d = (uint32_t)((k >> sh) & U((U(1) << nb) - 1));
+                    const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);
+                    if (__ballot(v && d != d0) == 0) {
+                        if ((t & 63) == 0) atomicAdd(&hist[p * RADIX + d0], (uint32_t)__popcll(act));
+                    } else if (v) {
+                        atomicAdd(&hist[p * RADIX + d], 1u);
+                    }
+                }
             }
         }
     }
@@ -389,6 +478,9 @@ struct SortWord<uint32_t> {
     __device__ __forceinline__ SortWord map(F f) const {
         return {(uint64_t(f((uint32_t)(w >> 32))) << 32) | uint64_t(f((uint32_t)w))};
     }
+    __device__ static __forceinline__ SortWord select(bool c, const SortWord& a, const SortWord& b) {
+        return {c ? a.w : b.w};
+    }
 };
 
 template <>
@@ -403,6 +495,10 @@ struct SortWord<uint64_t> {
     template <typename F>
     __device__ __forceinline__ SortWord map(F f) const {
         return {(uint64_t(f((uint32_t)(k >> 32))) << 32) | uint64_t(f((uint32_t)k)), f(p)};
+    }
+    // field by field: a select of the whole struct goes through scratch memory
+    __device__ static __forceinline__ SortWord select(bool c, const SortWord& a, const SortWord& b) {
+        return {c ? a.k : b.k, c ? a.p : b.p};
     }
 };
 
@@ -419,8 +515,8 @@ __device__ __forceinline__ uint32_t xor_lane(uint32_t v) {
 template <typename W>
 __device__ __forceinline__ void compare_exchange(W& a, W& b, bool ascending) {
     const bool swap = ascending ? b.lt(a) : a.lt(b);
-    const W x = swap ? b : a;
-    const W y = swap ? a : b;
+    const W x = W::select(swap, b, a);
+    const W y = W::select(swap, a, b);
     a = x;
     b = y;
 }
@@ -438,7 +534,7 @@ __device__ __forceinline__ void bitonic_cross(W (&w)[R], int k) {
         const bool asc = (((int)l * R + r) & k) == 0;
         const bool keep_small = lower == asc;
         const bool o_smaller = o.lt(w[r]);
-        w[r] = (keep_small == o_smaller) ? o : w[r];
+        w[r] = W::select(keep_small == o_smaller, o, w[r]);
     }
 }
 
@@ -469,6 +565,53 @@ __device__ __forceinline__ void bitonic_wave(W (&w)[R]) {
                 else bitonic_cross<R, 32>(w, k);
             }
         }
+    }
+}
+
+// Bitonic sort of the G words held one per lane by each aligned group of G
+// lanes (G <= 32): every exchange distance is below G, so groups never mix,
+// and the direction bit is taken from the lane's index inside its group.
+template <int G, typename W>
+__device__ __forceinline__ void bitonic_group(W& w) {
+    const uint32_t i = lane_id() & (G - 1);
+#pragma unroll
+    for (int k = 2; k <= G; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            W o;
+            if (j == 1) o = w.map([](uint32_t x) { return xor_lane<1>(x); });
+            else if (j == 2) o = w.map([](uint32_t x) { return xor_lane<2>(x); });
+            else if (j == 4) o = w.map([](uint32_t x) { return xor_lane<4>(x); });
+            else if (j == 8) o = w.map([](uint32_t x) { return xor_lane<8>(x); });
+            else o = w.map([](uint32_t x) { return xor_lane<16>(x); });
+            const bool asc = (i & (uint32_t)k) == 0;
+            const bool lower = (i & (uint32_t)j) == 0;
+            const bool keep_small = lower == asc;
+            w = W::select(keep_small == o.lt(w), o, w);
+        }
+    }
+}
+
+// Each aligned group of G lanes sorts its own range [begin, begin+n) of
+// buffer src (n <= G; n == 0 for a group without a range): lane i of the
+// group loads item i, the group's network sorts (order key, i) words, and
+// lane i writes the i-th smallest.
+template <int G, typename U, typename V, bool HAS_V>
+__device__ __forceinline__ void group_sort(const Bufs& b, int64_t begin, int n, int src, const KeyMode& km) {
+    using W = SortWord<U>;
+    const int i = (int)(lane_id() & (G - 1));
+    const U* ks = (const U*)b.k[src];
+    const V* vs = (const V*)b.v[src];
+    U* ko = buf_ptr<U>(b.k[BUF_OUT]);
+    V* vo = buf_ptr<V>(b.v[BUF_OUT]);
+    const U rm = range_mask<U>(km.begin_bit, km.end_bit);
+    W w = W::inf();
+    if (i < n) w = W::make(U(twiddle_in<U>(ks[begin + i], km.kind, km.desc) & rm), (uint32_t)i);
+    bitonic_group<G>(w);
+    if (i < n) {
+        const uint32_t s = w.pos();
+        ko[begin + i] = rm == Bits<U>::ALL ? twiddle_out<U>(U(w.key()), km.kind, km.desc) : ks[begin + s];
+        if constexpr (HAS_V) vo[begin + i] = vs[begin + s];
     }
 }
 

@@ -85,6 +85,10 @@ def test_tuning_table(pkg):
         pkg.set_tuning(4, 4, 513, 0xF, 1)
     with pytest.raises(pkg.SegsortError):
         pkg.set_tuning(4, 4, 32, 0xF, 1, 2)
+    with pytest.raises(pkg.SegsortError):
+        pkg.set_tuning(4, 4, 32, 0x1FF, 1)
+    pkg.set_tuning(4, 4, 64, 0xFF, 1)  # every wave-group and tile class
+    assert pkg.get_tuning(4, 4)["tile_enable"] == 0xFF
     pkg.set_tuning(4, 4, **before)
     caps = pkg.tile_caps(4, 4)
     assert caps == sorted(caps) and caps[-1] >= 8192

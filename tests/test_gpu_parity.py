@@ -194,3 +194,38 @@ def test_edge_sizes(pkg, torch_cuda, orc):
         n = int(off[-1])
         keys = pkg.gen.keys_u32(len(lengths) + n, 1, n)
         check(pkg, torch_cuda, orc, keys, np.arange(n, dtype=np.uint32), off)
+
+
+TUNINGS = [
+    {"wave_cap": 256, "tile_enable": 0x1F, "grid_mult": 1},  # groups of 4 only
+    {"wave_cap": 256, "tile_enable": 0x2F, "grid_mult": 1},  # groups of 8
+    {"wave_cap": 256, "tile_enable": 0x4F, "grid_mult": 2},  # groups of 16
+    {"wave_cap": 512, "tile_enable": 0x8F, "grid_mult": 1},  # groups of 32
+    {"wave_cap": 2, "tile_enable": 0xFF, "grid_mult": 1},    # every group class, no wave tier
+    {"wave_cap": 64, "tile_enable": 0xF0, "grid_mult": 1},   # no LDS tiles: everything above 64 is MSD
+    {"wave_cap": 2, "tile_enable": 0x00, "grid_mult": 1},    # nothing but the MSD tiers above 2 items
+]
+
+
+@pytest.mark.parametrize("tuning", TUNINGS, ids=lambda t: f"wc{t['wave_cap']}-te{t['tile_enable']:02x}")
+def test_tuning_variants(pkg, torch_cuda, orc, tuning):
+    """Every class layout the tuning can select sorts bit-exactly (batched and plain),
+    for 32- and 64-bit keys, on lengths that hit every group, wave, tile and MSD path."""
+    rng = np.random.default_rng(31)
+    lengths = np.concatenate([rng.integers(0, 40, 3000), rng.integers(40, 600, 200), rng.integers(600, 20000, 20),
+                              [0, 1, 2, 3, 4, 5, 8, 9, 16, 17, 32, 33, 64, 65, 70000]])
+    rng.shuffle(lengths)
+    off = pkg.gen.offsets_from_lengths(lengths)
+    n = int(off[-1])
+    raw = pkg.gen.u64(31, 1, n)
+    befores = {(kw, vb): pkg.get_tuning(kw, vb) for kw, vb in ((4, 4), (8, 4), (4, 0))}
+    try:
+        for kw, vb in befores:
+            pkg.set_tuning(kw, vb, tuning["wave_cap"], tuning["tile_enable"], tuning["grid_mult"], 0)
+        k32 = (raw >> np.uint64(40)).astype(np.uint32)  # 24-bit keys: ties inside segments
+        check(pkg, torch_cuda, orc, k32, np.arange(n, dtype=np.uint32), off, algos=("batched", "plain"))
+        check(pkg, torch_cuda, orc, k32.view(np.float32), None, off, algos=("batched", "plain"))
+        check(pkg, torch_cuda, orc, raw, np.arange(n, dtype=np.uint32), off, algos=("batched", "plain"))
+    finally:
+        for (kw, vb), t in befores.items():
+            pkg.set_tuning(kw, vb, **t)

@@ -42,7 +42,7 @@ def _parse(sig: str):
 
 
 # LDS tile classes (bits 0..3) and wave-group classes (bits 4..7: groups of 4, 8, 16, 32 lanes)
-TILE_MASKS = (0xF, 0xE, 0xD, 0xC)
+TILE_MASKS = (0xF, 0xE, 0xD, 0xC, 0x7, 0x6)
 GROUP_MASKS = (0x00, 0xE0, 0xA0, 0x60, 0xF0)
 
 
@@ -115,10 +115,10 @@ def _same(torch, a, b) -> bool:
     return torch.equal(a.view(torch.uint8), b.view(torch.uint8))
 
 
-def tune(pkg, torch, keys, vals, offsets, steps=5, warmup=2, verbose=True, quick=False):
-    """Time every candidate on device-resident (keys, vals, offsets) tensors; return
-    (best tuning, {candidate: ms}). Candidates whose output differs from the default
-    tuning's are rejected (they would be a bug)."""
+def tune(pkg, torch, keys, vals, offsets, steps=5, warmup=2, verbose=True, quick=False, cands=None):
+    """Time every candidate (default: all of `candidates(quick)`) on device-resident
+    (keys, vals, offsets) tensors; return (best tuning, {candidate: ms}). Candidates
+    whose output differs from the default tuning's are rejected (they would be a bug)."""
     kt = pkg.key_type_of(keys.dtype)
     kw = keys.element_size()
     vb = 0 if vals is None else vals.element_size()
@@ -132,7 +132,7 @@ def tune(pkg, torch, keys, vals, offsets, steps=5, warmup=2, verbose=True, quick
     ref_k, ref_v = kout.clone(), (vout.clone() if vout is not None else None)
     results, best = {}, None
     try:
-        for c in candidates(quick):
+        for c in (cands if cands is not None else candidates(quick)):
             pkg.set_tuning(kw, vb, **c)
             ms = _time(pkg, torch, args, steps, warmup)
             same = _same(torch, kout, ref_k) and (vout is None or _same(torch, vout, ref_v))
@@ -153,7 +153,23 @@ def store(sig: str, tuning: dict, ms: float, path: Path = TABLE, note: str = "")
     path.write_text(json.dumps(table, indent=1, sort_keys=True) + "\n")
 
 
+def _score(runs):
+    """Sum over workloads of each candidate's time relative to that workload's best."""
+    score = {}
+    for results in runs:
+        best = min(v for v in results.values() if v is not None)
+        for cand, ms in results.items():
+            if ms is None or score.get(cand, 0.0) is None:
+                score[cand] = None
+            else:
+                score[cand] = score.get(cand, 0.0) + ms / best
+    return score
+
+
 def main(argv=None):
+    """Two stages per signature: every candidate timed briefly on every workload of
+    the signature, then the best few re-timed with six times the steps; the stored
+    tuning minimises the sum of times relative to each workload's best."""
     import argparse
 
     import torch
@@ -161,6 +177,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", action="append", default=None)
     ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--finalists", type=int, default=6)
     args = ap.parse_args(argv)
     root = Path(__file__).resolve().parent.parent
     sys.path.insert(0, str(root))
@@ -173,26 +190,26 @@ def main(argv=None):
         keys, vals, off = bench.make_workload(pkg, wl)
         kdt = {"u": {4: torch.uint32, 8: torch.uint64}, "f": {4: torch.float32, 8: torch.float64},
                "i": {4: torch.int32, 8: torch.int64}}[keys.dtype.kind][keys.dtype.itemsize]
-        dk = bench.to_dev(torch, keys).view(kdt)
-        dv = bench.to_dev(torch, vals).view(torch.uint32)
-        do = torch.from_numpy(off).cuda()
-        best, results = tune(pkg, torch, dk, dv, do, steps=args.steps)
+        dev = (bench.to_dev(torch, keys).view(kdt), bench.to_dev(torch, vals).view(torch.uint32),
+               torch.from_numpy(off).cuda())
+        best, results = tune(pkg, torch, *dev, steps=args.steps, verbose=False)
         sig = signature(keys.size, off.size - 1, keys.dtype.itemsize, vals.dtype.itemsize)
-        by_sig.setdefault(sig, []).append((wl, best[1], results))
-        print(json.dumps({"workload": wl, "signature": sig, "best": best[0], "ms": round(best[1], 4)}), flush=True)
-        del dk, dv, do
-    # workloads that share a signature: keep the candidate with the smallest sum of
-    # times relative to each workload's own best
+        by_sig.setdefault(sig, []).append((wl, dev, results))
+        print(json.dumps({"workload": wl, "signature": sig, "screen_best": best[0], "ms": round(best[1], 4)}),
+              flush=True)
     for sig, runs in by_sig.items():
-        score = {}
-        for _, best_ms, results in runs:
-            for cand, ms in results.items():
-                if ms is None or score.get(cand, 0.0) is None:
-                    score[cand] = None
-                else:
-                    score[cand] = score.get(cand, 0.0) + ms / best_ms
+        score = _score([r[2] for r in runs])
+        ranked = sorted((c for c, v in score.items() if v is not None), key=lambda c: score[c])
+        finalists = [json.loads(c) for c in ranked[:args.finalists]]
+        refined = []
+        for wl, dev, _ in runs:
+            _, res = tune(pkg, torch, *dev, steps=args.steps * 6, verbose=False, cands=finalists)
+            refined.append(res)
+            print(json.dumps({"workload": wl, "finalists": {c: round(v, 4) for c, v in res.items() if v}}),
+                  flush=True)
+        score = _score(refined)
         cand = min((c for c, v in score.items() if v is not None), key=lambda c: score[c])
-        ms = sum(r[2][cand] for r in runs) / len(runs)
+        ms = sum(r[cand] for r in refined) / len(refined)
         store(sig, json.loads(cand), ms, note="tuned on " + ", ".join(r[0] for r in runs))
         print(json.dumps({"signature": sig, "chosen": json.loads(cand), "relative": round(score[cand], 4)}),
               flush=True)

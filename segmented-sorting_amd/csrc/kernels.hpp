@@ -285,13 +285,39 @@ __global__ __launch_bounds__(NT, 1024 / NT) void k_tile_tier(Bufs b, Sched sc, i
     using T = RadixTile<NT, IPT, U, V, HAS_V>;
     __shared__ typename T::Smem sm;
     const int count = tile_count(sc, cls);
-    const int stride = persistent ? (int)gridDim.x : 0x7FFFFFFF;
-    for (int i = blockIdx.x; i < count; i += stride) {
-        const TileItem it = sc.tq[cls][i];
-        const int src = ti_src(it);
-        T::sort(sm, (const U*)b.k[src], (const V*)b.v[src], buf_ptr<U>(b.k[BUF_OUT]), buf_ptr<V>(b.v[BUF_OUT]),
-                it.begin, ti_n(it), km);
-        if (!persistent) break;
+    int i = blockIdx.x;
+    if (i >= count) return;
+    if constexpr (IPT > 4) {
+        // no register room for a second set of keys: one range at a time
+        for (; i < count; i += persistent ? (int)gridDim.x : count) {
+            const TileItem cur = sc.tq[cls][i];
+            const int src = ti_src(cur);
+            T::sort(sm, (const U*)b.k[src], (const V*)b.v[src], buf_ptr<U>(b.k[BUF_OUT]), buf_ptr<V>(b.v[BUF_OUT]),
+                    cur.begin, ti_n(cur), km);
+        }
+        return;
+    }
+    // software pipeline: the next range's keys are loaded while this one is
+    // written out
+    TileItem it = sc.tq[cls][i];
+    U raw[IPT];
+    T::load_raw((const U*)b.k[ti_src(it)], it.begin, ti_n(it), raw);
+    for (;;) {
+        const TileItem cur = it;
+        U keys[IPT];
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) keys[j] = raw[j];
+        const int next = persistent ? i + (int)gridDim.x : count;
+        const int src = ti_src(cur);
+        T::sort_loaded(sm, keys, (const U*)b.k[src], (const V*)b.v[src], buf_ptr<U>(b.k[BUF_OUT]),
+                       buf_ptr<V>(b.v[BUF_OUT]), cur.begin, ti_n(cur), km, [&] {
+                           if (next < count) {
+                               it = sc.tq[cls][next];
+                               T::load_raw((const U*)b.k[ti_src(it)], it.begin, ti_n(it), raw);
+                           }
+                       });
+        if (next >= count) break;
+        i = next;
     }
 }
 

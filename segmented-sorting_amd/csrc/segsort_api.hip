@@ -239,6 +239,8 @@ struct Cfg {
     static constexpr int NT3 = 1024, IPT3 = ITEM_BYTES <= 8 ? 16 : 8;
     static constexpr int CAP[4] = {NT0 * IPT0, NT1 * IPT1, NT2 * IPT2, NT3 * IPT3};
     static constexpr int TILE_MAX = NT3 * IPT3;
+    // chunks of the multi-workgroup MSD level (the largest tile: fewer chunks keep the per-range scan short)
+    static constexpr int NTC = NT3, IPTC = IPT3, CHUNK = NTC * IPTC;
     static constexpr int NPOS = int(sizeof(U));  // 8-bit digit positions of a key
 };
 
@@ -305,7 +307,7 @@ Layout make_layout(const Plan& p, int num_items, int num_segments, int levels, b
     };
     const size_t N = (size_t)num_items, S = (size_t)num_segments;
     L.lq_cap = N / (size_t)(p.tile_max + 1) + 1;
-    L.chunks = N / (size_t)C::TILE_MAX + L.lq_cap + 1;
+    L.chunks = N / (size_t)C::CHUNK + L.lq_cap + 1;
     // every partition (level 0 or deeper) queues at most RADIX tile children; the
     // ranges partitioned at one depth are disjoint and larger than a tile, and
     // their children are disjoint non-empty ranges (at most N of them)
@@ -409,7 +411,7 @@ struct Launcher {
             sc.hist = (uint32_t*)(base + L.hist);
             sc.off = (uint32_t*)(base + L.off);
         }
-        sc.chunk = C::TILE_MAX;
+        sc.chunk = C::CHUNK;
         sc.npos_max = C::NPOS;
         for (int c = 0; c < p.nclass; ++c) sc.tq_cap[c] = (uint32_t)std::min<size_t>(L.tq_cap[c], 0xFFFFFFFFu);
         sc.lq_cap = (uint32_t)L.lq_cap;
@@ -448,16 +450,16 @@ struct Launcher {
         SEGSORT_LAUNCH("bin", st, -1, 0.0, 8.0 * S,
                        (k_bin<U, V, HAS_V><<<bin_grid(S), BIN_NT, 0, st>>>(db, de, S, b, sc, p.cc, km.end_bit, 1, bin_spt(S))));
         if (levels > 0) {
-            auto kh = k_large_hist<C::NT3, C::IPT3, U, V, HAS_V>;
-            auto ks = k_large_scatter<C::NT3, C::IPT3, U, V, HAS_V>;
+            auto kh = k_large_hist<C::NTC, C::IPTC, U, V, HAS_V>;
+            auto ks = k_large_scatter<C::NTC, C::IPTC, U, V, HAS_V>;
             auto kl = k_large_local<C::NT3, C::IPT3, U, V, HAS_V>;
-            const int gh = resident_blocks(kh, C::NT3) * 2;
-            const int gs = resident_blocks(ks, C::NT3) * tu.grid_mult;
+            const int gh = resident_blocks(kh, C::NTC) * 2;
+            const int gs = resident_blocks(ks, C::NTC) * tu.grid_mult;
             const int gl = resident_blocks(kl, C::NT3);
-            SEGSORT_LAUNCH("large_hist", st, 100, (double)sizeof(U), 0.0, (kh<<<gh, C::NT3, 0, st>>>(b, sc, 0, km)));
+            SEGSORT_LAUNCH("large_hist", st, 100, (double)sizeof(U), 0.0, (kh<<<gh, C::NTC, 0, st>>>(b, sc, 0, km)));
             SEGSORT_LAUNCH("large_scan", st, -1, 0.0, 0.0,
                            (k_large_scan<<<device_cus(), SCAN_NT, 0, st>>>(sc, 0, km, p.cc, p.tile_max)));
-            SEGSORT_LAUNCH("large_scatter", st, 100, PAIR_BYTES, 0.0, (ks<<<gs, C::NT3, 0, st>>>(b, sc, 0, km)));
+            SEGSORT_LAUNCH("large_scatter", st, 100, PAIR_BYTES, 0.0, (ks<<<gs, C::NTC, 0, st>>>(b, sc, 0, km)));
             SEGSORT_LAUNCH("large_local", st, 101, PAIR_BYTES + sizeof(U), 0.0,
                            (kl<<<gl, C::NT3, 0, st>>>(b, sc, 1, km, p.cc, p.tile_max)));
         }

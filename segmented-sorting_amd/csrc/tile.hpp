@@ -250,13 +250,41 @@ struct RadixTile {
     // once from its source at the end.
     __device__ static void sort(Smem& sm, const U* ks, const V* vs, U* ko, V* vo, int64_t begin, int n,
                                 const KeyMode& km) {
+        U raw[IPT];
+        load_raw(ks, begin, n, raw);
+        sort_loaded(sm, raw, ks, vs, ko, vo, begin, n, km, [] {});
+    }
+
+    // The raw keys of this thread's slots (clamped, always in bounds), as
+    // sort_loaded takes them. A persistent caller issues the loads of its next
+    // range from sort_loaded's `prefetch` hook, which runs once the current
+    // keys are no longer needed in registers (after the last pass), so they
+    // overlap the current range's write-out. Workgroup barriers wait on LDS
+    // only, so the loads stay in flight across them.
+    __device__ static __forceinline__ void load_raw(const U* ks, int64_t begin, int n, U (&raw)[IPT]) {
+        if (wave() < active_waves(n)) {
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int p = pos(j);
+                raw[j] = ks[begin + (p < n ? p : n - 1)];
+            }
+        }
+    }
+
+    template <typename Prefetch>
+    __device__ static void sort_loaded(Smem& sm, const U (&raw)[IPT], const U* ks, const V* vs, U* ko, V* vo,
+                                       int64_t begin, int n, const KeyMode& km, Prefetch prefetch) {
         const bool act = wave() < active_waves(n);
         U k[IPT];
-        if (act) load_keys(ks, begin, n, km, k);
+        if (act) {
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) k[j] = pos(j) < n ? twiddle_in<U>(raw[j], km.kind, km.desc) : Bits<U>::ALL;
+        }
         U o, a;
         reduce_or_and(sm, k, n, o, a);
         const U varying = U((o ^ a) & range_mask<U>(km.begin_bit, km.end_bit));
         if (varying == 0) {
+            prefetch();
             if (act) {
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
@@ -297,6 +325,7 @@ struct RadixTile {
                 }
             }
         }
+        prefetch();
         if constexpr (HAS_V && sizeof(V) <= sizeof(U)) {
             // Values: one coalesced read of the range into the key area of LDS
             // (free once the sorted keys are out), then the permutation is

@@ -137,6 +137,16 @@ int segsort_get_tuning(int key_width, int value_bytes, int* wave_cap, int* tile_
  * writes up to 4 values and returns how many. */
 int segsort_tile_caps(int key_width, int value_bytes, int* caps);
 
+/* Graph replay of the batched form (no counterpart in the reference, whose
+ * calls are single kernel launches per pass). 0 (the default, or SEGSORT_GRAPH
+ * unset/0): every call launches its kernels on the caller's stream. 1 (or
+ * SEGSORT_GRAPH=1): the first call with a given argument set is captured into a
+ * hipGraph on a private stream and later calls replay it (16 graphs cached).
+ * On MI355X with ROCm 7.2 direct launches measured faster than replay. Either
+ * way the batched call is capture-safe: a caller may capture it into its own
+ * graph (no host sync, no allocation, no memset). Returns the previous mode. */
+int segsort_set_graph_replay(int on);
+
 /* Per-launch timing (no counterpart in the reference, whose drivers time the
  * whole call with cudaEvents, src/radixseg.cu:121-139). While enabled, every
  * kernel launch is bracketed by HIP events on its stream and, after each

@@ -33,7 +33,7 @@ EXPORTS = (
     "segsort_sort", "segsort_sort_pairs_u32", "segsort_sort_keys_u32", "segsort_bb_segsort",
     "segsort_mgpu_segmented_sort", "segsort_row_iota", "segsort_perm_scatter_f32", "segsort_set_tuning",
     "segsort_get_tuning", "segsort_tile_caps", "segsort_last_error", "segsort_version", "segsort_profile_enable",
-    "segsort_profile_report",
+    "segsort_profile_report", "segsort_set_graph_replay",
 )
 
 _lib = None
@@ -79,6 +79,8 @@ def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
     lib.segsort_last_error.restype = ctypes.c_char_p
     lib.segsort_version.argtypes = []
     lib.segsort_version.restype = i32
+    lib.segsort_set_graph_replay.argtypes = [i32]
+    lib.segsort_set_graph_replay.restype = i32
     lib.segsort_profile_enable.argtypes = [i32]
     lib.segsort_profile_enable.restype = i32
     lib.segsort_profile_report.argtypes = [ctypes.c_char_p, sz]
@@ -250,6 +252,12 @@ def get_tuning(key_width, value_bytes):
     _check(load_library().segsort_get_tuning(key_width, value_bytes, ctypes.byref(a), ctypes.byref(b),
                                              ctypes.byref(c), ctypes.byref(d)), "get_tuning")
     return {"wave_cap": a.value, "tile_enable": b.value, "grid_mult": c.value, "concurrent": d.value}
+
+
+def set_graph_replay(on: bool) -> bool:
+    """Replay the batched form from cached hipGraphs (True) or launch directly
+    (False, the default); returns the previous setting."""
+    return bool(load_library().segsort_set_graph_replay(1 if on else 0))
 
 
 def profile(on: bool) -> None:

@@ -130,13 +130,13 @@ hipError_t replay(GraphEntry& g, hipStream_t st) {
 std::mutex g_graph_mu;
 std::vector<GraphEntry> g_graphs;
 uint64_t g_graph_clock = 0;
-int g_graph_mode = -1;  // -1: read SEGSORT_GRAPH once (default on), 0 off, 1 on
+int g_graph_mode = -1;  // -1: read SEGSORT_GRAPH once (default off), 0 off, 1 on
 constexpr size_t GRAPH_CACHE = 16;
 
 bool graphs_enabled() {
     if (g_graph_mode < 0) {
         const char* e = getenv("SEGSORT_GRAPH");
-        g_graph_mode = (e && e[0] == '0') ? 0 : 1;
+        g_graph_mode = (e && e[0] == '1') ? 1 : 0;
     }
     return g_graph_mode == 1;
 }
@@ -641,6 +641,13 @@ int dispatch_algo(int algo, void* d_temp, size_t* temp_bytes, const void* ki, vo
 extern "C" {
 
 int segsort_version(void) { return 100; }
+
+int segsort_set_graph_replay(int on) {
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    const int before = graphs_enabled() ? 1 : 0;
+    g_graph_mode = on ? 1 : 0;
+    return before;
+}
 
 int segsort_profile_enable(int on) {
     Profiler::get().enable(on != 0);

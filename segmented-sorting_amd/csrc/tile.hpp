@@ -93,9 +93,16 @@ struct RadixTile {
     // group's size; the other lanes store into a private dummy word, so the
     // code has no branch. LDS operations of one wave complete in order, so the
     // next item's read sees this item's store.
-    __device__ static __forceinline__ uint32_t count_one(uint32_t* wc, uint32_t* dummy, uint32_t d, uint64_t peers) {
-        const uint32_t below = lanes_below(peers);
+    // The count is read before the peers are matched, so that the match's
+    // instructions cover the LDS round trip.
+    template <int NBITS>
+    __device__ static __forceinline__ uint32_t count_one(uint32_t* wc, uint32_t* dummy, uint32_t d, int bits) {
         const uint32_t base = wc[d];
+        __builtin_amdgcn_sched_barrier(0);  // keep the read ahead of the match
+        uint64_t peers;
+        if constexpr (NBITS > 0) peers = match_digit<NBITS>(d);
+        else peers = match_digit_rt(d, bits);
+        const uint32_t below = lanes_below(peers);
         uint32_t* dst = below == 0 ? &wc[d] : dummy;
         *dst = base + (uint32_t)__popcll(peers);
         return base + below;
@@ -124,13 +131,13 @@ struct RadixTile {
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
                     const uint32_t d = (uint32_t)((k[j] >> shift) & U(RADIX - 1));
-                    dst[j] = count_one(wc, dummy, d, match_digit<RADIX_BITS>(d));
+                    dst[j] = count_one<RADIX_BITS>(wc, dummy, d, RADIX_BITS);
                 }
             } else {
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
                     const uint32_t d = (uint32_t)((k[j] >> shift) & dmask);
-                    dst[j] = count_one(wc, dummy, d, match_digit_rt(d, bits));
+                    dst[j] = count_one<0>(wc, dummy, d, bits);
                 }
             }
         }
@@ -138,25 +145,33 @@ struct RadixTile {
         if constexpr (NW <= 4) {
             // one wave scans the counts: lane l takes digits l, l+64, l+128, l+192
             if (w == 0) {
-                uint32_t carry = 0;
+                // all counts read first (one LDS round trip), four independent scans,
+                // then all bases written
+                constexpr int G = RADIX / WAVE;
+                uint32_t c[G][NW];
+                uint32_t s[G], incl[G];
 #pragma unroll
-                for (int i = 0; i < RADIX / WAVE; ++i) {
-                    const int d = i * WAVE + (int)l;
-                    uint32_t c[NW];
-                    uint32_t s = 0;
+                for (int i = 0; i < G; ++i) {
+                    s[i] = 0;
 #pragma unroll
                     for (int ww = 0; ww < NW; ++ww) {
-                        c[ww] = ww < wact ? sm.wcnt[ww * RADIX + d] : 0u;
-                        s += c[ww];
+                        c[i][ww] = ww < wact ? sm.wcnt[ww * RADIX + i * WAVE + (int)l] : 0u;
+                        s[i] += c[i][ww];
                     }
-                    const uint32_t incl = wave_incl_scan(s);
-                    uint32_t base = carry + incl - s;
-                    carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+                }
+#pragma unroll
+                for (int i = 0; i < G; ++i) incl[i] = wave_incl_scan(s[i]);
+                uint32_t carry = 0;
+#pragma unroll
+                for (int i = 0; i < G; ++i) {
+                    const int d = i * WAVE + (int)l;
+                    uint32_t base = carry + incl[i] - s[i];
+                    carry += (uint32_t)__builtin_amdgcn_readlane((int)incl[i], 63);
                     sm.dstart[d] = base;
 #pragma unroll
                     for (int ww = 0; ww < NW; ++ww) {
                         if (ww < wact) sm.wcnt[ww * RADIX + d] = base;
-                        base += c[ww];
+                        base += c[i][ww];
                     }
                 }
             }

@@ -250,11 +250,13 @@ __global__ __launch_bounds__(256) void k_wave_tier(Bufs b, Sched sc, int cls, Ke
     const int wid = blockIdx.x * waves_per_block + (threadIdx.x >> 6);
     const int stride = persistent ? gridDim.x * waves_per_block : 0x7FFFFFFF;
     const int count = tile_count(sc, cls);
+    __shared__ V stage[HAS_V ? 4 * 512 : 1];  // 512 values per wave (blocks of 4 waves)
+    V* my_stage = stage + (HAS_V ? (threadIdx.x >> 6) * 512 : 0);
     for (int i = wid; i < count; i += stride) {
         const TileItem it = sc.tq[cls][i];
         const int src = ti_src(it);
         wave_sort<U, V, HAS_V>((const U*)b.k[src], (const V*)b.v[src], buf_ptr<U>(b.k[BUF_OUT]),
-                               buf_ptr<V>(b.v[BUF_OUT]), it.begin, ti_n(it), km, rank_max);
+                               buf_ptr<V>(b.v[BUF_OUT]), it.begin, ti_n(it), km, rank_max, my_stage);
         if (!persistent) break;
     }
 }
@@ -269,11 +271,28 @@ __global__ __launch_bounds__(256) void k_group_tier(Bufs b, Sched sc, int cls, K
     const int nwaves = gridDim.x * waves_per_block;
     const int count = tile_count(sc, cls);
     const int g = (int)(lane_id() / G);
-    for (int base = wid * PER_WAVE; base < count; base += nwaves * PER_WAVE) {
-        const int i = base + g;
+    const int step = nwaves * PER_WAVE;
+    int base = wid * PER_WAVE;
+    if (base >= count) return;  // whole wave; the kernel has no barriers
+    // pipeline: queue entries two batches ahead, keys and values one batch ahead
+    auto entry = [&](int at) {
         TileItem it = make_ti(0, 0, BUF_IN);
-        if (i < count) it = sc.tq[cls][i];
-        group_sort<G, U, V, HAS_V>(b, it.begin, i < count ? ti_n(it) : 0, ti_src(it), km);
+        if (at + g < count) it = sc.tq[cls][at + g];
+        return it;
+    };
+    GroupItems<U, V> cur, nxt;
+    group_load<G, U, V, HAS_V>(b, entry(base), base + g < count, cur);
+    TileItem it_next = entry(base + step);
+    for (;;) {
+        const int next = base + step;
+        if (next < count) {
+            group_load<G, U, V, HAS_V>(b, it_next, next + g < count, nxt);
+            it_next = entry(next + step);
+        }
+        group_sort_loaded<G, U, V, HAS_V>(b, cur, km);
+        if (next >= count) break;
+        cur = nxt;
+        base = next;
     }
 }
 

@@ -636,26 +636,62 @@ __device__ __forceinline__ void bitonic_group(W& w) {
     }
 }
 
-// Each aligned group of G lanes sorts its own range [begin, begin+n) of
-// buffer src (n <= G; n == 0 for a group without a range): lane i of the
-// group loads item i, the group's network sorts (order key, i) words, and
-// lane i writes the i-th smallest.
+// The range of one group of G lanes and, in lane i of the group, item i's key
+// and value (i < n), as loaded ahead of the group sort.
+template <typename U, typename V>
+struct GroupItems {
+    int64_t begin;
+    int n;
+    int src;
+    U key;
+    V val;
+};
+
 template <int G, typename U, typename V, bool HAS_V>
-__device__ __forceinline__ void group_sort(const Bufs& b, int64_t begin, int n, int src, const KeyMode& km) {
-    using W = SortWord<U>;
+__device__ __forceinline__ void group_load(const Bufs& b, const TileItem& it, bool valid, GroupItems<U, V>& gi) {
     const int i = (int)(lane_id() & (G - 1));
-    const U* ks = (const U*)b.k[src];
-    const V* vs = (const V*)b.v[src];
-    U* ko = buf_ptr<U>(b.k[BUF_OUT]);
-    V* vo = buf_ptr<V>(b.v[BUF_OUT]);
+    gi.begin = it.begin;
+    gi.n = valid ? ti_n(it) : 0;
+    gi.src = ti_src(it);
+    if (i < gi.n) {
+        gi.key = ((const U*)b.k[gi.src])[gi.begin + i];
+        if constexpr (HAS_V) gi.val = ((const V*)b.v[gi.src])[gi.begin + i];
+    }
+}
+
+// Value of `x` in lane `lane` (all lanes take part).
+template <typename T>
+__device__ __forceinline__ T lane_fetch(T x, int lane) {
+    if constexpr (sizeof(T) == 8) {
+        const uint64_t u = (uint64_t)x;
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(lane << 2, (int)(uint32_t)u);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(lane << 2, (int)(uint32_t)(u >> 32));
+        return (T)((uint64_t(hi) << 32) | lo);
+    } else {
+        return (T)(uint32_t)__builtin_amdgcn_ds_bpermute(lane << 2, (int)(uint32_t)x);
+    }
+}
+
+// Each aligned group of G lanes sorts its own range (n <= G; n == 0 for a
+// group without a range): the group's network sorts (order key, i) words and
+// lane i writes the i-th smallest, taking its key and value from the lane
+// that loaded them (cross-lane permute, no second read of global memory).
+template <int G, typename U, typename V, bool HAS_V>
+__device__ __forceinline__ void group_sort_loaded(const Bufs& b, const GroupItems<U, V>& gi, const KeyMode& km) {
+    using W = SortWord<U>;
+    const int lane = (int)lane_id();
+    const int i = lane & (G - 1);
     const U rm = range_mask<U>(km.begin_bit, km.end_bit);
     W w = W::inf();
-    if (i < n) w = W::make(U(twiddle_in<U>(ks[begin + i], km.kind, km.desc) & rm), (uint32_t)i);
+    if (i < gi.n) w = W::make(U(twiddle_in<U>(gi.key, km.kind, km.desc) & rm), (uint32_t)i);
     bitonic_group<G>(w);
-    if (i < n) {
-        const uint32_t s = w.pos();
-        ko[begin + i] = rm == Bits<U>::ALL ? twiddle_out<U>(U(w.key()), km.kind, km.desc) : ks[begin + s];
-        if constexpr (HAS_V) vo[begin + i] = vs[begin + s];
+    const int from = (lane & ~(G - 1)) + (int)(w.pos() & (G - 1));
+    const U key = lane_fetch<U>(gi.key, from);
+    V val{};
+    if constexpr (HAS_V) val = lane_fetch<V>(gi.val, from);
+    if (i < gi.n) {
+        buf_ptr<U>(b.k[BUF_OUT])[gi.begin + i] = key;
+        if constexpr (HAS_V) buf_ptr<V>(b.v[BUF_OUT])[gi.begin + i] = val;
     }
 }
 
@@ -664,20 +700,30 @@ __device__ __forceinline__ void group_sort(const Bufs& b, int64_t begin, int n, 
 // network, and written in rank order; keys are rebuilt from the words when
 // the whole key takes part in the order, else read back by position, and
 // values are read by position.
+// Values are read with the keys and staged in the wave's LDS region `stage`
+// (64*R entries) after the network, so the permutation is applied from LDS
+// (LDS operations of one wave complete in order: no barrier is needed).
 template <int R, typename U, typename V, bool HAS_V>
 __device__ __forceinline__ void wave_bitonic_sort(const U* ks, const V* vs, U* ko, V* vo, int64_t begin, int n,
-                                                  const KeyMode& km) {
+                                                  const KeyMode& km, V* stage) {
     using W = SortWord<U>;
     const uint32_t l = lane_id();
     const U rm = range_mask<U>(km.begin_bit, km.end_bit);
     W w[R];
+    V v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int p = r * WAVE + (int)l;
-        const U raw = ks[begin + (p < n ? p : n - 1)];
+        const int pc = p < n ? p : n - 1;
+        const U raw = ks[begin + pc];
+        if constexpr (HAS_V) v[r] = vs[begin + pc];
         w[r] = p < n ? W::make(U(twiddle_in<U>(raw, km.kind, km.desc) & rm), (uint32_t)p) : W::inf();
     }
     bitonic_wave<R>(w);
+    if constexpr (HAS_V) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) stage[r * WAVE + (int)l] = v[r];
+    }
     const bool full = rm == Bits<U>::ALL;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -685,7 +731,7 @@ __device__ __forceinline__ void wave_bitonic_sort(const U* ks, const V* vs, U* k
         if (q < n) {
             const uint32_t src = w[r].pos();
             ko[begin + q] = full ? twiddle_out<U>(U(w[r].key()), km.kind, km.desc) : ks[begin + src];
-            if constexpr (HAS_V) vo[begin + q] = vs[begin + src];
+            if constexpr (HAS_V) vo[begin + q] = stage[src];
         }
     }
 }
@@ -714,15 +760,16 @@ __device__ __forceinline__ void wave_rank_sort(const U* ks, const V* vs, U* ko, 
 
 // Sort one range of n <= 512 items with one wave: compare-count for the
 // shortest ranges, else the bitonic network on the smallest power of two
-// of words per lane that holds n.
+// of words per lane that holds n (values staged in the wave's 512 LDS
+// entries at `stage`).
 template <typename U, typename V, bool HAS_V>
 __device__ __forceinline__ void wave_sort(const U* ks, const V* vs, U* ko, V* vo, int64_t begin, int n,
-                                          const KeyMode& km, int rank_max) {
+                                          const KeyMode& km, int rank_max, V* stage) {
     if (n <= rank_max) wave_rank_sort<U, V, HAS_V>(ks, vs, ko, vo, begin, n, km);
-    else if (n <= 64) wave_bitonic_sort<1, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km);
-    else if (n <= 128) wave_bitonic_sort<2, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km);
-    else if (n <= 256) wave_bitonic_sort<4, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km);
-    else wave_bitonic_sort<8, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km);
+    else if (n <= 64) wave_bitonic_sort<1, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km, stage);
+    else if (n <= 128) wave_bitonic_sort<2, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km, stage);
+    else if (n <= 256) wave_bitonic_sort<4, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km, stage);
+    else wave_bitonic_sort<8, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km, stage);
 }
 
 }  // namespace ssort

@@ -138,10 +138,11 @@ __device__ __forceinline__ void push_large(const Sched& sc, int level, int begin
 // Bin every segment by length: length 1 is copied here, the rest go to the
 // queue of their class (batched and plain forms) or to MSD level 0 (large).
 // Each block bins spt * BIN_NT segments (spt <= BIN_SPT, chosen by the host so
-// that the grid keeps a few hundred blocks); slots are taken from LDS counters
+// that the grid keeps about a hundred blocks: the global queue counters see one
+// atomic per block and class); slots are taken from LDS counters
 // (one LDS atomic per wave and class) and then from the global counters with
 // one atomic per block and class, so the hot global counters see few atomics.
-constexpr int BIN_NT = 256;
+constexpr int BIN_NT = 1024;
 constexpr int BIN_SPT = 8;
 
 template <typename U, typename V, bool HAS_V>
@@ -159,6 +160,7 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
     int beg[BIN_SPT], n[BIN_SPT], cls[BIN_SPT];
     uint32_t slot[BIN_SPT];
     const int s0 = blockIdx.x * (BIN_NT * spt) + threadIdx.x;
+    // every offset load issued before any is used
 #pragma unroll
     for (int j = 0; j < BIN_SPT; ++j) {
         const int s = s0 + j * BIN_NT;
@@ -166,29 +168,49 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
         n[j] = 0;
         if (j < spt && s < num_segments) {
             beg[j] = d_begin[s];
-            n[j] = d_end[s] - beg[j];
-            if (n[j] < 0) n[j] = 0;
+            n[j] = d_end[s];
         }
+    }
+#pragma unroll
+    for (int j = 0; j < BIN_SPT; ++j) {
+        n[j] = n[j] > beg[j] ? n[j] - beg[j] : 0;
         if (n[j] == 1) {
             buf_ptr<U>(b.k[BUF_OUT])[beg[j]] = ((const U*)b.k[BUF_IN])[beg[j]];
             if constexpr (HAS_V) buf_ptr<V>(b.v[BUF_OUT])[beg[j]] = ((const V*)b.v[BUF_IN])[beg[j]];
         }
         cls[j] = n[j] >= 2 ? class_of(cc, n[j]) : -2;
-        slot[j] = 0;
-        for (int c = 0; c < cc.nclass; ++c) {
-            const uint64_t m = __ballot(cls[j] == c);
-            if (m == 0) continue;
-            const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
-            const uint32_t items = wave_sum(cls[j] == c ? (uint32_t)n[j] : 0u);
-            uint32_t base = 0;
-            if (lane_id() == leader) {
-                base = atomicAdd(&lcount[c], (uint32_t)__popcll(m));
-                atomicAdd(&litems[c], (unsigned long long)items);
-            }
-            base = (uint32_t)__builtin_amdgcn_readlane((int)base, (int)leader);
-            if (cls[j] == c) slot[j] = base + lanes_below(m);
-        }
         if (cls[j] == -1 && large_to_level0) push_large(sc, 0, beg[j], n[j], BUF_IN, end_bit);
+    }
+    // per class: the wave's slots counted with ballots in registers, then one LDS
+    // atomic per class and wave for the wave's base inside the block
+    uint32_t wbase[MAX_TILE_CLASSES];
+#pragma unroll
+    for (int c = 0; c < MAX_TILE_CLASSES; ++c) {
+        wbase[c] = 0;
+        if (c >= cc.nclass) continue;
+        uint32_t run = 0, items = 0;
+#pragma unroll
+        for (int j = 0; j < BIN_SPT; ++j) {
+            const uint64_t m = __ballot(cls[j] == c);
+            if (cls[j] == c) {
+                slot[j] = run + lanes_below(m);
+                items += (uint32_t)n[j];
+            }
+            run += (uint32_t)__popcll(m);
+        }
+        if (run == 0) continue;
+        items = wave_sum(items);
+        if (lane_id() == 0) {
+            wbase[c] = atomicAdd(&lcount[c], run);
+            atomicAdd(&litems[c], (unsigned long long)items);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < MAX_TILE_CLASSES; ++c) {
+        const uint32_t wb = (uint32_t)__builtin_amdgcn_readfirstlane((int)wbase[c]);
+#pragma unroll
+        for (int j = 0; j < BIN_SPT; ++j)
+            if (cls[j] == c) slot[j] += wb;
     }
     __syncthreads();
     if (threadIdx.x < (unsigned)cc.nclass) {

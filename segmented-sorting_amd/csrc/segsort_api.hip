@@ -31,8 +31,8 @@ int ok() {
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-// Segments per binning thread: one while that keeps >= 256 blocks, at most BIN_SPT.
-int bin_spt(int S) { return std::min(BIN_SPT, std::max(1, (S + BIN_NT * 256 - 1) / (BIN_NT * 256))); }
+// Segments per binning thread: one while that keeps >= 64 blocks, at most BIN_SPT.
+int bin_spt(int S) { return std::min(BIN_SPT, std::max(1, (S + BIN_NT * 64 - 1) / (BIN_NT * 64))); }
 int bin_grid(int S) { return (S + BIN_NT * bin_spt(S) - 1) / (BIN_NT * bin_spt(S)); }
 
 // Ranges up to this length use the wave's compare-count rank (about 4n VALU

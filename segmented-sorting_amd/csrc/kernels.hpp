@@ -29,15 +29,26 @@ struct ClassCaps {
     int nclass;
 };
 
+// What the MSD level's first pass records per chunk: the OR and AND of the
+// chunk's order keys inside the range's bits, and the highest digit position
+// whose bits vary inside the chunk (-1: every key equal); the chunk's digit
+// histogram is kept for that position only.
+struct ChunkInfo {
+    unsigned long long kor;
+    unsigned long long kand;
+    int lp;
+    int pad;
+};
+
 // Where the queues of one call live (all inside the caller's temp storage).
 struct Sched {
     Counters* ctr;
     TileItem* tq[MAX_TILE_CLASSES];
     LargeItem* lq[MAX_LEVELS + 1];
-    uint32_t* hist;  // [chunk][npos_max][RADIX]
+    uint32_t* hist;  // [chunk][RADIX], the histogram of the chunk's highest varying digit
+    ChunkInfo* cinfo;  // [chunk]
     uint32_t* off;   // [chunk][RADIX]
     int chunk;       // items per large-tier chunk
-    int npos_max;    // digit positions per chunk histogram
     // Capacities: a push beyond them is dropped and readers clamp their counts,
     // so overlapping (invalid) segments cannot write outside the temp storage.
     uint32_t tq_cap[MAX_TILE_CLASSES];
@@ -378,12 +389,19 @@ __device__ __forceinline__ int npos_of(int hi_bit, int begin_bit) {
     return (hi_bit - begin_bit + RADIX_BITS - 1) / RADIX_BITS;
 }
 
-// Pass 1 of a level: digit histograms of every chunk for every digit position
-// below the range's hi_bit.
+// Pass 1 of a level: per chunk, the OR and AND of the order keys (inside the
+// range's bits [begin_bit, hi_bit)) and the digit histogram of the chunk's
+// highest varying digit position. The level's digit is the range's highest
+// varying position, which no chunk exceeds; a chunk whose own top is lower
+// holds a single digit value there, known from its AND, so one histogram per
+// chunk is all the scan needs.
 template <int NT, int IPT, typename U, typename V, bool HAS_V>
 __global__ __launch_bounds__(NT) void k_large_hist(Bufs b, Sched sc, int level, KeyMode km) {
-    using T = RadixTile<NT, IPT, U, V, HAS_V>;
-    __shared__ uint32_t hist[(sizeof(U) * 8 / RADIX_BITS) * RADIX];
+    constexpr int NW = NT / WAVE;
+    __shared__ uint32_t hist[RADIX];
+    __shared__ U red_or[NW];
+    __shared__ U red_and[NW];
+    const int t = threadIdx.x;
     int count, chunks;
     level_counts(sc, level, count, chunks);
     const LargeItem* items = sc.lq[level];
@@ -392,35 +410,95 @@ __global__ __launch_bounds__(NT) void k_large_hist(Bufs b, Sched sc, int level, 
         const int c = g - it.chunk_base;
         const int64_t start = (int64_t)it.begin + (int64_t)c * sc.chunk;
         const int n = (it.n - c * sc.chunk) < sc.chunk ? (it.n - c * sc.chunk) : sc.chunk;
-        const int npos = npos_of(it.hi_bit, km.begin_bit);
-        for (int i = threadIdx.x; i < npos * RADIX; i += NT) hist[i] = 0;
+        const U rmask = range_mask<U>(km.begin_bit, it.hi_bit);
+        const U* ks = (const U*)b.k[it.src];
+        U key[IPT];
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            const int i = j * NT + t;
+            key[j] = i < n ? ks[start + i] : U(0);
+        }
+        U o = 0, a = Bits<U>::ALL;
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            if (j * NT + t < n) {
+                key[j] = U(twiddle_in<U>(key[j], km.kind, km.desc) & rmask);
+                o |= key[j];
+                a &= key[j];
+            }
+        }
+        o = wave_or(o);
+        a = wave_and(a);
+        if ((t & 63) == 0) {
+            red_or[t >> 6] = o;
+            red_and[t >> 6] = a;
+        }
+        for (int i = t; i < RADIX; i += NT) hist[i] = 0;
         __syncthreads();
-        T::histogram(hist, (const U*)b.k[it.src], start, n, km, km.begin_bit, npos, it.hi_bit);
+        o = 0;
+        a = Bits<U>::ALL;
+        for (int w = 0; w < NW; ++w) {
+            o |= red_or[w];
+            a &= red_and[w];
+        }
+        const U var = U((o ^ a) & rmask);
+        const int lp = var ? (bit_length<U>(var) - 1 - km.begin_bit) / RADIX_BITS : -1;
+        if (lp >= 0) {
+            const int sh = km.begin_bit + lp * RADIX_BITS;
+            const U dm = U((U(1) << ((it.hi_bit - sh) < RADIX_BITS ? (it.hi_bit - sh) : RADIX_BITS)) - 1);
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                if (j * NT + (t & ~63) >= n) break;  // this and later rows: whole wave past the end
+                const bool v = j * NT + t < n;
+                const uint32_t d = (uint32_t)((key[j] >> sh) & dm);
+                const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);
+                const uint64_t act = __ballot(v);
+                if (__ballot(v && d != d0) == 0) {
+                    if ((t & 63) == 0) atomicAdd(&hist[d0], (uint32_t)__popcll(act));
+                } else if (v) {
+                    atomicAdd(&hist[d], 1u);
+                }
+            }
+        }
         __syncthreads();
-        uint32_t* out = sc.hist + (size_t)g * sc.npos_max * RADIX;
-        for (int i = threadIdx.x; i < npos * RADIX; i += NT) out[i] = hist[i];
+        if (t == 0) sc.cinfo[g] = ChunkInfo{(unsigned long long)o, (unsigned long long)a, lp, 0};
+        if (lp >= 0)
+            for (int i = t; i < RADIX; i += NT) sc.hist[(size_t)g * RADIX + i] = hist[i];
         __syncthreads();
     }
 }
 
-// Pass 2 of a level: one workgroup per range. Its SCAN_G groups of 256 threads
-// (one thread per digit) each reduce a contiguous share of the range's chunk
-// histograms; the workgroup then chooses the highest digit position whose
-// digit takes more than one value, writes every chunk's destination offsets
-// (each group for its own share of chunks, from the prefix of the groups
-// before it), and queues the child ranges.
+// Pass 2 of a level: one workgroup per range. The range's OR/AND (from its
+// chunks') give the highest digit position whose bits vary; its SCAN_G groups
+// of 256 threads (one thread per digit) then total that digit over a
+// contiguous share of the chunks each, write every chunk's destination
+// offsets (each group for its own chunks, after the groups before it), and
+// queue the child ranges.
 constexpr int SCAN_NT = 1024;
 constexpr int SCAN_G = SCAN_NT / RADIX;
 
+// Items of digit d in chunk gc at the level's position: the chunk's histogram
+// when that is its own top varying position, else all of its n items or none.
+// The histogram word is read whether or not it is used (it may be stale when
+// the chunk's top is lower), so that both loads of a chunk are in flight at once.
+__device__ __forceinline__ uint32_t chunk_digit_count(const Sched& sc, size_t gc, int pstar, int shift,
+                                                      unsigned long long dmask, uint32_t d, int n) {
+    const uint32_t h = sc.hist[gc * RADIX + d];
+    const ChunkInfo ci = sc.cinfo[gc];
+    if (ci.lp == pstar) return h;
+    return (uint32_t)((ci.kand >> shift) & dmask) == d ? (uint32_t)n : 0u;
+}
+
 __global__ __launch_bounds__(SCAN_NT) void k_large_scan(Sched sc, int level, KeyMode km, ClassCaps cc,
                                                         int tile_max) {
-    __shared__ uint32_t part[SCAN_G][MAX_LEVELS][RADIX];
-    __shared__ uint32_t base_s[RADIX];
+    __shared__ uint32_t part[SCAN_G][RADIX];
+    __shared__ unsigned long long red_or[SCAN_NT / WAVE];
+    __shared__ unsigned long long red_and[SCAN_NT / WAVE];
     __shared__ BlockPush bp;
-    __shared__ uint32_t nz[MAX_LEVELS];
     __shared__ uint32_t wsum[RADIX / WAVE];
-    const int d = threadIdx.x & (RADIX - 1);
-    const int g = threadIdx.x / RADIX;
+    const int t = threadIdx.x;
+    const int d = t & (RADIX - 1);
+    const int g = t / RADIX;
     int count, chunks;
     level_counts(sc, level, count, chunks);
     LargeItem* items = sc.lq[level];
@@ -428,75 +506,65 @@ __global__ __launch_bounds__(SCAN_NT) void k_large_scan(Sched sc, int level, Key
     for (int i = blockIdx.x; i < count; i += gridDim.x) {
         LargeItem it = items[i];
         const int nch = (it.n + sc.chunk - 1) / sc.chunk;
-        const int npos = npos_of(it.hi_bit, km.begin_bit);
         const int c0 = (nch * g) / SCAN_G, c1 = (nch * (g + 1)) / SCAN_G;
-        if (threadIdx.x < MAX_LEVELS) nz[threadIdx.x] = 0;
-        uint32_t tot[MAX_LEVELS];
-#pragma unroll
-        for (int p = 0; p < MAX_LEVELS; ++p) tot[p] = 0;
-#pragma unroll 4
-        for (int c = c0; c < c1; ++c) {
-            const uint32_t* h = sc.hist + (size_t)(it.chunk_base + c) * sc.npos_max * RADIX + d;
-#pragma unroll
-            for (int p = 0; p < MAX_LEVELS; ++p)
-                if (p < npos) tot[p] += h[p * RADIX];
+        auto chunk_n = [&](int c) { return (it.n - c * sc.chunk) < sc.chunk ? (it.n - c * sc.chunk) : sc.chunk; };
+        // the range's OR / AND over its chunks
+        unsigned long long o = 0, a = ~0ull;
+        for (int c = t; c < nch; c += SCAN_NT) {
+            const ChunkInfo ci = sc.cinfo[it.chunk_base + c];
+            o |= ci.kor;
+            a &= ci.kand;
         }
-#pragma unroll
-        for (int p = 0; p < MAX_LEVELS; ++p)
-            if (p < npos) part[g][p][d] = tot[p];
-        __syncthreads();
-        uint32_t all[MAX_LEVELS];
-#pragma unroll
-        for (int p = 0; p < MAX_LEVELS; ++p) {
-            all[p] = 0;
-            if (p < npos)
-                for (int gg = 0; gg < SCAN_G; ++gg) all[p] += part[gg][p][d];
-        }
-        if (g == 0) {
-#pragma unroll
-            for (int p = 0; p < MAX_LEVELS; ++p) {
-                const uint64_t m = __ballot(p < npos && all[p] > 0);
-                if (lane_id() == 0 && m) atomicAdd(&nz[p], (uint32_t)__popcll(m));
-            }
+        o = wave_or(o);
+        a = wave_and(a);
+        if ((t & 63) == 0) {
+            red_or[t >> 6] = o;
+            red_and[t >> 6] = a;
         }
         __syncthreads();
-        int pstar = -1;
-        for (int p = npos - 1; p >= 0; --p)
-            if (nz[p] > 1) {
-                pstar = p;
-                break;
-            }
-        if (pstar < 0) {
-            if (threadIdx.x == 0) {
+        o = 0;
+        a = ~0ull;
+        for (int w = 0; w < SCAN_NT / WAVE; ++w) {
+            o |= red_or[w];
+            a &= red_and[w];
+        }
+        const unsigned long long var = o ^ a;  // already inside [begin_bit, hi_bit)
+        if (var == 0) {
+            if (t == 0) {
                 it.mode = LMODE_COPY;
                 items[i] = it;
             }
             __syncthreads();
             continue;
         }
-        bool final_ = true;
-        for (int p = 0; p < pstar; ++p)
-            if (nz[p] > 1) final_ = false;
+        const int pstar = (bit_length<unsigned long long>(var) - 1 - km.begin_bit) / RADIX_BITS;
         const int shift = km.begin_bit + pstar * RADIX_BITS;
         const int bits = (it.hi_bit - shift) < RADIX_BITS ? (it.hi_bit - shift) : RADIX_BITS;
+        const unsigned long long dmask = (1ull << bits) - 1;
+        const bool final_ = (var & ((1ull << shift) - 1)) == 0;
+        uint32_t tot = 0;
+#pragma unroll 8
+        for (int c = c0; c < c1; ++c) {
+            const size_t gc = (size_t)(it.chunk_base + c);
+            tot += chunk_digit_count(sc, gc, pstar, shift, dmask, (uint32_t)d, chunk_n(c));
+        }
+        part[g][d] = tot;
+        __syncthreads();
         uint32_t mine = 0;
-#pragma unroll
-        for (int p = 0; p < MAX_LEVELS; ++p)
-            if (p == pstar) mine = all[p];
+        for (int gg = 0; gg < SCAN_G; ++gg) mine += part[gg][d];
         const uint32_t incl = wave_incl_scan(mine);
         if (g == 0 && lane_id() == 63) wsum[d >> 6] = incl;
         __syncthreads();
         uint32_t base = incl - mine;
         for (int ww = 0; ww < (d >> 6); ++ww) base += wsum[ww];
-        if (g == 0) base_s[d] = base;
         // this group's chunks start after the digit-d items of the groups before it
         uint32_t run = (uint32_t)it.begin + base;
-        for (int gg = 0; gg < g; ++gg) run += part[gg][pstar][d];
-#pragma unroll 4
+        for (int gg = 0; gg < g; ++gg) run += part[gg][d];
+#pragma unroll 8
         for (int c = c0; c < c1; ++c) {
             const size_t gc = (size_t)(it.chunk_base + c);
             sc.off[gc * RADIX + d] = run;
-            run += sc.hist[gc * sc.npos_max * RADIX + pstar * RADIX + d];
+            run += chunk_digit_count(sc, gc, pstar, shift, dmask, (uint32_t)d, chunk_n(c));
         }
         {
             const int cb = it.begin + (int)base;
@@ -505,7 +573,7 @@ __global__ __launch_bounds__(SCAN_NT) void k_large_scan(Sched sc, int level, Key
             push_tiles_block(sc, bp, cls, make_ti(cb, cls >= 0 ? cn : 0, child_buf));
             if (cn > tile_max) push_large(sc, level + 1, cb, cn, child_buf, shift);
         }
-        if (threadIdx.x == 0) {
+        if (t == 0) {
             it.shift = shift;
             it.bits = bits;
             it.mode = final_ ? LMODE_FINAL : LMODE_PARTITION;

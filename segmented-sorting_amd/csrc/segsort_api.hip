@@ -288,7 +288,7 @@ Plan make_plan(const Tuning& tu) {
 }
 
 struct Layout {
-    size_t ctr, tq[MAX_TILE_CLASSES], lq[MAX_LEVELS + 1], hist, off, tk[2], tv[2];
+    size_t ctr, tq[MAX_TILE_CLASSES], lq[MAX_LEVELS + 1], hist, cinfo, off, tk[2], tv[2];
     size_t total;
     size_t tq_cap[MAX_TILE_CLASSES];
     size_t lq_cap;
@@ -328,7 +328,8 @@ Layout make_layout(const Plan& p, int num_items, int num_segments, int levels, b
             L.lq[0] = take(L.lq_cap * sizeof(LargeItem));  // multi-workgroup level 0
             L.lq[1] = take(L.lq_cap * sizeof(LargeItem));  // ranges for the depth-first tier
         }
-        L.hist = take(L.chunks * C::NPOS * RADIX * sizeof(uint32_t));
+        L.hist = take(L.chunks * RADIX * sizeof(uint32_t));
+        L.cinfo = take(L.chunks * sizeof(ChunkInfo));
         L.off = take(L.chunks * RADIX * sizeof(uint32_t));
     }
     const bool need_tmp = plain ? (N > (size_t)p.tile_max) : (levels > 0);
@@ -409,10 +410,10 @@ struct Launcher {
         } else {
             for (int l = 0; l < MAX_LEVELS; ++l) sc.lq[l] = L.lq[l] ? (LargeItem*)(base + L.lq[l]) : nullptr;
             sc.hist = (uint32_t*)(base + L.hist);
+            sc.cinfo = (ChunkInfo*)(base + L.cinfo);
             sc.off = (uint32_t*)(base + L.off);
         }
         sc.chunk = C::CHUNK;
-        sc.npos_max = C::NPOS;
         for (int c = 0; c < p.nclass; ++c) sc.tq_cap[c] = (uint32_t)std::min<size_t>(L.tq_cap[c], 0xFFFFFFFFu);
         sc.lq_cap = (uint32_t)L.lq_cap;
         sc.chunk_cap = (uint32_t)L.chunks;

@@ -111,6 +111,12 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
     return wave_incl_scan_op(x, 0u, [](uint32_t a, uint32_t b) { return a + b; });
 }
 
+// Maximum of x over the wave, returned to every lane.
+__device__ __forceinline__ uint32_t wave_max(uint32_t x) {
+    x = wave_incl_scan_op(x, 0u, [](uint32_t a, uint32_t b) { return a > b ? a : b; });
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+}
+
 // Sum of x over the wave, returned to every lane.
 __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
     return (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan(x), 63);

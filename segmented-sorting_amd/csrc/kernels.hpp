@@ -283,13 +283,13 @@ __global__ __launch_bounds__(256) void k_wave_tier(Bufs b, Sched sc, int cls, Ke
     const int wid = blockIdx.x * waves_per_block + (threadIdx.x >> 6);
     const int stride = persistent ? gridDim.x * waves_per_block : 0x7FFFFFFF;
     const int count = tile_count(sc, cls);
-    __shared__ V stage[HAS_V ? 4 * 512 : 1];  // 512 values per wave (blocks of 4 waves)
-    V* my_stage = stage + (HAS_V ? (threadIdx.x >> 6) * 512 : 0);
+    __shared__ WaveSmem<U, V> wsm[4];  // one region per wave (blocks of 4 waves)
+    WaveSmem<U, V>& my = wsm[threadIdx.x >> 6];
     for (int i = wid; i < count; i += stride) {
         const TileItem it = sc.tq[cls][i];
         const int src = ti_src(it);
         wave_sort<U, V, HAS_V>((const U*)b.k[src], (const V*)b.v[src], buf_ptr<U>(b.k[BUF_OUT]),
-                               buf_ptr<V>(b.v[BUF_OUT]), it.begin, ti_n(it), km, rank_max, my_stage);
+                               buf_ptr<V>(b.v[BUF_OUT]), it.begin, ti_n(it), km, rank_max, my);
         if (!persistent) break;
     }
 }

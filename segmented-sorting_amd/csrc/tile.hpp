@@ -24,6 +24,9 @@ struct RadixTile {
     static constexpr int NDT = NT < RADIX ? NT : RADIX;  // threads in the digit scan
     static constexpr int DPT = RADIX / NDT;               // digits per scanning thread
     static constexpr int NDW = (NDT + WAVE - 1) / WAVE;   // waves in the digit scan
+    // top-digit pass plus in-bucket compare-count for tiles of up to 1024 items
+    static constexpr bool MSD_FINISH = NW <= 4 && TILE <= 1024;
+    static constexpr int MSD_GUARD = 24;
     static_assert(NT % WAVE == 0, "whole waves");
     static_assert(RADIX % NDT == 0, "digit split");
     static_assert(TILE <= 65536, "16-bit source indices");
@@ -36,7 +39,8 @@ struct RadixTile {
         };
         uint32_t wcnt[NW * RADIX];  // per-wave digit counts, then scatter bases
         uint32_t dummy[NT];         // store target of the lanes that do not advance a count
-        uint32_t dstart[RADIX];     // tile-exclusive start of each digit
+        uint32_t dstart[RADIX + 1]; // tile-exclusive start of each digit (dstart[RADIX]: slots counted)
+        uint32_t maxcnt;            // largest digit count of the last pass over real items (NW <= 4)
         uint32_t goff[RADIX];       // global destination of each digit (partition)
         uint32_t wsum[NDW];
         U red_or[NW];
@@ -161,18 +165,27 @@ struct RadixTile {
                 }
 #pragma unroll
                 for (int i = 0; i < G; ++i) incl[i] = wave_incl_scan(s[i]);
-                uint32_t carry = 0;
+                uint32_t carry = 0, big = 0;
+                // slots past n hold the all-ones key: they are counted in the last digit
+                const uint32_t pad = (uint32_t)(wact * WITEMS - n);
 #pragma unroll
                 for (int i = 0; i < G; ++i) {
                     const int d = i * WAVE + (int)l;
                     uint32_t base = carry + incl[i] - s[i];
                     carry += (uint32_t)__builtin_amdgcn_readlane((int)incl[i], 63);
                     sm.dstart[d] = base;
+                    const uint32_t real = (uint32_t)d == dmask ? s[i] - pad : s[i];
+                    big = real > big ? real : big;
 #pragma unroll
                     for (int ww = 0; ww < NW; ++ww) {
                         if (ww < wact) sm.wcnt[ww * RADIX + d] = base;
                         base += c[i][ww];
                     }
+                }
+                big = wave_max(big);
+                if (l == 0) {
+                    sm.dstart[RADIX] = carry;
+                    sm.maxcnt = big;
                 }
             }
         } else {
@@ -315,6 +328,54 @@ struct RadixTile {
         }
         const int lo = lowest_bit<U>(varying);
         const int hi = bit_length<U>(varying);
+        if constexpr (MSD_FINISH) {
+            // One stable pass on the top digit; when every bucket then holds at
+            // most MSD_GUARD items, each item's rank inside its bucket comes from
+            // comparing it with the bucket's items (smaller key, or equal key at
+            // an earlier position: the pass is stable) instead of three more passes.
+            if (hi - lo > RADIX_BITS) {
+                const int shift = hi - RADIX_BITS;
+                uint32_t dst[IPT];
+                rank_digits(sm, k, n, shift, RADIX_BITS, dst);
+                if (sm.maxcnt <= (uint32_t)MSD_GUARD) {
+                    if (act) {
+#pragma unroll
+                        for (int j = 0; j < IPT; ++j) {
+                            const int p = pos(j);
+                            if (p < n) {
+                                sm.keys[dst[j]] = k[j];
+                                if constexpr (HAS_V) sm.idx[0][dst[j]] = (uint16_t)p;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                    prefetch();
+                    if (act) {
+#pragma unroll
+                        for (int j = 0; j < IPT; ++j) {
+                            const int p = pos(j);
+                            if (p < n) {
+                                const U kk = sm.keys[p];
+                                const uint32_t d = (uint32_t)(kk >> shift) & (RADIX - 1);
+                                const int s0 = (int)sm.dstart[d];
+                                const int e0 = d + 1 < RADIX ? (int)sm.dstart[d + 1] : n;
+                                const int e = e0 < n ? e0 : n;
+                                uint32_t c = 0;
+                                for (int q = s0; q < e; ++q) {
+                                    const U kq = sm.keys[q];
+                                    c += (kq < kk) | ((kq == kk) & (q < p));
+                                }
+                                const int r = s0 + (int)c;
+                                ko[begin + r] = twiddle_out<U>(kk, km.kind, km.desc);
+                                if constexpr (HAS_V) vo[begin + r] = vs[begin + sm.idx[0][p]];
+                            }
+                        }
+                    }
+                    __syncthreads();
+                    return;
+                }
+            }
+        }
         int cur = 0;  // idx buffer holding the source index of each tile position
         for (int shift = lo; shift < hi; shift += RADIX_BITS) {
             const int bits = (hi - shift) < RADIX_BITS ? (hi - shift) : RADIX_BITS;
@@ -736,6 +797,148 @@ __device__ __forceinline__ void wave_bitonic_sort(const U* ks, const V* vs, U* k
     }
 }
 
+// ---- one wave, one counting pass and a compare-count inside the buckets ----------
+
+constexpr int WAVE_SORT_MAX = 512;  // items of one wave-tier range
+
+// The wave's LDS region for wave_msd_sort.
+template <typename U, typename V>
+struct WaveSmem {
+    U keys[WAVE_SORT_MAX];        // raw keys in bucket order
+    V vals[WAVE_SORT_MAX];        // values by original position
+    uint16_t pos[WAVE_SORT_MAX];  // original position of each bucket-ordered key
+    uint32_t cnt[RADIX + 1];      // running digit counts, then bucket starts (cnt[RADIX] = n)
+    uint32_t dummy[WAVE];         // store target of the lanes that do not advance a count
+};
+
+// Lanes whose `bits`-bit digit d equals the calling lane's (see
+// RadixTile::match_digit): per bit, one ballot and a v_bitop3 per 32-lane half.
+__device__ __forceinline__ uint64_t wave_match_rt(uint32_t d, int bits) {
+    uint32_t acc_lo = ~0u, acc_hi = ~0u;
+    for (int b = 0; b < bits; ++b) {
+        uint32_t t = (uint32_t)__builtin_amdgcn_sbfe((int)d, b, 1);
+        __asm__("" : "+v"(t));
+        const uint64_t bal = __ballot(t != 0);
+        acc_lo = __builtin_amdgcn_bitop3_b32(acc_lo, (uint32_t)bal, t, 0x90);
+        acc_hi = __builtin_amdgcn_bitop3_b32(acc_hi, (uint32_t)(bal >> 32), t, 0x90);
+    }
+    return (uint64_t(acc_hi) << 32) | uint64_t(acc_lo);
+}
+
+// One wave sorts a range of n <= 64*R items: a stable counting pass on the
+// highest (up to) 8 varying bits of the order keys, then, unless that digit
+// held every varying bit, each item's rank inside its bucket by comparing it
+// with the bucket's other items (smaller key, or equal key earlier in the
+// bucket, which after the stable pass is earlier in the input). Returns false,
+// having written nothing, when a bucket exceeds `guard` items (then a network
+// sort is cheaper). Lanes of one wave need no barrier: its LDS operations
+// complete in order.
+template <int R, typename U, typename V, bool HAS_V>
+__device__ bool wave_msd_sort(const U* ks, const V* vs, U* ko, V* vo, int64_t begin, int n, const KeyMode& km,
+                              WaveSmem<U, V>& ws, int guard) {
+    const int l = (int)lane_id();
+    const U rm = range_mask<U>(km.begin_bit, km.end_bit);
+    U raw[R], ok[R];
+    V v[R];
+    U o = 0, a = Bits<U>::ALL;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int p = r * WAVE + l;
+        const int pc = p < n ? p : n - 1;
+        raw[r] = ks[begin + pc];
+        if constexpr (HAS_V) v[r] = vs[begin + pc];
+        ok[r] = U(twiddle_in<U>(raw[r], km.kind, km.desc) & rm);
+        if (p < n) {
+            o |= ok[r];
+            a &= ok[r];
+        }
+    }
+    o = wave_or(o);
+    a = wave_and(a);
+    const U var = U(o ^ a);
+    if (var == 0) {  // every key equal within the bit range: already in order
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int p = r * WAVE + l;
+            if (p < n) {
+                ko[begin + p] = raw[r];
+                if constexpr (HAS_V) vo[begin + p] = v[r];
+            }
+        }
+        return true;
+    }
+    const int hi = bit_length<U>(var), lo = lowest_bit<U>(var);
+    const int shift = hi - RADIX_BITS > lo ? hi - RADIX_BITS : lo;
+    const uint32_t dm = (1u << (hi - shift)) - 1u;
+#pragma unroll
+    for (int i = 0; i < RADIX / WAVE; ++i) ws.cnt[i * WAVE + l] = 0;
+    uint32_t rk[R], dg[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const bool valid = r * WAVE + l < n;
+        dg[r] = (uint32_t)(ok[r] >> shift) & dm;
+        const uint32_t base = ws.cnt[dg[r]];
+        const uint64_t peers = wave_match_rt(dg[r], hi - shift) & __ballot(valid);
+        const uint32_t below = lanes_below(peers);
+        uint32_t* dst = (valid && below == 0) ? &ws.cnt[dg[r]] : &ws.dummy[l];
+        *dst = base + (uint32_t)__popcll(peers);
+        rk[r] = base + below;
+    }
+    // bucket starts: lane l scans digits l, l+64, l+128, l+192
+    uint32_t carry = 0, big = 0;
+#pragma unroll
+    for (int i = 0; i < RADIX / WAVE; ++i) {
+        const uint32_t c = ws.cnt[i * WAVE + l];
+        const uint32_t incl = wave_incl_scan(c);
+        ws.cnt[i * WAVE + l] = carry + incl - c;
+        carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+        big = c > big ? c : big;
+    }
+    if (l == 0) ws.cnt[RADIX] = (uint32_t)n;
+    const bool single = shift == lo;  // this digit held every varying bit
+    if (!single && wave_max(big) > (uint32_t)guard) return false;
+    uint32_t dst[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[r] = ws.cnt[dg[r]] + rk[r];
+    if (single) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (r * WAVE + l < n) {
+                ko[begin + dst[r]] = raw[r];
+                if constexpr (HAS_V) vo[begin + dst[r]] = v[r];
+            }
+        }
+        return true;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int p = r * WAVE + l;
+        if (p < n) {
+            ws.keys[dst[r]] = raw[r];
+            ws.pos[dst[r]] = (uint16_t)p;
+            if constexpr (HAS_V) ws.vals[p] = v[r];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int q0 = r * WAVE + l;
+        if (q0 < n) {
+            const U kr = ws.keys[q0];
+            const U kk = U(twiddle_in<U>(kr, km.kind, km.desc) & rm);
+            const uint32_t d = (uint32_t)(kk >> shift) & dm;
+            const int s = (int)ws.cnt[d], e = (int)ws.cnt[d + 1];
+            uint32_t c = 0;
+            for (int q = s; q < e; ++q) {
+                const U kq = U(twiddle_in<U>(ws.keys[q], km.kind, km.desc) & rm);
+                c += (kq < kk) | ((kq == kk) & (q < q0));
+            }
+            ko[begin + s + (int)c] = kr;
+            if constexpr (HAS_V) vo[begin + s + (int)c] = ws.vals[ws.pos[q0]];
+        }
+    }
+    return true;
+}
+
 // One wave sorts a range of n <= 64 items stably: the rank of item l is the
 // number of items whose masked order key is smaller, or equal and earlier.
 template <typename U, typename V, bool HAS_V>
@@ -759,17 +962,25 @@ __device__ __forceinline__ void wave_rank_sort(const U* ks, const V* vs, U* ko, 
 }
 
 // Sort one range of n <= 512 items with one wave: compare-count for the
-// shortest ranges, else the bitonic network on the smallest power of two
-// of words per lane that holds n (values staged in the wave's 512 LDS
-// entries at `stage`).
+// shortest ranges, a bitonic network up to 64 items, above that one counting
+// pass plus a compare-count inside the buckets (the network on the smallest
+// power of two of words per lane when a bucket is too large).
 template <typename U, typename V, bool HAS_V>
 __device__ __forceinline__ void wave_sort(const U* ks, const V* vs, U* ko, V* vo, int64_t begin, int n,
-                                          const KeyMode& km, int rank_max, V* stage) {
+                                          const KeyMode& km, int rank_max, WaveSmem<U, V>& ws) {
+    constexpr int GUARD = 16;  // largest bucket the compare-count finish takes on
     if (n <= rank_max) wave_rank_sort<U, V, HAS_V>(ks, vs, ko, vo, begin, n, km);
-    else if (n <= 64) wave_bitonic_sort<1, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km, stage);
-    else if (n <= 128) wave_bitonic_sort<2, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km, stage);
-    else if (n <= 256) wave_bitonic_sort<4, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km, stage);
-    else wave_bitonic_sort<8, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km, stage);
+    else if (n <= 64) wave_bitonic_sort<1, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km, ws.vals);
+    else if (n <= 128) {
+        if (!wave_msd_sort<2, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km, ws, GUARD))
+            wave_bitonic_sort<2, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km, ws.vals);
+    } else if (n <= 256) {
+        if (!wave_msd_sort<4, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km, ws, GUARD))
+            wave_bitonic_sort<4, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km, ws.vals);
+    } else {
+        if (!wave_msd_sort<8, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km, ws, GUARD))
+            wave_bitonic_sort<8, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km, ws.vals);
+    }
 }
 
 }  // namespace ssort

@@ -42,14 +42,14 @@ def _parse(sig: str):
 
 
 # LDS tile classes (bits 0..3) and wave-group classes (bits 4..7: groups of 4, 8, 16, 32 lanes)
-TILE_MASKS = (0xF, 0xE, 0xD, 0xC, 0x7, 0x6)
+TILE_MASKS = (0xF, 0xB, 0x7, 0xD, 0xE, 0x3)
 GROUP_MASKS = (0x00, 0xE0, 0xA0, 0x60, 0xF0)
 
 
 def candidates(quick: bool = False):
     wcs, tms, gms = (128, 256, 512), TILE_MASKS, GROUP_MASKS
     if quick:
-        wcs, tms, gms = (128, 512), (0xF, 0xC), (0x00, 0xE0)
+        wcs, tms, gms = (128, 512), (0xF, 0xB), (0x00, 0xE0)
     for wc, tm, gm_, mult in itertools.product(wcs, tms, gms, (1,) if quick else (1, 2)):
         yield {"wave_cap": wc, "tile_enable": tm | gm_, "grid_mult": mult, "concurrent": 0}
 

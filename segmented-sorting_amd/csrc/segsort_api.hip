@@ -233,9 +233,9 @@ SideStreams& side_streams() {
 template <typename U, typename V, bool HAS_V>
 struct Cfg {
     static constexpr int ITEM_BYTES = int(sizeof(U)) + (HAS_V ? int(sizeof(V)) : 0);
-    static constexpr int NT0 = 64, IPT0 = 4;
-    static constexpr int NT1 = 256, IPT1 = 4;
-    static constexpr int NT2 = 256, IPT2 = 16;
+    static constexpr int NT0 = 256, IPT0 = 4;
+    static constexpr int NT1 = 256, IPT1 = 16;
+    static constexpr int NT2 = 512, IPT2 = (sizeof(U) == 4 && ITEM_BYTES <= 8) ? 16 : 8;
     static constexpr int NT3 = 1024, IPT3 = ITEM_BYTES <= 8 ? 16 : 8;
     static constexpr int CAP[4] = {NT0 * IPT0, NT1 * IPT1, NT2 * IPT2, NT3 * IPT3};
     static constexpr int TILE_MAX = NT3 * IPT3;

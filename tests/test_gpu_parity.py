@@ -164,6 +164,25 @@ def test_bit_ranges_descending(pkg, torch_cuda, orc):
     assert lengths.sum() == keys.size
 
 
+def test_bit_ranges_every_tier(pkg, torch_cuda, orc):
+    """Bit sub-ranges and descending order on lengths that reach every class
+    (wave groups, wave tier, each LDS tile, the MSD tiers), 32- and 64-bit keys."""
+    rng = np.random.default_rng(22)
+    lengths = np.concatenate([rng.integers(2, 40, 400), rng.integers(40, 600, 100), rng.integers(600, 5000, 20),
+                              [1024, 4096, 20000, 70000]])
+    rng.shuffle(lengths)
+    off = pkg.gen.offsets_from_lengths(lengths)
+    n = int(off[-1])
+    raw = pkg.gen.u64(22, 1, n)
+    vals = np.arange(n, dtype=np.uint32)
+    for keys in ((raw >> np.uint64(32)).astype(np.uint32), raw):
+        w = keys.dtype.itemsize * 8
+        for bb, eb in ((0, w), (4, 12), (8, w), (w - 12, w), (5, w - 7)):
+            for desc in (False, True):
+                check(pkg, torch_cuda, orc, keys, vals, off, begin_bit=bb, end_bit=eb, descending=desc,
+                      algos=("batched", "plain"))
+
+
 def test_gaps_unordered_and_empty(pkg, torch_cuda, orc):
     """Non-contiguous, unordered segments with gaps and end <= begin: positions outside
     the segments are never written."""

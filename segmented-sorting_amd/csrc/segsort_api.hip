@@ -315,8 +315,6 @@ Layout make_layout(const Plan& p, int num_items, int num_segments, int levels, b
         plain ? 0 : std::min((size_t)RADIX * L.lq_cap, std::max<size_t>(N, 1)) * (size_t)C::NPOS;
     L.ctr = take(sizeof(Counters));
     for (int c = 0; c < p.nclass; ++c) {
-        const size_t lo = c == 0 ? 1 : (size_t)p.cc.cap[c - 1] + 1;
-        (void)lo;
         L.tq_cap[c] = S + bucket_bound;
         L.tq[c] = take(L.tq_cap[c] * sizeof(TileItem));
     }

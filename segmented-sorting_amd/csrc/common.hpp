@@ -25,20 +25,23 @@ struct Bits {
 // Map a raw key to the unsigned pattern whose integer order is the sort order:
 // unsigned identity, signed ^ sign bit, float (neg ? ~b : b ^ sign), and every
 // bit flipped for a descending sort (util_type.cuh:904-911, 947-955, 1017-1027).
+// Written as one XOR with a mask built from the (uniform) kind and direction,
+// without branches, so that loads feeding it can all be in flight at once.
 template <typename U>
 __device__ __forceinline__ U twiddle_in(U b, int kind, bool desc) {
-    U r = b;
-    if (kind == KIND_SIGNED) r = U(b ^ Bits<U>::HIGH);
-    else if (kind == KIND_FLOAT) r = U(b ^ ((b & Bits<U>::HIGH) ? Bits<U>::ALL : Bits<U>::HIGH));
-    return desc ? U(~r) : r;
+    const U neg = U(U(0) - U(b >> (Bits<U>::N - 1)));  // all ones when the sign bit is set
+    const U fmask = U(neg | Bits<U>::HIGH);
+    const U m = kind == KIND_FLOAT ? fmask : (kind == KIND_SIGNED ? Bits<U>::HIGH : U(0));
+    return U(b ^ m ^ (desc ? Bits<U>::ALL : U(0)));
 }
 
 template <typename U>
 __device__ __forceinline__ U twiddle_out(U r, int kind, bool desc) {
-    if (desc) r = U(~r);
-    if (kind == KIND_SIGNED) return U(r ^ Bits<U>::HIGH);
-    if (kind == KIND_FLOAT) return U(r ^ ((r & Bits<U>::HIGH) ? Bits<U>::HIGH : Bits<U>::ALL));
-    return r;
+    const U a = U(r ^ (desc ? Bits<U>::ALL : U(0)));
+    const U neg = U(U(0) - U(a >> (Bits<U>::N - 1)));
+    const U fmask = U(U(~neg) | Bits<U>::HIGH);  // sign set: the sign bit, else all ones
+    const U m = kind == KIND_FLOAT ? fmask : (kind == KIND_SIGNED ? Bits<U>::HIGH : U(0));
+    return U(a ^ m);
 }
 
 // How the keys of one call are ordered: key class, direction and the
@@ -85,6 +88,22 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
 constexpr int DPP_ROW_SHR = 0x110;
 constexpr int DPP_ROW_BCAST15 = 0x142;
 constexpr int DPP_ROW_BCAST31 = 0x143;
+constexpr int DPP_WAVE_SHL1 = 0x130;  // lane l reads lane l + 1 (lane 63 keeps `old`)
+constexpr int DPP_WAVE_SHR1 = 0x138;  // lane l reads lane l - 1 (lane 0 keeps `old`)
+
+// The value of the neighbouring lane (l - 1 for SHR, l + 1 for SHL) of a 32-
+// or 64-bit word; the lane without a neighbour gets `edge`.
+template <int CTRL, typename W>
+__device__ __forceinline__ W lane_shift(W x, W edge) {
+    if constexpr (sizeof(W) == 4) {
+        return (W)__builtin_amdgcn_update_dpp((int)edge, (int)x, CTRL, 0xF, 0xF, false);
+    } else {
+        const int lo = __builtin_amdgcn_update_dpp((int)(uint32_t)edge, (int)(uint32_t)x, CTRL, 0xF, 0xF, false);
+        const int hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(edge >> 32), (int)(uint32_t)(x >> 32), CTRL, 0xF,
+                                                   0xF, false);
+        return (W(uint32_t(hi)) << 32) | W(uint32_t(lo));
+    }
+}
 
 // One step of a DPP scan: the value of the source lane, or `identity` where
 // the control has no source lane or the row is masked off.

@@ -275,9 +275,10 @@ __global__ void k_bin_large_ids(const int* __restrict__ d_begin, const int* __re
 // ---- tile tiers --------------------------------------------------------------
 
 // Wave tier: each wave sorts one queued range of n <= 512 items (compare-count
-// up to rank_max items, a register bitonic network above).
+// up to rank_max items, a register bitonic network above). The per-wave LDS
+// area allows 6 waves per SIMD; the register budget is held to the same.
 template <typename U, typename V, bool HAS_V>
-__global__ __launch_bounds__(256) void k_wave_tier(Bufs b, Sched sc, int cls, KeyMode km, int persistent,
+__global__ __launch_bounds__(256, 6) void k_wave_tier(Bufs b, Sched sc, int cls, KeyMode km, int persistent,
                                                    int rank_max) {
     const int waves_per_block = blockDim.x / WAVE;
     const int wid = blockIdx.x * waves_per_block + (threadIdx.x >> 6);

@@ -28,6 +28,14 @@ struct RadixTile {
     // (measured slower than three more passes at 4096)
     static constexpr bool MSD_FINISH = NW <= 4 && TILE <= 1024;
     static constexpr int MSD_GUARD = 24;
+    // larger tiles: two passes on the top 16 varying bits, then a compare-count
+    // inside each run of equal top bits (runs of at most TOP_GUARD items)
+    // (keys only: when a 16-bit place per slot still fits in LDS)
+    static constexpr bool TOP2_FINISH =
+        !MSD_FINISH && (HAS_V || size_t(TILE) * (sizeof(U) + 2) + size_t(NW) * RADIX * 4 + size_t(NT) * 4 + 4096 <=
+                                     size_t(160 * 1024));
+    static constexpr int TOP_GUARD = 16;
+    static_assert(IPT <= 32, "one bit per slot");
     static_assert(NT % WAVE == 0, "whole waves");
     static_assert(RADIX % NDT == 0, "digit split");
     static_assert(TILE <= 65536, "16-bit source indices");
@@ -36,7 +44,9 @@ struct RadixTile {
         U keys[TILE];
         union {
             V vals[HAS_V ? TILE : 1];                       // partition(): values ride along
-            uint16_t idx[HAS_V ? 2 : 1][HAS_V ? TILE : 1];  // sort(): source index per item, ping-pong
+            // sort(): source index per item, ping-pong; the top-16 finish keeps
+            // each slot's final place in the last array (keys only: the only one)
+            uint16_t idx[HAS_V ? 2 : 1][(HAS_V || TOP2_FINISH) ? TILE : 1];
         };
         uint32_t wcnt[NW * RADIX];  // per-wave digit counts, then scatter bases
         uint32_t dummy[NT];         // store target of the lanes that do not advance a count
@@ -50,6 +60,15 @@ struct RadixTile {
 
     __device__ static __forceinline__ int wave() { return threadIdx.x >> 6; }
     __device__ static __forceinline__ int pos(int j) { return wave() * WITEMS + j * WAVE + (threadIdx.x & 63); }
+    // pos(0) tied to the range size, so that the per-slot addresses derived
+    // from it are computed per range instead of being hoisted out of a
+    // persistent caller's loop (where they would outlive the range in
+    // registers, or spill)
+    __device__ static __forceinline__ int lane_base(int n) {
+        int b = wave() * WITEMS + (threadIdx.x & 63);
+        __asm__("" : "+v"(b) : "s"(n));
+        return b;
+    }
     __device__ static __forceinline__ int active_waves(int n) { return (n + WITEMS - 1) / WITEMS; }
 
     // Lanes of this wave whose item has the same NBITS-bit digit d as the
@@ -190,34 +209,30 @@ struct RadixTile {
                 }
             }
         } else {
-            uint32_t tot[DPT];
-            uint32_t run = 0;
-            if (t < NDT) {
+            // thread d < RADIX owns digit d: all of its per-wave counts are read
+            // at once (one LDS round trip), summed, scanned across the digits,
+            // and the per-wave bases written back
+            static_assert(NDT == RADIX && DPT == 1, "one digit per scanning thread");
+            uint32_t c[NW];
+            uint32_t s = 0;
+            if (t < RADIX) {
 #pragma unroll
-                for (int i = 0; i < DPT; ++i) {
-                    const int d = t * DPT + i;
-                    uint32_t s = 0;
-                    for (int ww = 0; ww < wact; ++ww) {
-                        const uint32_t c = sm.wcnt[ww * RADIX + d];
-                        sm.wcnt[ww * RADIX + d] = s;
-                        s += c;
-                    }
-                    tot[i] = run;
-                    run += s;
-                }
+                for (int ww = 0; ww < NW; ++ww) c[ww] = ww < wact ? sm.wcnt[ww * RADIX + t] : 0u;
+#pragma unroll
+                for (int ww = 0; ww < NW; ++ww) s += c[ww];
             }
-            const uint32_t incl = wave_incl_scan(run);
-            if (t < NDT && l == 63) sm.wsum[w] = incl;
+            const uint32_t incl = wave_incl_scan(s);
+            if (t < RADIX && l == 63) sm.wsum[w] = incl;
             __syncthreads();
-            if (t < NDT) {
-                uint32_t excl = incl - run;
-                for (int ww = 0; ww < w; ++ww) excl += sm.wsum[ww];
+            if (t < RADIX) {
+                uint32_t base = incl - s;
 #pragma unroll
-                for (int i = 0; i < DPT; ++i) {
-                    const int d = t * DPT + i;
-                    const uint32_t base = excl + tot[i];
-                    sm.dstart[d] = base;
-                    for (int ww = 0; ww < wact; ++ww) sm.wcnt[ww * RADIX + d] += base;
+                for (int ww = 0; ww < NDW; ++ww) base += ww < w ? sm.wsum[ww] : 0u;
+                sm.dstart[t] = base;
+#pragma unroll
+                for (int ww = 0; ww < NW; ++ww) {
+                    if (ww < wact) sm.wcnt[ww * RADIX + t] = base;
+                    base += c[ww];
                 }
             }
         }
@@ -235,9 +250,10 @@ struct RadixTile {
     // element (always in bounds) and are then marked with the all-ones key.
     __device__ static __forceinline__ void load_keys(const U* ks, int64_t begin, int n, const KeyMode& km,
                                                      U (&k)[IPT]) {
+        const int lb = lane_base(n);
 #pragma unroll
         for (int j = 0; j < IPT; ++j) {
-            const int p = pos(j);
+            const int p = (lb + j * WAVE);
             const U raw = ks[begin + (p < n ? p : n - 1)];
             k[j] = p < n ? twiddle_in<U>(raw, km.kind, km.desc) : Bits<U>::ALL;
         }
@@ -245,11 +261,12 @@ struct RadixTile {
 
     // OR and AND of the valid keys over the whole tile (every thread gets them).
     __device__ static void reduce_or_and(Smem& sm, const U (&k)[IPT], int n, U& o, U& a) {
+        const int lb = lane_base(n);
         U vo = 0, va = Bits<U>::ALL;
         if (wave() < active_waves(n)) {
 #pragma unroll
             for (int j = 0; j < IPT; ++j) {
-                const bool v = pos(j) < n;
+                const bool v = (lb + j * WAVE) < n;
                 vo |= v ? k[j] : U(0);
                 va &= v ? k[j] : Bits<U>::ALL;
             }
@@ -277,7 +294,7 @@ struct RadixTile {
     // are ranked, so a low-entropy tile costs fewer passes. Values do not move
     // through the passes: a 16-bit source index does, and each value is read
     // once from its source at the end.
-    __device__ static void sort(Smem& sm, const U* ks, const V* vs, U* ko, V* vo, int64_t begin, int n,
+    __device__ static __forceinline__ void sort(Smem& sm, const U* ks, const V* vs, U* ko, V* vo, int64_t begin, int n,
                                 const KeyMode& km) {
         U raw[IPT];
         load_raw(ks, begin, n, raw);
@@ -291,23 +308,237 @@ struct RadixTile {
     // overlap the current range's write-out. Workgroup barriers wait on LDS
     // only, so the loads stay in flight across them.
     __device__ static __forceinline__ void load_raw(const U* ks, int64_t begin, int n, U (&raw)[IPT]) {
+        const int lb = lane_base(n);
         if (wave() < active_waves(n)) {
 #pragma unroll
             for (int j = 0; j < IPT; ++j) {
-                const int p = pos(j);
+                const int p = (lb + j * WAVE);
                 raw[j] = ks[begin + (p < n ? p : n - 1)];
             }
         }
     }
 
+    // Two stable passes on the varying bits [hi - 16, hi) order the tile by its
+    // top 16 varying bits; items with equal top bits then form short runs, and
+    // each item's final place is its run's start plus the number of run items
+    // with a smaller key or an equal key at an earlier position (the passes
+    // are stable). Returns false, with nothing written to global memory and
+    // the prefetch hook not run, when some run is longer than TOP_GUARD items.
     template <typename Prefetch>
-    __device__ static void sort_loaded(Smem& sm, const U (&raw)[IPT], const U* ks, const V* vs, U* ko, V* vo,
+    __device__ static __forceinline__ bool top2_finish(Smem& sm, U (&k)[IPT], const V* vs, U* ko, V* vo, int64_t begin, int n,
+                                       const KeyMode& km, int hi, Prefetch& prefetch) {
+        const int lb = lane_base(n);
+        const bool act = wave() < active_waves(n);
+        const int sh = hi - 2 * RADIX_BITS;
+        uint32_t dst[IPT];
+        rank_digits(sm, k, n, sh, RADIX_BITS, dst);
+        if (act) {
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int p = (lb + j * WAVE);
+                if (p < n) {
+                    sm.keys[dst[j]] = k[j];
+                    if constexpr (HAS_V) sm.idx[1][dst[j]] = (uint16_t)p;
+                }
+            }
+        }
+        __syncthreads();
+        if (act) {
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int p = (lb + j * WAVE);
+                k[j] = p < n ? sm.keys[p] : Bits<U>::ALL;
+            }
+        }
+        rank_digits(sm, k, n, sh + RADIX_BITS, RADIX_BITS, dst);
+        if (act) {
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int p = (lb + j * WAVE);
+                if (p < n) {
+                    sm.keys[dst[j]] = k[j];
+                    if constexpr (HAS_V) sm.idx[0][dst[j]] = sm.idx[1][p];
+                }
+            }
+        }
+        __syncthreads();
+        // the order covers only the key bits [begin_bit, end_bit); above hi
+        // those bits are the same in every item
+        const U rm = range_mask<U>(km.begin_bit, km.end_bit);
+        auto top = [&](U x) { return U((x & rm) >> sh); };
+        // bit j of `in_run`: the item in slot j shares its top bits with a
+        // neighbour; most items do not, and their place is their slot. The
+        // neighbours' top bits come from the adjacent lanes (slot j's lane 63
+        // is followed by slot j + 1's lane 0), and from LDS only across waves.
+        const uint32_t l = threadIdx.x & 63;
+        uint16_t* const place = sm.idx[HAS_V ? 1 : 0];  // free after the passes
+        uint32_t in_run = 0;
+        if (act) {
+            U t[IPT];
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int p = lb + j * WAVE;
+                t[j] = top(sm.keys[p < n ? p : n - 1]);
+            }
+            const int w0 = lb - (int)l;  // first slot of the wave
+            const U before_wave = top(sm.keys[w0 > 0 ? w0 - 1 : 0]);
+            const int w1 = w0 + IPT * WAVE;  // first slot after the wave
+            const U after_wave = top(sm.keys[w1 < n ? w1 : n - 1]);
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int p = lb + j * WAVE;
+                const U prev_last = j > 0 ? read_lane<U>(t[j > 0 ? j - 1 : 0], 63) : before_wave;
+                const U next_first = j + 1 < IPT ? read_lane<U>(t[j + 1 < IPT ? j + 1 : j], 0) : after_wave;
+                const U tl = lane_shift<DPP_WAVE_SHR1>(t[j], prev_last);
+                const U tr = lane_shift<DPP_WAVE_SHL1>(t[j], next_first);
+                const bool valid = p < n;
+                in_run |= uint32_t(valid & (((p > 0) & (tl == t[j])) | ((p + 1 < n) & (tr == t[j])))) << j;
+            }
+        }
+        // Items in runs are queued per wave (in the wave's digit-count words,
+        // free now: QCAP 16-bit entries) and ranked one per lane; the place
+        // of slot p goes to place[p]. A run longer than TOP_GUARD stops the
+        // finish before anything is written to global memory. LDS operations
+        // of one wave complete in order, so the queue needs no barrier.
+        int long_run = 0;
+        if (act) {
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int p = lb + j * WAVE;
+                if (p < n && !((in_run >> j) & 1u)) place[p] = (uint16_t)p;
+            }
+            constexpr uint32_t QCAP = 2 * RADIX;
+            uint16_t* queue = reinterpret_cast<uint16_t*>(sm.wcnt + wave() * RADIX);
+            int j = 0;
+            while (j < IPT) {
+                uint32_t cnt = 0;
+                for (; j < IPT; ++j) {
+                    const int p = lb + j * WAVE;
+                    const bool queued = p < n && ((in_run >> j) & 1u);
+                    const uint64_t m = __ballot(queued);
+                    const uint32_t c = (uint32_t)__popcll(m);
+                    if (cnt + c > QCAP) break;
+                    if (queued) queue[cnt + lanes_below(m)] = (uint16_t)p;
+                    cnt += c;
+                }
+                for (uint32_t e0 = 0; e0 < cnt; e0 += WAVE) {
+                    if (e0 + l < cnt) {
+                        const int p = queue[e0 + l];
+                        const U kk = sm.keys[p];
+                        const U km_ = U(kk & rm);
+                        const U tk = top(kk);
+                        // the run (contiguous, holds p) seen through a window of
+                        // +-3 slots; runs reaching the window's edge are walked
+                        constexpr int WIN = 3;
+                        uint32_t before = 0, c = 0;
+                        bool edge = false;
+#pragma unroll
+                        for (int d = -WIN; d <= WIN; ++d) {
+                            if (d == 0) continue;
+                            const int q = p + d;
+                            const bool in = q >= 0 && q < n;
+                            const U kq = sm.keys[in ? q : p];
+                            const U kqm = U(kq & rm);
+                            const bool same = in & (top(kq) == tk);
+                            before += uint32_t(same & (d < 0));
+                            c += uint32_t(same & ((kqm < km_) | ((kqm == km_) & (d < 0))));
+                            if (d == -WIN || d == WIN) edge |= same;
+                        }
+                        uint32_t r = (uint32_t)p - before + c;
+                        if (edge) {
+                            // bounded walks: a run longer than TOP_GUARD stops them
+                            int s = p;
+                            while (s > 0 && top(sm.keys[s - 1]) == tk) {
+                                --s;
+                                if (p - s > TOP_GUARD) break;
+                            }
+                            int e = p + 1;
+                            while (e < n && top(sm.keys[e]) == tk) {
+                                ++e;
+                                if (e - s > TOP_GUARD) break;
+                            }
+                            if (e - s > TOP_GUARD) {
+                                long_run = 1;
+                                r = (uint32_t)p;
+                            } else {
+                                c = 0;
+                                for (int q = s; q < e; ++q) {
+                                    const U kq = U(sm.keys[q] & rm);
+                                    c += (kq < km_) | ((kq == km_) & (q < p));
+                                }
+                                r = (uint32_t)s + c;
+                            }
+                        }
+                        place[p] = (uint16_t)r;
+                    }
+                }
+            }
+        }
+        if (__syncthreads_or(long_run)) return false;
+        prefetch();
+        constexpr bool STAGE = HAS_V && sizeof(V) <= sizeof(U);
+        V v[IPT];
+        if (act && STAGE) {
+            // the values' global read overlaps the keys' write-out
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int p = lb + j * WAVE;
+                v[j] = vs[begin + (p < n ? p : n - 1)];  // unconditional: the loads issue together
+            }
+        }
+        if (act) {
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int p = lb + j * WAVE;
+                const int pc = p < n ? p : n - 1;
+                const U kk = sm.keys[pc];
+                const uint32_t r = place[pc];
+                if (p < n) ko[begin + r] = twiddle_out<U>(kk, km.kind, km.desc);
+            }
+        }
+        if constexpr (STAGE) {
+            // values: staged in the key area of LDS (free once every run is
+            // ranked), then moved to their final places
+            __syncthreads();
+            V* stage = reinterpret_cast<V*>(sm.keys);
+            if (act) {
+#pragma unroll
+                for (int j = 0; j < IPT; ++j) {
+                    const int p = lb + j * WAVE;
+                    if (p < n) stage[p] = v[j];
+                }
+            }
+            __syncthreads();
+            if (act) {
+#pragma unroll
+                for (int j = 0; j < IPT; ++j) {
+                    const int p = lb + j * WAVE;
+                    if (p < n) vo[begin + place[p]] = stage[sm.idx[0][p]];
+                }
+            }
+        } else if constexpr (HAS_V) {
+            __syncthreads();
+            if (act) {
+#pragma unroll
+                for (int j = 0; j < IPT; ++j) {
+                    const int p = lb + j * WAVE;
+                    if (p < n) vo[begin + place[p]] = vs[begin + sm.idx[0][p]];
+                }
+            }
+        }
+        __syncthreads();
+        return true;
+    }
+
+    template <typename Prefetch>
+    __device__ static __forceinline__ void sort_loaded(Smem& sm, const U (&raw)[IPT], const U* ks, const V* vs, U* ko, V* vo,
                                        int64_t begin, int n, const KeyMode& km, Prefetch prefetch) {
+        const int lb = lane_base(n);
         const bool act = wave() < active_waves(n);
         U k[IPT];
         if (act) {
 #pragma unroll
-            for (int j = 0; j < IPT; ++j) k[j] = pos(j) < n ? twiddle_in<U>(raw[j], km.kind, km.desc) : Bits<U>::ALL;
+            for (int j = 0; j < IPT; ++j) k[j] = (lb + j * WAVE) < n ? twiddle_in<U>(raw[j], km.kind, km.desc) : Bits<U>::ALL;
         }
         U o, a;
         reduce_or_and(sm, k, n, o, a);
@@ -317,7 +548,7 @@ struct RadixTile {
             if (act) {
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
-                    const int p = pos(j);
+                    const int p = (lb + j * WAVE);
                     if (p < n) {
                         ko[begin + p] = twiddle_out<U>(k[j], km.kind, km.desc);
                         if constexpr (HAS_V) vo[begin + p] = vs[begin + p];
@@ -342,7 +573,7 @@ struct RadixTile {
                     if (act) {
 #pragma unroll
                         for (int j = 0; j < IPT; ++j) {
-                            const int p = pos(j);
+                            const int p = (lb + j * WAVE);
                             if (p < n) {
                                 sm.keys[dst[j]] = k[j];
                                 if constexpr (HAS_V) sm.idx[0][dst[j]] = (uint16_t)p;
@@ -356,7 +587,7 @@ struct RadixTile {
                         const U rm = range_mask<U>(km.begin_bit, km.end_bit);
 #pragma unroll
                         for (int j = 0; j < IPT; ++j) {
-                            const int p = pos(j);
+                            const int p = (lb + j * WAVE);
                             if (p < n) {
                                 const U kk = sm.keys[p];
                                 const U km_ = U(kk & rm);
@@ -380,6 +611,14 @@ struct RadixTile {
                 }
             }
         }
+        if constexpr (TOP2_FINISH) {
+            if (hi - lo > 2 * RADIX_BITS) {
+                if (top2_finish(sm, k, vs, ko, vo, begin, n, km, hi, prefetch)) return;
+                // a run of equal top bits is too long: the full LSD below, from
+                // the original keys (read again; the registers hold the passes')
+                if (act) load_keys(ks, begin, n, km, k);
+            }
+        }
         int cur = 0;  // idx buffer holding the source index of each tile position
         for (int shift = lo; shift < hi; shift += RADIX_BITS) {
             const int bits = (hi - shift) < RADIX_BITS ? (hi - shift) : RADIX_BITS;
@@ -388,7 +627,7 @@ struct RadixTile {
             if (act) {
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
-                    const int p = pos(j);
+                    const int p = (lb + j * WAVE);
                     if (p < n) {
                         sm.keys[dst[j]] = k[j];
                         if constexpr (HAS_V) sm.idx[cur ^ 1][dst[j]] = shift == lo ? (uint16_t)p : sm.idx[cur][p];
@@ -400,7 +639,7 @@ struct RadixTile {
             if (act && shift + RADIX_BITS < hi) {
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
-                    const int p = pos(j);
+                    const int p = (lb + j * WAVE);
                     k[j] = p < n ? sm.keys[p] : Bits<U>::ALL;  // slots past n stay last
                 }
             }
@@ -416,12 +655,12 @@ struct RadixTile {
             if (act) {
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
-                    const int p = pos(j);
-                    if (p < n) v[j] = vs[begin + p];
+                    const int p = (lb + j * WAVE);
+                    v[j] = vs[begin + (p < n ? p : n - 1)];  // unconditional: the loads issue together
                 }
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
-                    const int p = pos(j);
+                    const int p = (lb + j * WAVE);
                     if (p < n) {
                         srcs[j] = sm.idx[cur][p];
                         ko[begin + p] = twiddle_out<U>(sm.keys[p], km.kind, km.desc);
@@ -433,13 +672,13 @@ struct RadixTile {
             if (act) {
 #pragma unroll
                 for (int j = 0; j < IPT; ++j)
-                    if (pos(j) < n) stage[pos(j)] = v[j];
+                    if ((lb + j * WAVE) < n) stage[(lb + j * WAVE)] = v[j];
             }
             __syncthreads();
             if (act) {
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
-                    const int p = pos(j);
+                    const int p = (lb + j * WAVE);
                     if (p < n) vo[begin + p] = stage[srcs[j]];
                 }
             }
@@ -447,7 +686,7 @@ struct RadixTile {
             if (act) {
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
-                    const int p = pos(j);
+                    const int p = (lb + j * WAVE);
                     if (p < n) {
                         ko[begin + p] = twiddle_out<U>(sm.keys[p], km.kind, km.desc);
                         if constexpr (HAS_V) vo[begin + p] = vs[begin + sm.idx[cur][p]];
@@ -484,6 +723,7 @@ struct RadixTile {
 
     __device__ static void partition_body(Smem& sm, const U* ks, const V* vs, U* kd, V* vd, int64_t begin, int n,
                                           const KeyMode& km, int shift, int bits, const uint32_t* goff) {
+        const int lb = lane_base(n);
         const bool act = wave() < active_waves(n);
         U k[IPT];
         V v[IPT];
@@ -492,7 +732,7 @@ struct RadixTile {
             if constexpr (HAS_V) {
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
-                    const int p = pos(j);
+                    const int p = (lb + j * WAVE);
                     v[j] = vs[begin + (p < n ? p : n - 1)];
                 }
             }
@@ -502,7 +742,7 @@ struct RadixTile {
         if (act) {
 #pragma unroll
             for (int j = 0; j < IPT; ++j)
-                if (pos(j) < n) {
+                if ((lb + j * WAVE) < n) {
                     sm.keys[dst[j]] = k[j];
                     if constexpr (HAS_V) sm.vals[dst[j]] = v[j];
                 }
@@ -512,7 +752,7 @@ struct RadixTile {
         if (act) {
 #pragma unroll
             for (int j = 0; j < IPT; ++j) {
-                const int p = pos(j);
+                const int p = (lb + j * WAVE);
                 if (p < n) {
                     const U key = sm.keys[p];
                     const uint32_t d = (uint32_t)((key >> shift) & dmask);

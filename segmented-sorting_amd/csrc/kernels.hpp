@@ -334,7 +334,7 @@ __global__ __launch_bounds__(256) void k_group_tier(Bufs b, Sched sc, int cls, K
 // At least 4 waves per SIMD (16 per CU) so that LDS and memory latency can be
 // hidden; this caps the tile kernels at 128 VGPRs.
 template <int NT, int IPT, typename U, typename V, bool HAS_V>
-__global__ __launch_bounds__(NT, 1024 / NT) void k_tile_tier(Bufs b, Sched sc, int cls, KeyMode km, int persistent) {
+__global__ __launch_bounds__(NT, 4) void k_tile_tier(Bufs b, Sched sc, int cls, KeyMode km, int persistent) {
     using T = RadixTile<NT, IPT, U, V, HAS_V>;
     __shared__ typename T::Smem sm;
     const int count = tile_count(sc, cls);
@@ -342,6 +342,7 @@ __global__ __launch_bounds__(NT, 1024 / NT) void k_tile_tier(Bufs b, Sched sc, i
     if (i >= count) return;
     if constexpr (IPT > 4) {
         // no register room for a second set of keys: one range at a time
+        // (measured: pipelining the 16-item tiles spills and is slower)
         for (; i < count; i += persistent ? (int)gridDim.x : count) {
             const TileItem cur = sc.tq[cls][i];
             const int src = ti_src(cur);

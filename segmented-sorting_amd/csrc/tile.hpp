@@ -409,18 +409,15 @@ struct RadixTile {
             }
             constexpr uint32_t QCAP = 2 * RADIX;
             uint16_t* queue = reinterpret_cast<uint16_t*>(sm.wcnt + wave() * RADIX);
-            int j = 0;
-            while (j < IPT) {
-                uint32_t cnt = 0;
-                for (; j < IPT; ++j) {
-                    const int p = lb + j * WAVE;
-                    const bool queued = p < n && ((in_run >> j) & 1u);
-                    const uint64_t m = __ballot(queued);
-                    const uint32_t c = (uint32_t)__popcll(m);
-                    if (cnt + c > QCAP) break;
-                    if (queued) queue[cnt + lanes_below(m)] = (uint16_t)p;
-                    cnt += c;
-                }
+            // which lanes queue slot j: wave-uniform masks and counts
+            uint64_t mq[IPT];
+            uint32_t total = 0;
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                mq[j] = __ballot((lb + j * WAVE) < n && ((in_run >> j) & 1u));
+                total += (uint32_t)__popcll(mq[j]);
+            }
+            auto rank_queue = [&](uint32_t cnt) {
                 for (uint32_t e0 = 0; e0 < cnt; e0 += WAVE) {
                     if (e0 + l < cnt) {
                         const int p = queue[e0 + l];
@@ -471,6 +468,26 @@ struct RadixTile {
                         }
                         place[p] = (uint16_t)r;
                     }
+                }
+            };
+            auto push = [&](int j, uint32_t& at) {
+                if ((mq[j] >> l) & 1u) queue[at + lanes_below(mq[j])] = (uint16_t)(lb + j * WAVE);
+                at += (uint32_t)__popcll(mq[j]);
+            };
+            if (total <= QCAP) {
+                uint32_t at = 0;
+#pragma unroll
+                for (int j = 0; j < IPT; ++j) push(j, at);
+                rank_queue(total);
+            } else {
+                // more queued items than entries: QCAP / 64 slots at a time
+                constexpr int QS = int(QCAP / WAVE);
+#pragma unroll
+                for (int h = 0; h < IPT; h += QS) {
+                    uint32_t at = 0;
+#pragma unroll
+                    for (int j = h; j < h + QS && j < IPT; ++j) push(j, at);
+                    rank_queue(at);
                 }
             }
         }

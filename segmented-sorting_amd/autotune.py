@@ -6,8 +6,8 @@ most G = 4, 8, 16, 32 items per wave), the wave tier up to `wave_cap` items, the
 the enabled LDS tile classes (`tile_enable` bits 0..3, capacities from
 segsort_tile_caps), and the multi-workgroup MSD tier above the largest enabled
 class; `grid_mult` sizes the persistent grids.
-(`concurrent`, forking the tile classes onto parallel streams, is only honoured
-for direct launches and is not searched.)
+`concurrent` (the binned ranges sorted on a second stream while the MSD tier
+runs; direct launches only) is tried on the finalists of the screening stage.
 
 `tune()` times every candidate on a resident workload and keeps the fastest one
 whose output is bit-identical to the default tuning's. Results are stored in
@@ -168,8 +168,9 @@ def _score(runs):
 
 def main(argv=None):
     """Two stages per signature: every candidate timed briefly on every workload of
-    the signature, then the best few re-timed with six times the steps; the stored
-    tuning minimises the sum of times relative to each workload's best."""
+    the signature, then the best few re-timed with six times the steps, each with
+    `concurrent` off and on; the stored tuning minimises the sum of times relative
+    to each workload's best."""
     import argparse
 
     import torch
@@ -200,7 +201,7 @@ def main(argv=None):
     for sig, runs in by_sig.items():
         score = _score([r[2] for r in runs])
         ranked = sorted((c for c, v in score.items() if v is not None), key=lambda c: score[c])
-        finalists = [json.loads(c) for c in ranked[:args.finalists]]
+        finalists = [dict(json.loads(c), concurrent=k) for c in ranked[:args.finalists] for k in (0, 1)]
         refined = []
         for wl, dev, _ in runs:
             _, res = tune(pkg, torch, *dev, steps=args.steps * 6, verbose=False, cands=finalists)

@@ -8,14 +8,18 @@
 namespace ssort {
 
 constexpr int MAX_TILE_CLASSES = 9;  // wave-group classes, the wave tier, the LDS tiles
+// Queue q < MAX_TILE_CLASSES holds class q's ranges; when a call splits them,
+// MAX_TILE_CLASSES + c holds the ranges that the MSD tier queues for class c
+// (so the binned ranges can be sorted while the MSD tier still runs).
+constexpr int MAX_QUEUES = 2 * MAX_TILE_CLASSES;
 constexpr int MAX_LEVELS = 8;        // MSD levels of the large tier (64-bit keys)
 
 // Device-resident counters of one call; cleared by k_init at the start.
 struct Counters {
-    uint32_t tile_count[MAX_TILE_CLASSES];
+    uint32_t tile_count[MAX_QUEUES];
     uint32_t pad0[2];
     unsigned long long level[MAX_LEVELS + 1];  // (items << 32) | chunks, per MSD level
-    unsigned long long tile_items[MAX_TILE_CLASSES];  // elements queued per class
+    unsigned long long tile_items[MAX_QUEUES];  // elements queued per queue
     unsigned long long level_items[MAX_LEVELS + 1];   // elements queued per MSD level
     uint32_t claim;                                   // next task of the large tier
     uint32_t scans_done[MAX_LEVELS + 1];              // ranges scanned, per MSD level
@@ -43,7 +47,7 @@ struct ChunkInfo {
 // Where the queues of one call live (all inside the caller's temp storage).
 struct Sched {
     Counters* ctr;
-    TileItem* tq[MAX_TILE_CLASSES];
+    TileItem* tq[MAX_QUEUES];
     LargeItem* lq[MAX_LEVELS + 1];
     uint32_t* hist;  // [chunk][RADIX], the histogram of the chunk's highest varying digit
     ChunkInfo* cinfo;  // [chunk]
@@ -51,9 +55,10 @@ struct Sched {
     int chunk;       // items per large-tier chunk
     // Capacities: a push beyond them is dropped and readers clamp their counts,
     // so overlapping (invalid) segments cannot write outside the temp storage.
-    uint32_t tq_cap[MAX_TILE_CLASSES];
+    uint32_t tq_cap[MAX_QUEUES];
     uint32_t lq_cap;
     uint32_t chunk_cap;
+    int child_q;  // queue of class c's MSD children: child_q + c (0: the binned ranges' queue)
 };
 
 __device__ __forceinline__ int tile_count(const Sched& sc, int cls) {
@@ -115,13 +120,14 @@ __device__ __forceinline__ void push_tiles_block(const Sched& sc, BlockPush& bp,
     if (threadIdx.x < MAX_TILE_CLASSES) {
         const int c = threadIdx.x;
         const uint32_t k = bp.cnt[c];
-        bp.base[c] = k ? atomicAdd(&sc.ctr->tile_count[c], k) : 0u;
-        if (k) atomicAdd(&sc.ctr->tile_items[c], bp.items[c]);
+        bp.base[c] = k ? atomicAdd(&sc.ctr->tile_count[sc.child_q + c], k) : 0u;
+        if (k) atomicAdd(&sc.ctr->tile_items[sc.child_q + c], bp.items[c]);
     }
     __syncthreads();
     if (cls >= 0) {
+        const int q = sc.child_q + cls;
         const uint32_t g = bp.base[cls] + slot;
-        if (g < sc.tq_cap[cls]) sc.tq[cls][g] = it;
+        if (g < sc.tq_cap[q]) sc.tq[q][g] = it;
     }
 }
 

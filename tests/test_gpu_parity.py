@@ -226,10 +226,12 @@ TUNINGS = [
 ]
 
 
+@pytest.mark.parametrize("concurrent", [1, 0], ids=["overlap", "one-stream"])
 @pytest.mark.parametrize("tuning", TUNINGS, ids=lambda t: f"wc{t['wave_cap']}-te{t['tile_enable']:02x}")
-def test_tuning_variants(pkg, torch_cuda, orc, tuning):
+def test_tuning_variants(pkg, torch_cuda, orc, tuning, concurrent):
     """Every class layout the tuning can select sorts bit-exactly (batched and plain),
-    for 32- and 64-bit keys, on lengths that hit every group, wave, tile and MSD path."""
+    for 32- and 64-bit keys, on lengths that hit every group, wave, tile and MSD path,
+    with the binned ranges sorted on a second stream during the MSD tier or not."""
     rng = np.random.default_rng(31)
     lengths = np.concatenate([rng.integers(0, 40, 3000), rng.integers(40, 600, 200), rng.integers(600, 20000, 20),
                               [0, 1, 2, 3, 4, 5, 8, 9, 16, 17, 32, 33, 64, 65, 70000]])
@@ -240,7 +242,7 @@ def test_tuning_variants(pkg, torch_cuda, orc, tuning):
     befores = {(kw, vb): pkg.get_tuning(kw, vb) for kw, vb in ((4, 4), (8, 4), (4, 0))}
     try:
         for kw, vb in befores:
-            pkg.set_tuning(kw, vb, tuning["wave_cap"], tuning["tile_enable"], tuning["grid_mult"], 0)
+            pkg.set_tuning(kw, vb, tuning["wave_cap"], tuning["tile_enable"], tuning["grid_mult"], concurrent)
         k32 = (raw >> np.uint64(40)).astype(np.uint32)  # 24-bit keys: ties inside segments
         check(pkg, torch_cuda, orc, k32, np.arange(n, dtype=np.uint32), off, algos=("batched", "plain"))
         check(pkg, torch_cuda, orc, k32.view(np.float32), None, off, algos=("batched", "plain"))

@@ -280,22 +280,24 @@ __global__ void k_bin_large_ids(const int* __restrict__ d_begin, const int* __re
 
 // ---- tile tiers --------------------------------------------------------------
 
-// Wave tier: each wave sorts one queued range of n <= 512 items (compare-count
-// up to rank_max items, a register bitonic network above). The per-wave LDS
-// area allows 6 waves per SIMD; the register budget is held to the same.
-template <typename U, typename V, bool HAS_V>
-__global__ __launch_bounds__(256, 6) void k_wave_tier(Bufs b, Sched sc, int cls, KeyMode km, int persistent,
+// Wave tier: each wave sorts one queued range of n <= MAXN items (compare-count
+// up to rank_max items, a register bitonic network above). For MAXN = 512 the
+// per-wave LDS area allows 6 waves per SIMD and the register budget is held to
+// the same; ranges of at most 256 items need half the area and fewer
+// registers, and aim at 8.
+template <typename U, typename V, bool HAS_V, int MAXN>
+__global__ __launch_bounds__(256, MAXN <= 256 ? 8 : 6) void k_wave_tier(Bufs b, Sched sc, int cls, KeyMode km, int persistent,
                                                    int rank_max) {
     const int waves_per_block = blockDim.x / WAVE;
     const int wid = blockIdx.x * waves_per_block + (threadIdx.x >> 6);
     const int stride = persistent ? gridDim.x * waves_per_block : 0x7FFFFFFF;
     const int count = tile_count(sc, cls);
-    __shared__ WaveSmem<U, V> wsm[4];  // one region per wave (blocks of 4 waves)
-    WaveSmem<U, V>& my = wsm[threadIdx.x >> 6];
+    __shared__ WaveSmem<U, V, MAXN> wsm[4];  // one region per wave (blocks of 4 waves)
+    WaveSmem<U, V, MAXN>& my = wsm[threadIdx.x >> 6];
     for (int i = wid; i < count; i += stride) {
         const TileItem it = sc.tq[cls][i];
         const int src = ti_src(it);
-        wave_sort<U, V, HAS_V>((const U*)b.k[src], (const V*)b.v[src], buf_ptr<U>(b.k[BUF_OUT]),
+        wave_sort<U, V, HAS_V, MAXN>((const U*)b.k[src], (const V*)b.v[src], buf_ptr<U>(b.k[BUF_OUT]),
                                buf_ptr<V>(b.v[BUF_OUT]), it.begin, ti_n(it), km, rank_max, my);
         if (!persistent) break;
     }

@@ -367,11 +367,13 @@ struct Launcher {
         if (g > 0) SEGSORT_LAUNCH(name, st, cls, PAIR_BYTES, 0.0, (k<<<g, 256, 0, st>>>(b, sc, cls, km)));
     }
 
+    // wave_max: the class's longest range (for the wave tier's build)
     static void launch_class(int kind, Bufs b, Sched sc, int cls, KeyMode km, hipStream_t st, int grid, bool persistent,
-                             int grid_mult) {
+                             int grid_mult, int wave_max) {
         switch (kind) {
         case 0: {
-            auto k = k_wave_tier<U, V, HAS_V>;
+            // ranges of at most 256 items: the smaller-LDS, higher-occupancy build
+            auto k = wave_max <= 256 ? k_wave_tier<U, V, HAS_V, 256> : k_wave_tier<U, V, HAS_V, 512>;
             const int g = persistent ? resident_blocks(k, 256) * grid_mult : (grid + 3) / 4;
             if (g > 0)
                 SEGSORT_LAUNCH("wave_tier", st, cls, PAIR_BYTES, 0.0,
@@ -484,7 +486,8 @@ struct Launcher {
         if (split) {
             if ((e = hipEventRecord(ss->fork, st)) != hipSuccess) return e;
             if ((e = hipStreamWaitEvent(ss->s[0], ss->fork, 0)) != hipSuccess) return e;
-            for (int c = 0; c < p.nclass; ++c) launch_class(p.kind[c], b, sc, c, km, ss->s[0], 0, true, tu.grid_mult);
+            for (int c = 0; c < p.nclass; ++c)
+                launch_class(p.kind[c], b, sc, c, km, ss->s[0], 0, true, tu.grid_mult, p.cc.cap[c]);
             if ((e = hipEventRecord(ss->join[0], ss->s[0])) != hipSuccess) return e;
         }
         if (levels > 0) {
@@ -502,7 +505,7 @@ struct Launcher {
                            (kl<<<gl, C::NT3, 0, st>>>(b, sc, 1, km, p.cc, p.tile_max)));
         }
         for (int c = 0; c < p.nclass; ++c)
-            launch_class(p.kind[c], b, sc, sc.child_q + c, km, st, 0, true, tu.grid_mult);
+            launch_class(p.kind[c], b, sc, sc.child_q + c, km, st, 0, true, tu.grid_mult, p.cc.cap[c]);
         if (split && (e = hipStreamWaitEvent(st, ss->join[0], 0)) != hipSuccess) return e;
         profile_attribute(mark, sc, p.nclass, st);
         return hipGetLastError();
@@ -565,7 +568,7 @@ struct Launcher {
         e = hipStreamSynchronize(st);
         if (e != hipSuccess) return fail(e, "segsort_sort(plain): sync");
         for (int c = 0; c < p.nclass; ++c)
-            if (h.tile_count[c]) launch_class(p.kind[c], b, sc, c, km, st, (int)h.tile_count[c], false, 1);
+            if (h.tile_count[c]) launch_class(p.kind[c], b, sc, c, km, st, (int)h.tile_count[c], false, 1, p.cc.cap[c]);
         const int nlarge = (int)h.level[MAX_LEVELS];
         if (nlarge > 0) {
             if (km.end_bit > km.begin_bit)

@@ -1062,12 +1062,12 @@ __device__ __forceinline__ void wave_bitonic_sort(const U* ks, const V* vs, U* k
 
 constexpr int WAVE_SORT_MAX = 512;  // items of one wave-tier range
 
-// The wave's LDS region for wave_msd_sort.
-template <typename U, typename V>
+// The wave's LDS region for wave_msd_sort, for ranges of at most MAXN items.
+template <typename U, typename V, int MAXN = WAVE_SORT_MAX>
 struct WaveSmem {
-    U keys[WAVE_SORT_MAX];        // raw keys in bucket order
-    V vals[WAVE_SORT_MAX];        // values by original position
-    uint16_t pos[WAVE_SORT_MAX];  // original position of each bucket-ordered key
+    U keys[MAXN];        // raw keys in bucket order
+    V vals[MAXN];        // values by original position
+    uint16_t pos[MAXN];  // original position of each bucket-ordered key
     uint32_t cnt[RADIX + 1];      // running digit counts, then bucket starts (cnt[RADIX] = n)
     uint32_t dummy[WAVE];         // store target of the lanes that do not advance a count
 };
@@ -1094,9 +1094,9 @@ __device__ __forceinline__ uint64_t wave_match_rt(uint32_t d, int bits) {
 // having written nothing, when a bucket exceeds `guard` items (then a network
 // sort is cheaper). Lanes of one wave need no barrier: its LDS operations
 // complete in order.
-template <int R, typename U, typename V, bool HAS_V>
+template <int R, typename U, typename V, bool HAS_V, int MAXN>
 __device__ bool wave_msd_sort(const U* ks, const V* vs, U* ko, V* vo, int64_t begin, int n, const KeyMode& km,
-                              WaveSmem<U, V>& ws, int guard) {
+                              WaveSmem<U, V, MAXN>& ws, int guard) {
     const int l = (int)lane_id();
     const U rm = range_mask<U>(km.begin_bit, km.end_bit);
     U raw[R], ok[R];
@@ -1222,24 +1222,25 @@ __device__ __forceinline__ void wave_rank_sort(const U* ks, const V* vs, U* ko, 
     }
 }
 
-// Sort one range of n <= 512 items with one wave: compare-count for the
-// shortest ranges, a bitonic network up to 64 items, above that one counting
-// pass plus a compare-count inside the buckets (the network on the smallest
-// power of two of words per lane when a bucket is too large).
-template <typename U, typename V, bool HAS_V>
+// Sort one range of n <= MAXN (256 or 512) items with one wave: compare-count
+// for the shortest ranges, a bitonic network up to 64 items, above that one
+// counting pass plus a compare-count inside the buckets (the network on the
+// smallest power of two of words per lane when a bucket is too large).
+template <typename U, typename V, bool HAS_V, int MAXN>
 __device__ __forceinline__ void wave_sort(const U* ks, const V* vs, U* ko, V* vo, int64_t begin, int n,
-                                          const KeyMode& km, int rank_max, WaveSmem<U, V>& ws) {
+                                          const KeyMode& km, int rank_max, WaveSmem<U, V, MAXN>& ws) {
+    static_assert(MAXN == 256 || MAXN == 512, "wave-tier range limit");
     constexpr int GUARD = 16;  // largest bucket the compare-count finish takes on
     if (n <= rank_max) wave_rank_sort<U, V, HAS_V>(ks, vs, ko, vo, begin, n, km);
     else if (n <= 64) wave_bitonic_sort<1, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km, ws.vals);
     else if (n <= 128) {
-        if (!wave_msd_sort<2, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km, ws, GUARD))
+        if (!wave_msd_sort<2, U, V, HAS_V, MAXN>(ks, vs, ko, vo, begin, n, km, ws, GUARD))
             wave_bitonic_sort<2, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km, ws.vals);
-    } else if (n <= 256) {
-        if (!wave_msd_sort<4, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km, ws, GUARD))
+    } else if (MAXN == 256 || n <= 256) {
+        if (!wave_msd_sort<4, U, V, HAS_V, MAXN>(ks, vs, ko, vo, begin, n, km, ws, GUARD))
             wave_bitonic_sort<4, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km, ws.vals);
-    } else {
-        if (!wave_msd_sort<8, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km, ws, GUARD))
+    } else if constexpr (MAXN > 256) {
+        if (!wave_msd_sort<8, U, V, HAS_V, MAXN>(ks, vs, ko, vo, begin, n, km, ws, GUARD))
             wave_bitonic_sort<8, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km, ws.vals);
     }
 }

@@ -318,6 +318,29 @@ struct RadixTile {
         }
     }
 
+    // Whether runs of equal top 16 varying bits are likely short enough for the
+    // top-16 finish. Among the range's first 64 keys (a sample of unsorted
+    // input), the pairs with equal top 12 bits estimate sum(p^2) over the
+    // 12-bit values, and 1/16 of that (the next 4 bits spread inside a 12-bit
+    // value) the same for 16 bits; n times it is the expected number of items
+    // that share an item's top 16 bits. The finish is taken while that stays
+    // at most 2: pairs * n <= 2 * 16 * (64 * 63 / 2). Uniform keys see 0.5
+    // sample pairs on average (a 16K tile goes elsewhere with probability
+    // about 0.2%); skewed keys, such as floats of a normal distribution (whose
+    // top 16 bits are the sign, the exponent and 7 mantissa bits), see tens
+    // and keep the full LSD instead of overrunning the finish's guard. Every
+    // wave reads the same sample, so the answer is uniform over the tile.
+    __device__ static __forceinline__ bool top2_promising(Smem& sm, int n, const KeyMode& km, int hi) {
+        constexpr int SAMPLE_BITS = 12;
+        const U rm = range_mask<U>(km.begin_bit, km.end_bit);
+        const uint32_t l = threadIdx.x & 63;
+        const U kk = sm.keys[l];
+        const uint32_t t = (uint32_t)(U((kk & rm) >> (hi - SAMPLE_BITS)) & U((1u << SAMPLE_BITS) - 1));
+        const uint64_t peers = match_digit<SAMPLE_BITS>(t);
+        const uint32_t pairs = wave_sum((uint32_t)__popcll(peers) - 1u) / 2u;  // every pair counted twice
+        return (uint64_t)pairs * (uint64_t)n <= 2u * 16u * (64u * 63u / 2u);
+    }
+
     // Two stable passes on the varying bits [hi - 16, hi) order the tile by its
     // top 16 varying bits; items with equal top bits then form short runs, and
     // each item's final place is its run's start plus the number of run items
@@ -557,6 +580,11 @@ struct RadixTile {
 #pragma unroll
             for (int j = 0; j < IPT; ++j) k[j] = (lb + j * WAVE) < n ? twiddle_in<U>(raw[j], km.kind, km.desc) : Bits<U>::ALL;
         }
+        // the range's first 64 keys, read back by every wave after the barrier
+        // in reduce_or_and (the key area is not used before the first pass)
+        if constexpr (TOP2_FINISH) {
+            if (wave() == 0) sm.keys[threadIdx.x] = k[0];
+        }
         U o, a;
         reduce_or_and(sm, k, n, o, a);
         const U varying = U((o ^ a) & range_mask<U>(km.begin_bit, km.end_bit));
@@ -629,7 +657,7 @@ struct RadixTile {
             }
         }
         if constexpr (TOP2_FINISH) {
-            if (hi - lo > 2 * RADIX_BITS) {
+            if (hi - lo > 2 * RADIX_BITS && top2_promising(sm, n, km, hi)) {
                 if (top2_finish(sm, k, vs, ko, vo, begin, n, km, hi, prefetch)) return;
                 // a run of equal top bits is too long: the full LSD below, from
                 // the original keys (read again; the registers hold the passes')

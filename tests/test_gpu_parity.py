@@ -183,6 +183,41 @@ def test_bit_ranges_every_tier(pkg, torch_cuda, orc):
                       algos=("batched", "plain"))
 
 
+def _runs_segment(rng, n, run_len, head_uniform):
+    """n u32 keys whose top 16 bits come in runs of about run_len items (low 16 bits
+    random), optionally preceded by 64 keys with uniformly random top bits."""
+    tops = rng.integers(0, 1 << 16, max(1, n // max(run_len, 1)) + 1, dtype=np.uint32)
+    keys = (np.repeat(tops, max(run_len, 1))[:n] << np.uint32(16)) | rng.integers(0, 1 << 16, n, dtype=np.uint32)
+    rng.shuffle(keys)
+    if head_uniform:
+        keys[:64] = rng.integers(0, 1 << 32, 64, dtype=np.uint64).astype(np.uint32)
+    return keys.astype(np.uint32)
+
+
+def test_top16_finish_paths(pkg, torch_cuda, orc):
+    """Tiles of more than 1024 items whose keys take each path of the top-16 finish:
+    uniform keys (items alone in their run, and +-3 windows), runs of 4..16 equal top
+    bits (bounded walks), runs longer than 16 behind a uniform-looking first 64 keys
+    (the finish starts and falls back to the full LSD from the original keys), and
+    skewed keys from the start (the sample keeps the full LSD); with ties in the low
+    bits, for every tile size and descending order."""
+    rng = np.random.default_rng(41)
+    segs = []
+    for n in (1100, 2048, 4096, 6000, 8192, 12000, 16384):
+        for run_len, head in ((1, False), (3, False), (9, False), (15, True), (40, True), (300, False)):
+            segs.append(_runs_segment(rng, n, run_len, head))
+    ties = rng.integers(0, 1 << 16, 3000, dtype=np.uint32) << np.uint32(16)
+    ties[::7] = ties[0]  # equal full keys inside a run
+    segs.append(ties)
+    lengths = np.array([s.size for s in segs])
+    keys = np.concatenate(segs)
+    off = pkg.gen.offsets_from_lengths(lengths)
+    vals = np.arange(keys.size, dtype=np.uint32)
+    for desc in (False, True):
+        check(pkg, torch_cuda, orc, keys, vals, off, descending=desc, algos=("batched", "plain"))
+    check(pkg, torch_cuda, orc, keys, None, off, algos=("batched",))  # keys only
+
+
 def test_gaps_unordered_and_empty(pkg, torch_cuda, orc):
     """Non-contiguous, unordered segments with gaps and end <= begin: positions outside
     the segments are never written."""

@@ -203,12 +203,12 @@ hipError_t run_graph(const GraphKey& key, hipStream_t st, F&& enqueue) {
 }
 
 // Per-thread, per-device second stream and fork/join events of the batched
-// form (only s[0] and join[0] are used). All streams keep the default priority.
+// form's `concurrent` mode. The stream keeps the default priority.
 struct SideStreams {
     bool ok = false;
-    hipStream_t s[ssort::MAX_TILE_CLASSES];
+    hipStream_t s;
     hipEvent_t fork;
-    hipEvent_t join[ssort::MAX_TILE_CLASSES];
+    hipEvent_t join;
 };
 
 SideStreams& side_streams() {
@@ -216,12 +216,9 @@ SideStreams& side_streams() {
     const int dev = current_device() & 15;
     SideStreams& ss = per_dev[dev];
     if (!ss.ok) {
-        bool good = hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming) == hipSuccess;
-        for (int c = 0; c < ssort::MAX_TILE_CLASSES && good; ++c) {
-            good = hipStreamCreateWithFlags(&ss.s[c], hipStreamNonBlocking) == hipSuccess &&
-                   hipEventCreateWithFlags(&ss.join[c], hipEventDisableTiming) == hipSuccess;
-        }
-        ss.ok = good;
+        ss.ok = hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming) == hipSuccess &&
+                hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking) == hipSuccess &&
+                hipEventCreateWithFlags(&ss.join, hipEventDisableTiming) == hipSuccess;
     }
     return ss;
 }
@@ -485,10 +482,10 @@ struct Launcher {
         hipError_t e = hipSuccess;
         if (split) {
             if ((e = hipEventRecord(ss->fork, st)) != hipSuccess) return e;
-            if ((e = hipStreamWaitEvent(ss->s[0], ss->fork, 0)) != hipSuccess) return e;
+            if ((e = hipStreamWaitEvent(ss->s, ss->fork, 0)) != hipSuccess) return e;
             for (int c = 0; c < p.nclass; ++c)
-                launch_class(p.kind[c], b, sc, c, km, ss->s[0], 0, true, tu.grid_mult, p.cc.cap[c]);
-            if ((e = hipEventRecord(ss->join[0], ss->s[0])) != hipSuccess) return e;
+                launch_class(p.kind[c], b, sc, c, km, ss->s, 0, true, tu.grid_mult, p.cc.cap[c]);
+            if ((e = hipEventRecord(ss->join, ss->s)) != hipSuccess) return e;
         }
         if (levels > 0) {
             auto kh = k_large_hist<C::NTC, C::IPTC, U, V, HAS_V>;
@@ -506,7 +503,7 @@ struct Launcher {
         }
         for (int c = 0; c < p.nclass; ++c)
             launch_class(p.kind[c], b, sc, sc.child_q + c, km, st, 0, true, tu.grid_mult, p.cc.cap[c]);
-        if (split && (e = hipStreamWaitEvent(st, ss->join[0], 0)) != hipSuccess) return e;
+        if (split && (e = hipStreamWaitEvent(st, ss->join, 0)) != hipSuccess) return e;
         profile_attribute(mark, sc, p.nclass, st);
         return hipGetLastError();
     }

@@ -27,25 +27,35 @@ def _setup(pkg, torch):
     return keys, vals, off, kout, vout, call
 
 
-def test_caller_captures_and_replays(pkg, torch_cuda, orc):
+@pytest.mark.parametrize("concurrent", [0, 1], ids=["one-stream", "overlap"])
+def test_caller_captures_and_replays(pkg, torch_cuda, orc, concurrent):
+    """With `concurrent` on, the direct call forks a second stream (its result is checked
+    too) and the captured call stays on the caller's stream."""
     torch = torch_cuda
     keys, vals, off, kout, vout, call = _setup(pkg, torch)
     ek, ev = orc.seg_sort(keys, vals, offsets=off)
-    s = torch.cuda.Stream()
-    with torch.cuda.stream(s):
-        call(s.cuda_stream)  # warm-up outside the capture
-    torch.cuda.synchronize()
-    g = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g, stream=s):
-        call(torch.cuda.current_stream().cuda_stream)
-    for rep in range(3):
-        kout.fill_(0)  # eager work on the device between replays
-        vout.fill_(0)
+    before = pkg.get_tuning(4, 4)
+    pkg.set_tuning(4, 4, before["wave_cap"], before["tile_enable"], before["grid_mult"], concurrent)
+    try:
+        s = torch.cuda.Stream()
+        with torch.cuda.stream(s):
+            call(s.cuda_stream)  # warm-up outside the capture
         torch.cuda.synchronize()
-        g.replay()
-        torch.cuda.synchronize()
-        assert np.array_equal(kout.cpu().numpy().view(np.uint32), ek), rep
-        assert np.array_equal(vout.cpu().numpy().view(np.uint32), ev), rep
+        assert np.array_equal(kout.cpu().numpy().view(np.uint32), ek)
+        assert np.array_equal(vout.cpu().numpy().view(np.uint32), ev)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            call(torch.cuda.current_stream().cuda_stream)
+        for rep in range(3):
+            kout.fill_(0)  # eager work on the device between replays
+            vout.fill_(0)
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            assert np.array_equal(kout.cpu().numpy().view(np.uint32), ek), rep
+            assert np.array_equal(vout.cpu().numpy().view(np.uint32), ev), rep
+    finally:
+        pkg.set_tuning(4, 4, **before)
 
 
 def test_library_graph_replay_matches_direct(pkg, torch_cuda, orc):

@@ -344,21 +344,36 @@ __global__ __launch_bounds__(256) void k_group_tier(Bufs b, Sched sc, int cls, K
 template <int NT, int IPT, typename U, typename V, bool HAS_V>
 __global__ __launch_bounds__(NT, 4) void k_tile_tier(Bufs b, Sched sc, int cls, KeyMode km, int persistent) {
     using T = RadixTile<NT, IPT, U, V, HAS_V>;
+    if constexpr (IPT > 4) {
+        // One range at a time (no register room for a second set of keys;
+        // measured: pipelining the 16-item tiles spills and is slower). In
+        // tiles of up to 4096 items a range of at most half the tile is sorted
+        // by the build with IPT / 2 items per thread, so that every wave takes
+        // part and each wave's serial ranking steps halve (in larger tiles the
+        // second build made the code spill and slowed full-size ranges).
+        using T2 = RadixTile<NT, IPT / 2, U, V, HAS_V>;
+        __shared__ union {
+            typename T::Smem full;
+            typename T2::Smem half;
+        } sm;
+        const int count = tile_count(sc, cls);
+        for (int i = blockIdx.x; i < count; i += persistent ? (int)gridDim.x : count) {
+            const TileItem cur = sc.tq[cls][i];
+            const int src = ti_src(cur);
+            const int n = ti_n(cur);
+            const U* ks = (const U*)b.k[src];
+            const V* vs = (const V*)b.v[src];
+            U* ko = buf_ptr<U>(b.k[BUF_OUT]);
+            V* vo = buf_ptr<V>(b.v[BUF_OUT]);
+            if (NT * IPT <= 4096 && n <= T2::TILE) T2::sort(sm.half, ks, vs, ko, vo, cur.begin, n, km);
+            else T::sort(sm.full, ks, vs, ko, vo, cur.begin, n, km);
+        }
+        return;
+    }
     __shared__ typename T::Smem sm;
     const int count = tile_count(sc, cls);
     int i = blockIdx.x;
     if (i >= count) return;
-    if constexpr (IPT > 4) {
-        // no register room for a second set of keys: one range at a time
-        // (measured: pipelining the 16-item tiles spills and is slower)
-        for (; i < count; i += persistent ? (int)gridDim.x : count) {
-            const TileItem cur = sc.tq[cls][i];
-            const int src = ti_src(cur);
-            T::sort(sm, (const U*)b.k[src], (const V*)b.v[src], buf_ptr<U>(b.k[BUF_OUT]), buf_ptr<V>(b.v[BUF_OUT]),
-                    cur.begin, ti_n(cur), km);
-        }
-        return;
-    }
     // software pipeline: the next range's keys are loaded while this one is
     // written out
     TileItem it = sc.tq[cls][i];

@@ -218,6 +218,29 @@ def test_top16_finish_paths(pkg, torch_cuda, orc):
     check(pkg, torch_cuda, orc, keys, None, off, algos=("batched",))  # keys only
 
 
+def test_large_input_properties(pkg, torch_cuda):
+    """2^26 u32 pairs (one 2^25-item segment, then 2^15 ragged segments) checked through
+    size-independent properties instead of the oracle: every segment is sorted, its values
+    (the input positions) are a permutation of the segment's positions that carry their
+    own keys, and equal keys keep their input order."""
+    n = 1 << 26
+    lengths = np.concatenate([[1 << 25], pkg.gen.ragged_lognormal_lengths(61, s=1 << 15, total=1 << 25)])
+    off = pkg.gen.offsets_from_lengths(lengths)
+    keys = pkg.gen.keys_u32(61, 1, n, 30)  # 30-bit keys: ties inside the long segment
+    vals = np.arange(n, dtype=np.uint32)
+    seg = np.repeat(np.arange(lengths.size), lengths)
+    for algo in ("batched", "plain"):
+        print(f"[case] n={n} algo={algo} large-input properties", flush=True)
+        ko, vo = gpu_sort(pkg, torch_cuda, keys, vals, off, algo=algo)
+        same_seg = seg[1:] == seg[:-1]
+        assert np.all(ko[1:][same_seg] >= ko[:-1][same_seg]), algo
+        assert np.array_equal(np.bincount(vo, minlength=n), np.ones(n, dtype=np.int64)), algo
+        assert np.array_equal(seg[vo], seg), algo
+        assert np.array_equal(keys[vo], ko), algo
+        ties = same_seg & (ko[1:] == ko[:-1])
+        assert np.all(vo[1:][ties] > vo[:-1][ties]), algo
+
+
 def test_gaps_unordered_and_empty(pkg, torch_cuda, orc):
     """Non-contiguous, unordered segments with gaps and end <= begin: positions outside
     the segments are never written."""

@@ -372,6 +372,21 @@ def main():
                    "gb_s": (v["bytes"] / v["launches"]) / (v["ms"] / v["launches"] * 1e-3) / 1e9 if v["ms"] else 0}
                for k, v in rep.items()}
 
+    # measured device-copy ceiling beside the nominal peak: 1 GiB device to device,
+    # read + write bytes over the copy time
+    a = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+    c = torch.empty_like(a)
+    c.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        c.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_gb_s = 2 * a.numel() * 4 / (e0.elapsed_time(e1) / 10 * 1e-3) / 1e9
+    del a, c
+
     ms = res["batched"]["ms"]
     out = {
         "metric": METRIC,
@@ -391,7 +406,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(args.workload, dname),
                      "kernel": dname, "avg_launch_us": round(avg_ms * 1e3, 2),
-                     "algorithmic_bytes_per_launch": int(bytes_per_launch)},
+                     "algorithmic_bytes_per_launch": int(bytes_per_launch),
+                     "measured_copy_gb_s": round(copy_gb_s, 1)},
         "end_to_end": {"algorithmic_bytes": algo_bytes,
                        "achieved_gb_s": round(algo_bytes / (ms * 1e-3) / 1e9, 1),
                        "frac": round(res["batched"]["hbm_frac"], 4)},

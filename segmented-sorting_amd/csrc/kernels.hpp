@@ -891,7 +891,10 @@ __global__ void k_fix_composite(const U* __restrict__ keys, const int* __restric
 }
 
 // Upsweep of one fix-pass digit pass: per-chunk digit counts, digit-major.
-template <int NT>
+// The passes over segment-id bits see long runs of one digit (a chunk covers
+// few segments), so a wave whose valid lanes share one digit adds its count
+// with a single LDS atomic instead of 64 to one address.
+template <int NT, int IPT>
 __global__ __launch_bounds__(NT) void k_fix_upsweep(const unsigned long long* __restrict__ comp, const int* d_begin,
                                                     const int* d_end, int S, const unsigned long long* dmax,
                                                     int segbits, int pass, int chunk, int nchunks_max,
@@ -902,51 +905,79 @@ __global__ __launch_bounds__(NT) void k_fix_upsweep(const unsigned long long* __
     const int shift = pass * RADIX_BITS;
     if (shift >= total_bits) return;
     const int nch = (m + chunk - 1) / chunk;
+    const int t = threadIdx.x;
     for (int g = blockIdx.x; g < nch; g += gridDim.x) {
-        for (int i = threadIdx.x; i < RADIX; i += NT) h[i] = 0;
+        for (int i = t; i < RADIX; i += NT) h[i] = 0;
         __syncthreads();
         const int start = g * chunk;
         const int n = (m - start) < chunk ? (m - start) : chunk;
-        for (int i = threadIdx.x; i < n; i += NT) atomicAdd(&h[(uint32_t)(comp[start + i] >> shift) & 0xFFu], 1u);
+        for (int i0 = 0; i0 < n; i0 += NT * IPT) {
+            uint32_t dg[IPT];
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int i = i0 + j * NT + t;
+                dg[j] = (uint32_t)(comp[start + (i < n ? i : n - 1)] >> shift) & 0xFFu;
+            }
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const bool v = i0 + j * NT + t < n;
+                const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)dg[j]);
+                const uint64_t act = __ballot(v);
+                if (__ballot(v && dg[j] != d0) == 0) {
+                    if (lane_id() == 0 && act) atomicAdd(&h[d0], (uint32_t)__popcll(act));
+                } else if (v) {
+                    atomicAdd(&h[dg[j]], 1u);
+                }
+            }
+        }
         __syncthreads();
-        for (int i = threadIdx.x; i < RADIX; i += NT) hist[(size_t)i * nchunks_max + g] = h[i];
+        for (int i = t; i < RADIX; i += NT) hist[(size_t)i * nchunks_max + g] = h[i];
         __syncthreads();
     }
 }
 
-// Spine of one fix-pass digit pass: exclusive scan over hist[digit][chunk] in
-// digit-major order (one workgroup).
-template <int NT>
+// Spine of one fix-pass digit pass, one workgroup per digit: the exclusive
+// scan of the digit's per-chunk counts over the chunks (in place), and the
+// digit's total, from which each downsweep workgroup forms the digit bases.
+template <int NT, int IPT>
 __global__ __launch_bounds__(NT) void k_fix_spine(const int* d_begin, const int* d_end, int S,
                                                   const unsigned long long* dmax, int segbits, int pass, int chunk,
-                                                  int nchunks_max, uint32_t* hist) {
+                                                  int nchunks_max, uint32_t* hist, uint32_t* totals) {
     __shared__ uint32_t wsum[NT / WAVE];
-    __shared__ uint32_t carry;
     const int m = d_end[S - 1] - d_begin[0];
     const int total_bits = bit_length<unsigned long long>(*dmax) + segbits;
     if (pass * RADIX_BITS >= total_bits) return;
     const int nch = (m + chunk - 1) / chunk;
-    const int total = RADIX * nch;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (int t0 = 0; t0 < total; t0 += NT) {
-        const int i = t0 + (int)threadIdx.x;
-        uint32_t x = 0;
-        size_t addr = 0;
-        if (i < total) {
-            addr = (size_t)(i / nch) * nchunks_max + (i % nch);
-            x = hist[addr];
+    uint32_t* h = hist + (size_t)blockIdx.x * nchunks_max;
+    const int t = threadIdx.x;
+    uint32_t carry = 0;
+    for (int c0 = 0; c0 < nch; c0 += NT * IPT) {
+        const int first = c0 + t * IPT;
+        uint32_t x[IPT];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            x[j] = first + j < nch ? h[first + j] : 0u;
+            sum += x[j];
         }
-        const uint32_t incl = wave_incl_scan(x);
-        if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = incl;
+        const uint32_t incl = wave_incl_scan(sum);
+        if ((t & 63) == 63) wsum[t >> 6] = incl;
         __syncthreads();
-        uint32_t base = carry + incl - x;
-        for (int ww = 0; ww < (int)(threadIdx.x >> 6); ++ww) base += wsum[ww];
-        if (i < total) hist[addr] = base;
-        __syncthreads();
-        if (threadIdx.x == NT - 1) carry = base + x;
+        uint32_t base = carry + incl - sum, blk = 0;
+        for (int w = 0; w < NT / WAVE; ++w) {
+            const uint32_t ws = wsum[w];
+            if (w < (t >> 6)) base += ws;
+            blk += ws;
+        }
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            if (first + j < nch) h[first + j] = base;
+            base += x[j];
+        }
+        carry += blk;
         __syncthreads();
     }
+    if (t == 0) totals[blockIdx.x] = carry;
 }
 
 // Downsweep of one fix-pass digit pass: stable partition of each chunk of
@@ -956,14 +987,32 @@ __global__ __launch_bounds__(NT) void k_fix_downsweep(const unsigned long long* 
                                                       unsigned long long* comp_out, uint32_t* idx_out,
                                                       const int* d_begin, const int* d_end, int S,
                                                       const unsigned long long* dmax, int segbits, int pass,
-                                                      int nchunks_max, const uint32_t* hist) {
+                                                      int nchunks_max, const uint32_t* hist,
+                                                      const uint32_t* totals) {
     using T = RadixTile<NT, IPT, unsigned long long, uint32_t, true>;
+    static_assert(NT >= RADIX, "one thread per digit for the digit bases");
     __shared__ typename T::Smem sm;
     __shared__ uint32_t goff[RADIX];
+    __shared__ uint32_t dbase[RADIX];
+    __shared__ uint32_t tws[RADIX / WAVE];
     const int m = d_end[S - 1] - d_begin[0];
     const int total_bits = bit_length<unsigned long long>(*dmax) + segbits;
     const int shift = pass * RADIX_BITS;
     if (shift >= total_bits) return;
+    // digit bases: exclusive scan of the spine's digit totals
+    const int t = threadIdx.x;
+    if (t < RADIX) {
+        const uint32_t tt = totals[t];
+        const uint32_t inc = wave_incl_scan(tt);
+        if ((t & 63) == 63) tws[t >> 6] = inc;
+        dbase[t] = inc - tt;
+    }
+    __syncthreads();
+    if (t < RADIX) {
+        uint32_t add = 0;
+        for (int w = 0; w < (t >> 6); ++w) add += tws[w];
+        dbase[t] += add;
+    }
     const int bits = (total_bits - shift) < RADIX_BITS ? (total_bits - shift) : RADIX_BITS;
     const int nch = (m + T::TILE - 1) / T::TILE;
     KeyMode km;
@@ -972,7 +1021,7 @@ __global__ __launch_bounds__(NT) void k_fix_downsweep(const unsigned long long* 
     km.begin_bit = 0;
     km.end_bit = 64;
     for (int g = blockIdx.x; g < nch; g += gridDim.x) {
-        for (int d = threadIdx.x; d < RADIX; d += NT) goff[d] = hist[(size_t)d * nchunks_max + g];
+        for (int d = t; d < RADIX; d += NT) goff[d] = hist[(size_t)d * nchunks_max + g] + dbase[d];
         __syncthreads();
         const int start = g * T::TILE;
         const int n = (m - start) < T::TILE ? (m - start) : T::TILE;

@@ -597,6 +597,7 @@ int fixpass(void* d_temp, size_t* temp_bytes, const void* ki, void* ko, const vo
     const size_t o_c0 = take(M * 8), o_c1 = take(M * 8);
     const size_t o_i0 = take(M * 4), o_i1 = take(M * 4);
     const size_t o_h = take(nch * RADIX * 4);
+    const size_t o_t = take(RADIX * 4);
     const size_t total = at + 255;
     if (!d_temp) {
         *temp_bytes = total;
@@ -610,6 +611,7 @@ int fixpass(void* d_temp, size_t* temp_bytes, const void* ki, void* ko, const vo
     unsigned long long* comp[2] = {(unsigned long long*)(base + o_c0), (unsigned long long*)(base + o_c1)};
     uint32_t* idx[2] = {(uint32_t*)(base + o_i0), (uint32_t*)(base + o_i1)};
     uint32_t* hist = (uint32_t*)(base + o_h);
+    uint32_t* totals = (uint32_t*)(base + o_t);
     int segbits = 0;
     while (segbits < 31 && (1ll << segbits) < (long long)S) ++segbits;  // bit length of S-1
     const int cus = device_cus();
@@ -625,13 +627,14 @@ int fixpass(void* d_temp, size_t* temp_bytes, const void* ki, void* ko, const vo
     for (int ps = 0; ps < max_passes; ++ps) {
         const int s = ps & 1;
         SEGSORT_LAUNCH("fix_upsweep", st, -1, 0.0, (double)N * 8,
-                       (k_fix_upsweep<256><<<cus * 4, 256, 0, st>>>(comp[s], db, de, S, dmax, segbits, ps, FIX_TILE,
-                                                                    (int)nch, hist)));
+                       (k_fix_upsweep<256, 8><<<cus * 4, 256, 0, st>>>(comp[s], db, de, S, dmax, segbits, ps,
+                                                                       FIX_TILE, (int)nch, hist)));
         SEGSORT_LAUNCH("fix_spine", st, -1, 0.0, 0.0,
-                       (k_fix_spine<1024><<<1, 1024, 0, st>>>(db, de, S, dmax, segbits, ps, FIX_TILE, (int)nch, hist)));
+                       (k_fix_spine<256, 8><<<RADIX, 256, 0, st>>>(db, de, S, dmax, segbits, ps, FIX_TILE, (int)nch,
+                                                                   hist, totals)));
         SEGSORT_LAUNCH("fix_downsweep", st, -1, 0.0, (double)N * 24,
                        (kd<<<gd, FIX_NT, 0, st>>>(comp[s], idx[s], comp[s ^ 1], idx[s ^ 1], db, de, S, dmax, segbits,
-                                                 ps, (int)nch, hist)));
+                                                 ps, (int)nch, hist, totals)));
     }
     Bufs b;
     memset(&b, 0, sizeof(b));

@@ -847,46 +847,147 @@ __global__ __launch_bounds__(NT) void k_seg_lsd(Bufs b, Sched sc, KeyMode km) {
 // ---- fix-pass: one global LSD over (segment id, key) composites ---------------
 
 // Max of the order keys over [begin[0], end[S-1]) (fixcub.cu:141-143 takes the
-// max to size the segment-id shift).
-template <typename U>
-__global__ void k_fix_max(const U* __restrict__ keys, const int* d_begin, const int* d_end, int S, KeyMode km,
-                          unsigned long long* dmax) {
+// max to size the segment-id shift). IPT loads per thread are in flight at
+// once; the clamped tail loads repeat the last key, which leaves the max as is.
+template <typename U, int NT, int IPT>
+__global__ __launch_bounds__(NT) void k_fix_max(const U* __restrict__ keys, const int* d_begin, const int* d_end,
+                                                int S, KeyMode km, unsigned long long* dmax) {
     const int lo = d_begin[0];
     const int hi = d_end[S - 1];
     const U rm = range_mask<U>(km.begin_bit, km.end_bit);
     unsigned long long m = 0;
-    for (int i = lo + blockIdx.x * blockDim.x + threadIdx.x; i < hi; i += gridDim.x * blockDim.x) {
-        const unsigned long long okey = (unsigned long long)((twiddle_in<U>(keys[i], km.kind, km.desc) & rm) >> km.begin_bit);
-        m = okey > m ? okey : m;
+    for (int64_t b0 = lo + (int64_t)blockIdx.x * NT * IPT; b0 < hi; b0 += (int64_t)gridDim.x * NT * IPT) {
+        U x[IPT];
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) {
+            const int64_t i = b0 + k * NT + threadIdx.x;
+            x[k] = keys[i < hi ? i : hi - 1];
+        }
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) {
+            const unsigned long long okey = (unsigned long long)((twiddle_in<U>(x[k], km.kind, km.desc) & rm) >> km.begin_bit);
+            m = okey > m ? okey : m;
+        }
     }
     for (int o = 1; o < WAVE; o <<= 1) {
         const unsigned long long y = __shfl_xor(m, o, WAVE);
         m = y > m ? y : m;
     }
-    if (lane_id() == 0) atomicMax(dmax, m);
+    // one atomic per workgroup: thousands of same-address atomics serialise
+    __shared__ unsigned long long wm[NT / WAVE];
+    if (lane_id() == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < NT / WAVE; ++w) m = wm[w] > m ? wm[w] : m;
+        if (m) atomicMax(dmax, m);
+    }
 }
 
-// composite[i - lo] = (segment id << kbits) | order key, idx[i - lo] = i - lo.
+// composite[i - lo] = (segment id << kbits) | order key, and a payload per element.
 // Segments must tile [begin[0], end[S-1]) in order (CSR offsets), as the
-// reference's fix-pass assumes (fixcub.cu:103-108).
-template <typename U>
-__global__ void k_fix_composite(const U* __restrict__ keys, const int* __restrict__ d_begin, const int* d_end, int S,
-                                KeyMode km, const unsigned long long* dmax, unsigned long long* comp, uint32_t* idx) {
+// reference's fix-pass assumes (fixcub.cu:103-108); an element belongs to the
+// last segment whose begin is at or before it. Each workgroup takes NT * IPT
+// consecutive elements: two binary searches give the segments that start in
+// it, each marks its first element in LDS (the largest id wins, so empty
+// segments sharing a begin give way to the one that holds the elements), and
+// a running max over the marks gives every element its segment.
+//
+// The payload is the input index (FIX_PAY_INDEX: the last step gathers keys and
+// values by it), the input value itself (FIX_PAY_VALUE), or nothing
+// (FIX_PAY_NONE, keys only). With a value or no payload the downsweeps write
+// the output themselves; when no pass runs at all (one segment, every order
+// key 0) the input is already in order and is copied to the output here.
+constexpr int FIX_PAY_INDEX = 0, FIX_PAY_VALUE = 1, FIX_PAY_NONE = 2;
+
+template <typename U, int NT, int IPT>
+__global__ __launch_bounds__(NT) void k_fix_composite(const U* __restrict__ keys, const int* __restrict__ d_begin,
+                                                      const int* d_end, int S, KeyMode km,
+                                                      const unsigned long long* dmax, int segbits, int pay_mode,
+                                                      const uint32_t* __restrict__ vals, unsigned long long* comp,
+                                                      uint32_t* pay, U* keys_out, uint32_t* vals_out) {
+    constexpr int CH = NT * IPT;
+    constexpr int PAD = CH + CH / 16;  // one spare word per 16 keeps the per-thread runs off one bank
+    __shared__ uint32_t head[PAD];
+    __shared__ uint32_t wmax[NT / WAVE];
+    __shared__ int srange[2];
+    auto at = [](int p) { return p + (p >> 4); };
     const int lo = d_begin[0];
     const int hi = d_end[S - 1];
+    const int m = hi - lo;
     const U rm = range_mask<U>(km.begin_bit, km.end_bit);
     const int kbits = bit_length<unsigned long long>(*dmax);
-    for (int i = lo + blockIdx.x * blockDim.x + threadIdx.x; i < hi; i += gridDim.x * blockDim.x) {
-        // last segment whose begin <= i
-        int a = 0, z = S - 1;
-        while (a < z) {
-            const int mid = (a + z + 1) >> 1;
-            if (d_begin[mid] <= i) a = mid;
-            else z = mid - 1;
+    const int t = threadIdx.x;
+    if (pay_mode != FIX_PAY_INDEX && kbits + segbits == 0) {
+        for (int64_t i = lo + (int64_t)blockIdx.x * NT + t; i < hi; i += (int64_t)gridDim.x * NT) {
+            keys_out[i] = keys[i];
+            if (pay_mode == FIX_PAY_VALUE) vals_out[i] = vals[i];
         }
-        const unsigned long long okey = (unsigned long long)((twiddle_in<U>(keys[i], km.kind, km.desc) & rm) >> km.begin_bit);
-        comp[i - lo] = ((unsigned long long)a << kbits) | okey;
-        idx[i - lo] = (uint32_t)(i - lo);
+        return;
+    }
+    for (int64_t c0 = (int64_t)blockIdx.x * CH; c0 < m; c0 += (int64_t)gridDim.x * CH) {
+        const int base = lo + (int)c0;
+        const int n = (m - (int)c0) < CH ? (m - (int)c0) : CH;
+        // keys (and values) first, unconditionally at clamped positions, so that
+        // the loads are in flight during the segment search below
+        U kx[IPT];
+        uint32_t vx[IPT];
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) {
+            const int p = k * NT + t < n ? k * NT + t : n - 1;
+            kx[k] = keys[base + p];
+            vx[k] = pay_mode == FIX_PAY_VALUE ? vals[base + p] : 0u;
+        }
+        for (int p = t; p < CH; p += NT) head[at(p)] = 0;
+        if (t < 2) {  // t 0: last segment with begin <= base; t 1: last with begin < base + n
+            const int key = t == 0 ? base : base + n - 1;
+            int a = 0, z = S - 1;
+            while (a < z) {
+                const int mid = (a + z + 1) >> 1;
+                if (d_begin[mid] <= key) a = mid;
+                else z = mid - 1;
+            }
+            srange[t] = a;
+        }
+        __syncthreads();
+        const int s0 = srange[0], s1 = srange[1];
+        for (int sg = s0 + 1 + t; sg <= s1; sg += NT) {
+            const int p = d_begin[sg] - base;  // in [1, n) for non-decreasing begins; others are ignored
+            if (p >= 0 && p < n) atomicMax(&head[at(p)], (uint32_t)sg + 1u);
+        }
+        __syncthreads();
+        // running max over this thread's IPT consecutive positions, then across threads
+        uint32_t run = 0;
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            const uint32_t h = head[at(t * IPT + j)];
+            run = h > run ? h : run;
+        }
+        const uint32_t incl = wave_incl_scan_op(run, 0u, [](uint32_t x, uint32_t y) { return x > y ? x : y; });
+        if ((t & 63) == 63) wmax[t >> 6] = incl;
+        uint32_t prev = (uint32_t)__shfl_up((int)incl, 1, WAVE);
+        if (lane_id() == 0) prev = 0;
+        __syncthreads();
+        for (int w = 0; w < (t >> 6); ++w) prev = wmax[w] > prev ? wmax[w] : prev;
+        uint32_t cur = prev > (uint32_t)s0 + 1u ? prev : (uint32_t)s0 + 1u;
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            const uint32_t h = head[at(t * IPT + j)];
+            cur = h > cur ? h : cur;
+            head[at(t * IPT + j)] = cur - 1u;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) {
+            const int p = k * NT + t;
+            if (p < n) {
+                const unsigned long long okey =
+                    (unsigned long long)((twiddle_in<U>(kx[k], km.kind, km.desc) & rm) >> km.begin_bit);
+                comp[c0 + p] = ((unsigned long long)head[at(p)] << kbits) | okey;
+                if (pay_mode == FIX_PAY_INDEX) pay[c0 + p] = (uint32_t)(c0 + p);
+                else if (pay_mode == FIX_PAY_VALUE) pay[c0 + p] = vx[k];
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -981,21 +1082,27 @@ __global__ __launch_bounds__(NT) void k_fix_spine(const int* d_begin, const int*
 }
 
 // Downsweep of one fix-pass digit pass: stable partition of each chunk of
-// (composite, index) pairs by the pass digit into the other ping-pong buffer.
-template <int NT, int IPT>
-__global__ __launch_bounds__(NT) void k_fix_downsweep(const unsigned long long* comp_in, const uint32_t* idx_in,
-                                                      unsigned long long* comp_out, uint32_t* idx_out,
+// composites (with their payload: the input index, or the value itself) by the
+// pass digit into the other ping-pong buffer. With keys_final set (the order
+// key is the whole 32-bit key, so a composite's low kbits bits, kbits the bit
+// length of the key max, are the twiddled key) the last pass that runs writes
+// the sorted keys and values to the output
+// instead; which pass is last depends on the key max, so each pass checks.
+template <int NT, int IPT, bool HAS_P>
+__global__ __launch_bounds__(NT) void k_fix_downsweep(const unsigned long long* comp_in, const uint32_t* pay_in,
+                                                      unsigned long long* comp_out, uint32_t* pay_out,
                                                       const int* d_begin, const int* d_end, int S,
                                                       const unsigned long long* dmax, int segbits, int pass,
                                                       int nchunks_max, const uint32_t* hist,
-                                                      const uint32_t* totals) {
-    using T = RadixTile<NT, IPT, unsigned long long, uint32_t, true>;
+                                                      const uint32_t* totals, uint32_t* keys_final,
+                                                      uint32_t* vals_final, KeyMode kmu) {
+    using T = RadixTile<NT, IPT, unsigned long long, uint32_t, HAS_P>;
     static_assert(NT >= RADIX, "one thread per digit for the digit bases");
     __shared__ typename T::Smem sm;
-    __shared__ uint32_t goff[RADIX];
     __shared__ uint32_t dbase[RADIX];
     __shared__ uint32_t tws[RADIX / WAVE];
-    const int m = d_end[S - 1] - d_begin[0];
+    const int lo = d_begin[0];
+    const int m = d_end[S - 1] - lo;
     const int total_bits = bit_length<unsigned long long>(*dmax) + segbits;
     const int shift = pass * RADIX_BITS;
     if (shift >= total_bits) return;
@@ -1014,6 +1121,8 @@ __global__ __launch_bounds__(NT) void k_fix_downsweep(const unsigned long long* 
         dbase[t] += add;
     }
     const int bits = (total_bits - shift) < RADIX_BITS ? (total_bits - shift) : RADIX_BITS;
+    const bool last = keys_final != nullptr && shift + RADIX_BITS >= total_bits;
+    const unsigned long long kmask = (1ull << (total_bits - segbits)) - 1;  // order-key bits of a composite
     const int nch = (m + T::TILE - 1) / T::TILE;
     KeyMode km;
     km.kind = KIND_UNSIGNED;
@@ -1021,11 +1130,24 @@ __global__ __launch_bounds__(NT) void k_fix_downsweep(const unsigned long long* 
     km.begin_bit = 0;
     km.end_bit = 64;
     for (int g = blockIdx.x; g < nch; g += gridDim.x) {
-        for (int d = t; d < RADIX; d += NT) goff[d] = hist[(size_t)d * nchunks_max + g] + dbase[d];
-        __syncthreads();
+        // thread d wrote dbase[d] itself; partition_with passes barriers before it reads goff
+        for (int d = t; d < RADIX; d += NT) sm.goff[d] = hist[(size_t)d * nchunks_max + g] + dbase[d];
         const int start = g * T::TILE;
         const int n = (m - start) < T::TILE ? (m - start) : T::TILE;
-        T::partition(sm, comp_in, idx_in, comp_out, idx_out, start, n, km, shift, bits, goff);
+        if (last) {
+            T::partition_with(sm, comp_in, pay_in, start, n, km, shift, bits, sm.goff,
+                              [&](uint32_t gi, unsigned long long key, uint32_t val) {
+                                  keys_final[lo + gi] = twiddle_out<uint32_t>((uint32_t)(key & kmask), kmu.kind,
+                                                                              kmu.desc);
+                                  if constexpr (HAS_P) vals_final[lo + gi] = val;
+                              });
+        } else {
+            T::partition_with(sm, comp_in, pay_in, start, n, km, shift, bits, sm.goff,
+                              [&](uint32_t gi, unsigned long long key, uint32_t val) {
+                                  comp_out[gi] = key;
+                                  if constexpr (HAS_P) pay_out[gi] = val;
+                              });
+        }
     }
 }
 

@@ -617,13 +617,24 @@ int fixpass(void* d_temp, size_t* temp_bytes, const void* ki, void* ko, const vo
     const int cus = device_cus();
     SEGSORT_LAUNCH("init", st, -1, 0.0, 0.0, (k_init<<<1, 64, 0, st>>>(ctr)));
     SEGSORT_LAUNCH("fix_max", st, -1, 0.0, (double)N * sizeof(U),
-                   (k_fix_max<U><<<cus * 4, 256, 0, st>>>((const U*)ki, db, de, S, km, dmax)));
-    SEGSORT_LAUNCH("fix_composite", st, -1, 0.0, (double)N * (sizeof(U) + 12),
-                   (k_fix_composite<U><<<cus * 8, 256, 0, st>>>((const U*)ki, db, de, S, km, dmax, comp[0], idx[0])));
+                   (k_fix_max<U, 1024, 8><<<cus * 2, 1024, 0, st>>>((const U*)ki, db, de, S, km, dmax)));
+    // With the whole 32-bit key as the order key, a composite's order-key bits
+    // are the twiddled key: the passes carry the values (or nothing, keys only) and the
+    // last one writes the output. A bit sub-range carries input indices and
+    // gathers keys and values by them at the end. 64-bit values always gather.
+    const bool direct = km.begin_bit == 0 && km.end_bit == 32 && (!HAS_V || sizeof(V) == 4);
+    const int pay_mode = !direct ? FIX_PAY_INDEX : HAS_V ? FIX_PAY_VALUE : FIX_PAY_NONE;
+    const int pay_bytes = pay_mode == FIX_PAY_NONE ? 0 : 4;
+    SEGSORT_LAUNCH("fix_composite", st, -1, 0.0, (double)N * (sizeof(U) + 8 + 2 * pay_bytes),
+                   (k_fix_composite<U, 256, 16><<<cus * 4, 256, 0, st>>>(
+                       (const U*)ki, db, de, S, km, dmax, segbits, pay_mode, (const uint32_t*)vi, comp[0], idx[0],
+                       (U*)ko, (uint32_t*)vo)));
     const int kb = km.end_bit - km.begin_bit;
     const int max_passes = (kb + segbits + RADIX_BITS - 1) / RADIX_BITS;
-    auto kd = k_fix_downsweep<FIX_NT, FIX_IPT>;
+    auto kd = pay_mode == FIX_PAY_NONE ? k_fix_downsweep<FIX_NT, FIX_IPT, false> : k_fix_downsweep<FIX_NT, FIX_IPT, true>;
     const int gd = resident_blocks(kd, FIX_NT);
+    uint32_t* kfin = direct ? (uint32_t*)ko : nullptr;
+    uint32_t* vfin = direct && HAS_V ? (uint32_t*)vo : nullptr;
     for (int ps = 0; ps < max_passes; ++ps) {
         const int s = ps & 1;
         SEGSORT_LAUNCH("fix_upsweep", st, -1, 0.0, (double)N * 8,
@@ -632,9 +643,14 @@ int fixpass(void* d_temp, size_t* temp_bytes, const void* ki, void* ko, const vo
         SEGSORT_LAUNCH("fix_spine", st, -1, 0.0, 0.0,
                        (k_fix_spine<256, 8><<<RADIX, 256, 0, st>>>(db, de, S, dmax, segbits, ps, FIX_TILE, (int)nch,
                                                                    hist, totals)));
-        SEGSORT_LAUNCH("fix_downsweep", st, -1, 0.0, (double)N * 24,
+        SEGSORT_LAUNCH("fix_downsweep", st, -1, 0.0, (double)N * (16 + 2 * pay_bytes),
                        (kd<<<gd, FIX_NT, 0, st>>>(comp[s], idx[s], comp[s ^ 1], idx[s ^ 1], db, de, S, dmax, segbits,
-                                                 ps, (int)nch, hist, totals)));
+                                                 ps, (int)nch, hist, totals, kfin, vfin, km)));
+    }
+    if (direct) {
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return fail(e, "segsort_sort(fixpass): launch");
+        return ok();
     }
     Bufs b;
     memset(&b, 0, sizeof(b));

@@ -768,6 +768,17 @@ struct RadixTile {
 
     __device__ static void partition_body(Smem& sm, const U* ks, const V* vs, U* kd, V* vd, int64_t begin, int n,
                                           const KeyMode& km, int shift, int bits, const uint32_t* goff) {
+        partition_with(sm, ks, vs, begin, n, km, shift, bits, goff, [&](uint32_t g, U key, V val) {
+            kd[g] = twiddle_out<U>(key, km.kind, km.desc);
+            if constexpr (HAS_V) vd[g] = val;
+        });
+    }
+
+    // partition_body with the stores left to `out(g, key, value)`: g is the
+    // item's global index, key the (twiddled) key as it sits in LDS.
+    template <typename Out>
+    __device__ static void partition_with(Smem& sm, const U* ks, const V* vs, int64_t begin, int n,
+                                          const KeyMode& km, int shift, int bits, const uint32_t* goff, Out out) {
         const int lb = lane_base(n);
         const bool act = wave() < active_waves(n);
         U k[IPT];
@@ -802,8 +813,8 @@ struct RadixTile {
                     const U key = sm.keys[p];
                     const uint32_t d = (uint32_t)((key >> shift) & dmask);
                     const uint32_t g = goff[d] + ((uint32_t)p - sm.dstart[d]);
-                    kd[g] = twiddle_out<U>(key, km.kind, km.desc);
-                    if constexpr (HAS_V) vd[g] = sm.vals[p];
+                    if constexpr (HAS_V) out(g, key, sm.vals[p]);
+                    else out(g, key, V(0));
                 }
             }
         }

@@ -273,6 +273,36 @@ def test_edge_sizes(pkg, torch_cuda, orc):
         check(pkg, torch_cuda, orc, keys, np.arange(n, dtype=np.uint32), off)
 
 
+def test_fixpass_pass_counts(pkg, torch_cuda, orc):
+    """The fix-pass runs as many digit passes as the key max and the segment count
+    need, and with the whole 32-bit key as the order key its last pass writes the
+    output: no pass at all (one segment, every key 0: the input is copied), a
+    single pass (small keys, one segment), only segment-id passes (all keys 0,
+    many segments), a leading gap before the first segment, keys only and pairs,
+    ascending and descending, and the index-and-gather route (bit sub-range,
+    64-bit values) on the same inputs."""
+    rng = np.random.default_rng(41)
+    cases = []
+    for n in (1, 5000, 70000):
+        cases.append((np.zeros(n, np.uint32), np.array([0, n])))
+        cases.append((rng.integers(0, 200, n, dtype=np.uint32), np.array([0, n])))
+    lengths = rng.integers(0, 300, 900)
+    off = pkg.gen.offsets_from_lengths(lengths)
+    n = int(off[-1])
+    cases.append((np.zeros(n, np.uint32), off))
+    cases.append((rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32), off + 7))  # gap of 7 in front
+    for keys, offs in cases:
+        total = int(offs[-1])
+        if keys.size < total:
+            keys = np.concatenate([rng.integers(0, 1 << 32, total - keys.size, dtype=np.uint64).astype(np.uint32),
+                                   keys])
+        for vals in (None, np.arange(total, dtype=np.uint32), np.arange(total, dtype=np.uint64)):
+            for desc in (False, True):
+                check(pkg, torch_cuda, orc, keys, vals, offs, algos=("fixpass",), descending=desc)
+        check(pkg, torch_cuda, orc, keys, np.arange(total, dtype=np.uint32), offs, algos=("fixpass",),
+              begin_bit=0, end_bit=31)
+
+
 TUNINGS = [
     {"wave_cap": 256, "tile_enable": 0x1F, "grid_mult": 1},  # groups of 4 only
     {"wave_cap": 256, "tile_enable": 0x2F, "grid_mult": 1},  # groups of 8

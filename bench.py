@@ -179,7 +179,8 @@ def profiler_name(kernel_name):
     m = re.search(r"k_group_tier<(\d+),", kernel_name)
     if m:
         return f"group_{m.group(1)}"
-    return {"k_wave_tier": "wave_tier", "k_bin": "bin", "k_init": "init", "k_large_hist": "large_hist",
+    return {"k_wave_tier": "wave_tier", "k_wave_tile_tier": "wave_tile", "k_bin": "bin", "k_init": "init",
+            "k_large_hist": "large_hist",
             "k_large_scan": "large_scan", "k_large_scatter": "large_scatter",
             "k_large_local": "large_local"}.get(base)
 

@@ -338,6 +338,37 @@ __global__ __launch_bounds__(256) void k_group_tier(Bufs b, Sched sc, int cls, K
     }
 }
 
+// The tile tier's loop for tiles of at most 4 items per thread: software
+// pipeline, the next range's keys are loaded while this one is written out.
+template <int NT, int IPT, typename U, typename V, bool HAS_V>
+__device__ __forceinline__ void tile_loop_pipelined(typename RadixTile<NT, IPT, U, V, HAS_V>::Smem& sm, const Bufs& b,
+                                                    const Sched& sc, int cls, const KeyMode& km, int persistent) {
+    using T = RadixTile<NT, IPT, U, V, HAS_V>;
+    const int count = tile_count(sc, cls);
+    int i = blockIdx.x;
+    if (i >= count) return;
+    TileItem it = sc.tq[cls][i];
+    U raw[IPT];
+    T::load_raw((const U*)b.k[ti_src(it)], it.begin, ti_n(it), raw);
+    for (;;) {
+        const TileItem cur = it;
+        U keys[IPT];
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) keys[j] = raw[j];
+        const int next = persistent ? i + (int)gridDim.x : count;
+        const int src = ti_src(cur);
+        T::sort_loaded(sm, keys, (const U*)b.k[src], (const V*)b.v[src], buf_ptr<U>(b.k[BUF_OUT]),
+                       buf_ptr<V>(b.v[BUF_OUT]), cur.begin, ti_n(cur), km, [&] {
+                           if (next < count) {
+                               it = sc.tq[cls][next];
+                               T::load_raw((const U*)b.k[ti_src(it)], it.begin, ti_n(it), raw);
+                           }
+                       });
+        if (next >= count) break;
+        i = next;
+    }
+}
+
 // LDS tier: each workgroup sorts one queued range of n <= NT*IPT items.
 // At least 4 waves per SIMD (16 per CU) so that LDS and memory latency can be
 // hidden; this caps the tile kernels at 128 VGPRs.
@@ -371,31 +402,37 @@ __global__ __launch_bounds__(NT, 4) void k_tile_tier(Bufs b, Sched sc, int cls, 
         return;
     }
     __shared__ typename T::Smem sm;
-    const int count = tile_count(sc, cls);
-    int i = blockIdx.x;
-    if (i >= count) return;
-    // software pipeline: the next range's keys are loaded while this one is
-    // written out
-    TileItem it = sc.tq[cls][i];
-    U raw[IPT];
-    T::load_raw((const U*)b.k[ti_src(it)], it.begin, ti_n(it), raw);
-    for (;;) {
-        const TileItem cur = it;
-        U keys[IPT];
-#pragma unroll
-        for (int j = 0; j < IPT; ++j) keys[j] = raw[j];
-        const int next = persistent ? i + (int)gridDim.x : count;
-        const int src = ti_src(cur);
-        T::sort_loaded(sm, keys, (const U*)b.k[src], (const V*)b.v[src], buf_ptr<U>(b.k[BUF_OUT]),
-                       buf_ptr<V>(b.v[BUF_OUT]), cur.begin, ti_n(cur), km, [&] {
-                           if (next < count) {
-                               it = sc.tq[cls][next];
-                               T::load_raw((const U*)b.k[ti_src(it)], it.begin, ti_n(it), raw);
-                           }
-                       });
-        if (next >= count) break;
-        i = next;
+    tile_loop_pipelined<NT, IPT, U, V, HAS_V>(sm, b, sc, cls, km, persistent);
+}
+
+// Wave tier and the 256 x IPT (IPT <= 4) tile tier in one persistent launch:
+// each wave first sorts ranges of the wave-tier queue qw on its own (no
+// barriers), then the workgroup sorts ranges of the tile queue qt together.
+// One launch instead of two, and a workgroup's tile ranges start while other
+// workgroups still sort their last wave-tier ranges. Both tiers run at 8 waves
+// per SIMD with about 15 KB of LDS per workgroup, so neither loses occupancy.
+template <typename U, typename V, bool HAS_V, int MAXN, int IPT>
+__global__ __launch_bounds__(256, 8) void k_wave_tile_tier(Bufs b, Sched sc, int qw, int qt, KeyMode km,
+                                                           int rank_max) {
+    using T = RadixTile<256, IPT, U, V, HAS_V>;
+    static_assert(IPT <= 4, "the pipelined tile loop");
+    __shared__ union {
+        WaveSmem<U, V, MAXN> wsm[4];
+        typename T::Smem tsm;
+    } sm;
+    {
+        const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+        const int count = tile_count(sc, qw);
+        WaveSmem<U, V, MAXN>& my = sm.wsm[threadIdx.x >> 6];
+        for (int i = wid; i < count; i += (int)gridDim.x * 4) {
+            const TileItem it = sc.tq[qw][i];
+            const int src = ti_src(it);
+            wave_sort<U, V, HAS_V, MAXN>((const U*)b.k[src], (const V*)b.v[src], buf_ptr<U>(b.k[BUF_OUT]),
+                                         buf_ptr<V>(b.v[BUF_OUT]), it.begin, ti_n(it), km, rank_max, my);
+        }
     }
+    __syncthreads();  // the wave regions become the tile's LDS
+    tile_loop_pipelined<256, IPT, U, V, HAS_V>(sm.tsm, b, sc, qt, km, 1);
 }
 
 // ---- large tier (MSD partition over many workgroups) --------------------------

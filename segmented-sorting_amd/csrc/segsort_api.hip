@@ -388,6 +388,23 @@ struct Launcher {
         }
     }
 
+    // The classes of one queue set (queue q0 + c for class c), smallest first.
+    // A wave tier of ranges up to 256 items followed by the 256 x 4 tile class
+    // runs as one launch (k_wave_tile_tier).
+    static void launch_classes(const Plan& p, Bufs b, Sched sc, int q0, KeyMode km, hipStream_t st, int grid_mult) {
+        for (int c = 0; c < p.nclass; ++c) {
+            if (p.kind[c] == KIND_WAVE && p.cc.cap[c] <= 256 && c + 1 < p.nclass && p.kind[c + 1] == KIND_TILE + 0) {
+                auto k = k_wave_tile_tier<U, V, HAS_V, 256, C::IPT0>;
+                const int g = resident_blocks(k, 256) * grid_mult;
+                SEGSORT_LAUNCH2("wave_tile", st, q0 + c, q0 + c + 1, PAIR_BYTES,
+                                (k<<<g, 256, 0, st>>>(b, sc, q0 + c, q0 + c + 1, km, WAVE_RANK_MAX)));
+                ++c;
+                continue;
+            }
+            launch_class(p.kind[c], b, sc, q0 + c, km, st, 0, true, grid_mult, p.cc.cap[c]);
+        }
+    }
+
     // In profiling mode: read the call's element counts back and attribute bytes.
     static void profile_attribute(size_t from, const Sched& sc, int nclass, hipStream_t st) {
         if (!Profiler::get().on()) return;
@@ -483,8 +500,7 @@ struct Launcher {
         if (split) {
             if ((e = hipEventRecord(ss->fork, st)) != hipSuccess) return e;
             if ((e = hipStreamWaitEvent(ss->s, ss->fork, 0)) != hipSuccess) return e;
-            for (int c = 0; c < p.nclass; ++c)
-                launch_class(p.kind[c], b, sc, c, km, ss->s, 0, true, tu.grid_mult, p.cc.cap[c]);
+            launch_classes(p, b, sc, 0, km, ss->s, tu.grid_mult);
             if ((e = hipEventRecord(ss->join, ss->s)) != hipSuccess) return e;
         }
         if (levels > 0) {
@@ -501,8 +517,7 @@ struct Launcher {
             SEGSORT_LAUNCH("large_local", st, 101, PAIR_BYTES + sizeof(U), 0.0,
                            (kl<<<gl, C::NT3, 0, st>>>(b, sc, 1, km, p.cc, p.tile_max)));
         }
-        for (int c = 0; c < p.nclass; ++c)
-            launch_class(p.kind[c], b, sc, sc.child_q + c, km, st, 0, true, tu.grid_mult, p.cc.cap[c]);
+        launch_classes(p, b, sc, sc.child_q, km, st, tu.grid_mult);
         if (split && (e = hipStreamWaitEvent(st, ss->join, 0)) != hipSuccess) return e;
         profile_attribute(mark, sc, p.nclass, st);
         return hipGetLastError();

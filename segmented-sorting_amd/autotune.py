@@ -7,7 +7,8 @@ the enabled LDS tile classes (`tile_enable` bits 0..3, capacities from
 segsort_tile_caps), and the multi-workgroup MSD tier above the largest enabled
 class; `grid_mult` sizes the persistent grids.
 `concurrent` (the binned ranges sorted on a second stream while the MSD tier
-runs; direct launches only) is tried on the finalists of the screening stage.
+runs; direct launches only) and the larger grid_mult values are tried on the
+finalists of the screening stage.
 
 `tune()` times every candidate on a resident workload and keeps the fastest one
 whose output is bit-identical to the default tuning's. Results are stored in
@@ -44,6 +45,12 @@ def _parse(sig: str):
 # LDS tile classes (bits 0..3) and wave-group classes (bits 4..7: groups of 4, 8, 16, 32 lanes)
 TILE_MASKS = (0xF, 0xB, 0x7, 0xD, 0xE, 0x3)
 GROUP_MASKS = (0x00, 0xE0, 0xA0, 0x60, 0xF0)
+
+
+# persistent grid sizes tried on the finalists (workgroups per resident slot): more than one
+# lets a workgroup that finishes early be replaced by a fresh one, which balances the tiers'
+# ranges without any per-range atomics
+FINAL_GRID_MULTS = (1, 2, 3, 4, 6)
 
 
 def candidates(quick: bool = False):
@@ -168,7 +175,8 @@ def _score(runs):
 
 def main(argv=None):
     """Two stages per signature: every candidate timed briefly on every workload of
-    the signature, then the best few re-timed with six times the steps, each with
+    the signature, then the tier layouts (wave_cap, tile_enable) of the best few
+    re-timed with six times the steps at each grid_mult of FINAL_GRID_MULTS, with
     `concurrent` off and on; the stored tuning minimises the sum of times relative
     to each workload's best."""
     import argparse
@@ -201,7 +209,15 @@ def main(argv=None):
     for sig, runs in by_sig.items():
         score = _score([r[2] for r in runs])
         ranked = sorted((c for c, v in score.items() if v is not None), key=lambda c: score[c])
-        finalists = [dict(json.loads(c), concurrent=k) for c in ranked[:args.finalists] for k in (0, 1)]
+        shapes = []  # (wave_cap, tile_enable) of the best screened candidates, in rank order
+        for c in ranked:
+            t = json.loads(c)
+            if (t["wave_cap"], t["tile_enable"]) not in shapes:
+                shapes.append((t["wave_cap"], t["tile_enable"]))
+            if len(shapes) == args.finalists:
+                break
+        finalists = [{"wave_cap": wc, "tile_enable": te, "grid_mult": gm, "concurrent": k}
+                     for wc, te in shapes for gm in FINAL_GRID_MULTS for k in (0, 1)]
         refined = []
         for wl, dev, _ in runs:
             _, res = tune(pkg, torch, *dev, steps=args.steps * 6, verbose=False, cands=finalists)

@@ -477,10 +477,11 @@ struct Launcher {
         // hundred workgroups at most, latency-bound) partitions the long ones on
         // the caller's stream; the MSD tier's children are then sorted on the
         // caller's stream and the second stream is joined back. Measured on
-        // MI355X: about 5% faster for ragged lengths with long segments, but up
-        // to 20% slower when no segment is long, because the (then empty) MSD
-        // launches wait for CUs behind the persistent class grids. The host
-        // cannot tell the two apart, so the tuning table chooses.
+        // MI355X (three persistent workgroups per slot): about 10% faster for
+        // ragged lengths with long segments, but up to 20% slower when no
+        // segment is long, because the (then empty) MSD launches wait for CUs
+        // behind the persistent class grids. The host cannot tell the two
+        // apart, so the tuning table chooses per signature.
         SideStreams* ss = nullptr;
         if (levels > 0 && allow_fork && tu.concurrent) {
             hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;

@@ -162,10 +162,16 @@ hipError_t run_graph(const GraphKey& key, hipStream_t st, F&& enqueue) {
             return replay(g, st);
         }
     }
-    static thread_local hipStream_t cap = nullptr;
+    // one capture stream per thread and device (a stream belongs to the device
+    // that was current when it was created)
+    static thread_local std::map<int, hipStream_t> caps;
+    hipStream_t& cap = caps[key.dev];
     if (!cap) {
         hipError_t e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
-        if (e != hipSuccess) return e;
+        if (e != hipSuccess) {
+            cap = nullptr;
+            return e;
+        }
     }
     hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
@@ -206,16 +212,19 @@ hipError_t run_graph(const GraphKey& key, hipStream_t st, F&& enqueue) {
 // form's `concurrent` mode. The stream keeps the default priority.
 struct SideStreams {
     bool ok = false;
+    bool tried = false;
     hipStream_t s;
     hipEvent_t fork;
     hipEvent_t join;
 };
 
+// Created on first use per thread and device; if creation fails the entry
+// stays !ok and the batched form runs on the caller's stream alone.
 SideStreams& side_streams() {
-    static thread_local SideStreams per_dev[16];
-    const int dev = current_device() & 15;
-    SideStreams& ss = per_dev[dev];
-    if (!ss.ok) {
+    static thread_local std::map<int, SideStreams> per_dev;
+    SideStreams& ss = per_dev[current_device()];
+    if (!ss.ok && !ss.tried) {
+        ss.tried = true;
         ss.ok = hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming) == hipSuccess &&
                 hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking) == hipSuccess &&
                 hipEventCreateWithFlags(&ss.join, hipEventDisableTiming) == hipSuccess;
@@ -487,7 +496,7 @@ struct Launcher {
             hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
             if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) {
                 ss = &side_streams();
-                if (!ss->ok) return hipErrorOutOfMemory;
+                if (!ss->ok) ss = nullptr;  // no second stream: one stream it is
             }
         }
         const bool split = ss != nullptr;

@@ -1,5 +1,7 @@
 // C-ABI of libsegsort.so: argument checks, temp-storage layout, tuning table
 // and the launch sequences of the batched, plain and fix-pass forms.
+#include <hip/hip_ext.h>
+
 #include "kernels.hpp"
 #include "profile.hpp"
 #include "../../include/segsort.h"
@@ -504,11 +506,18 @@ struct Launcher {
         const Bufs b = make_bufs(base, L, ki, ko, vi, vo);
         const size_t mark = Profiler::get().mark();
         SEGSORT_LAUNCH("init", st, -1, 0.0, 0.0, (k_init<<<1, 64, 0, st>>>(sc.ctr)));
-        SEGSORT_LAUNCH("bin", st, -1, 0.0, 8.0 * S,
-                       (k_bin<U, V, HAS_V><<<bin_grid(S), BIN_NT, 0, st>>>(db, de, S, b, sc, p.cc, km.end_bit, 1, bin_spt(S))));
+        if (split) {
+            // the fork event completes with the binning kernel itself (no separate marker on the caller's stream)
+            SEGSORT_LAUNCH("bin", st, -1, 0.0, 8.0 * S,
+                           hipExtLaunchKernelGGL(k_bin<U, V, HAS_V>, dim3(bin_grid(S)), dim3(BIN_NT), 0, st, nullptr,
+                                                 ss->fork, 0, db, de, S, b, sc, p.cc, km.end_bit, 1, bin_spt(S)));
+        } else {
+            SEGSORT_LAUNCH("bin", st, -1, 0.0, 8.0 * S,
+                           (k_bin<U, V, HAS_V><<<bin_grid(S), BIN_NT, 0, st>>>(db, de, S, b, sc, p.cc, km.end_bit, 1, bin_spt(S))));
+        }
         hipError_t e = hipSuccess;
         if (split) {
-            if ((e = hipEventRecord(ss->fork, st)) != hipSuccess) return e;
+            if ((e = hipGetLastError()) != hipSuccess) return e;
             if ((e = hipStreamWaitEvent(ss->s, ss->fork, 0)) != hipSuccess) return e;
             launch_classes(p, b, sc, 0, km, ss->s, tu.grid_mult);
             if ((e = hipEventRecord(ss->join, ss->s)) != hipSuccess) return e;

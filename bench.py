@@ -307,8 +307,8 @@ def main():
         print(json.dumps(t, indent=1, sort_keys=True))
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(segments shard with no exchange, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported (segments are independent, so "
+              "each GPU would sort its own contiguous share of them with no exchange)", file=sys.stderr)
         sys.exit(2)
 
     import torch

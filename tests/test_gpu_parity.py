@@ -120,6 +120,26 @@ def test_ragged_lognormal_2e24(pkg, torch_cuda, orc, dup):
     check(pkg, torch_cuda, orc, keys, vals, off)
 
 
+@pytest.mark.parametrize("which", ["ragged", "ragged_dup", "equal_256"])
+def test_table_tuning_2e24(pkg, torch_cuda, orc, which):
+    """The 2^24-pair workloads of the stored tuning table's headline signature, sorted with
+    the tuning the table picks for them (as the benchmark runs them: several persistent
+    workgroups per slot, the binned ranges on a second stream while the MSD tier runs),
+    twice in a row, bit-exact against the oracle."""
+    if which == "equal_256":
+        keys, vals, off = pkg.gen.equal_segments(1 << 24, 1 << 16, seed=0x5E650A7 + 1)
+    else:
+        keys, vals, off = pkg.gen.ragged_lognormal(0x5E650A7 + 2, dup_heavy=which == "ragged_dup")
+    before = pkg.get_tuning(4, 4)
+    try:
+        t = pkg.autotune.apply(pkg, keys.size, off.size - 1, 4, 4)
+        print(f"[case] table tuning {t}", flush=True)
+        for _ in range(2):
+            check(pkg, torch_cuda, orc, keys, vals, off, algos=("batched",))
+    finally:
+        pkg.set_tuning(4, 4, **before)
+
+
 @pytest.mark.parametrize("key", ["f32", "u64"])
 def test_tiny_segments_reduced(pkg, torch_cuda, orc, key):
     """2^16 segments of length U[1,32]: f32 N(0,1) with +-0.0 and denormals, or u64."""

@@ -65,6 +65,10 @@ def make_workload(pkg, name):
         return g.tiny_segments(0x5E650A7 + 3, key="f32")
     if name == "tiny_u64":
         return g.tiny_segments(0x5E650A7 + 3, key="u64")
+    if name == "diff_rows":  # the differentiable sort's forward pass: f32 rows, row-local indices as values
+        x, off = g.diff_rows(0x5E650A7 + 4)
+        local = np.concatenate([np.arange(b - a, dtype=np.uint32) for a, b in zip(off[:-1], off[1:])])
+        return x, local, off
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -230,6 +234,7 @@ def read_traffic(workload, kernel):
 def bench_diff(pkg, torch, steps, warmup):
     """Forward + backward of the differentiable pair sort per step (4096 rows)."""
     x, off = pkg.gen.diff_rows(0x5E650A7 + 4)
+    pkg.autotune.apply(pkg, x.size, off.size - 1, 4, 4)  # the forward's pair sort: f32 keys, int32 indices
     xd = torch.from_numpy(x).cuda().requires_grad_(True)
     od = torch.from_numpy(off).cuda()
     g = torch.from_numpy(pkg.gen.normal(98, 1, x.size).astype(np.float32)).cuda()

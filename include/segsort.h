@@ -110,7 +110,9 @@ int segsort_row_iota(const int* d_begin_offsets, const int* d_end_offsets, int r
 
 /* Differentiable pair sort, backward: d_grad_in[d_begin[r] + d_perm[i]] =
  * d_grad_out[i] for every i of row r; positions outside every row are left
- * untouched. d_perm holds the row-local stable permutation of the forward. */
+ * untouched. d_perm holds the row-local stable permutation of the forward
+ * (each row's entries a permutation of 0..n-1; rows of at most 8192 items are
+ * permuted in LDS and written out in order). */
 int segsort_perm_scatter_f32(const float* d_grad_out, const int* d_perm,
                              const int* d_begin_offsets, const int* d_end_offsets, int rows,
                              float* d_grad_in, void* stream);

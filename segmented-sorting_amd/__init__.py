@@ -309,10 +309,13 @@ def _autograd():
             y, perm = segmented_sort(x, begin=begin, end=end, values=iota, algo=algo, full=True)
             ctx.save_for_backward(perm, offsets)
             ctx.mark_non_differentiable(perm)
+            ctx.set_materialize_grads(False)  # no zero-filled gradient for perm (or an unused y)
             return y, perm
 
         @staticmethod
         def backward(ctx, grad_y, grad_perm):
+            if grad_y is None:
+                return None, None, None
             perm, offsets = ctx.saved_tensors
             grad_y = grad_y.contiguous().float()
             gin = torch.empty_like(grad_y)  # rows cover every position

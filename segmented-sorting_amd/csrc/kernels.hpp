@@ -1220,12 +1220,47 @@ __global__ void k_row_iota(const int* d_begin, const int* d_end, int rows, int* 
 }
 
 // Backward of the pair sort: gin[begin[r] + perm[i]] = gout[i] for every i of
-// row r, perm holding row-local source indices (one workgroup per row).
-__global__ void k_perm_scatter_f32(const float* __restrict__ gout, const int* __restrict__ perm, const int* d_begin,
-                                   const int* d_end, int rows, float* __restrict__ gin) {
+// row r, perm holding row-local source indices (one workgroup per row). A row
+// of at most PERM_LDS items is permuted inside LDS: gout and perm are read in
+// order, each value is stored at its row position in LDS, and the row is
+// written out in order (a direct scatter writes 4 bytes at a time to scattered
+// addresses and runs at a fraction of the bandwidth). Longer rows scatter
+// straight to global memory. Positions a row's perm does not name are left
+// unwritten in the direct scatter; perm must be a permutation of 0..n-1 (as
+// the forward sort makes it) for the two paths to agree.
+constexpr int PERM_LDS = 8192;
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_perm_scatter_f32(const float* __restrict__ gout, const int* __restrict__ perm,
+                                                         const int* d_begin, const int* d_end, int rows,
+                                                         float* __restrict__ gin) {
+    constexpr int U = 4;  // loads in flight per thread
+    __shared__ float buf[PERM_LDS];
+    const int t = threadIdx.x;
     for (int r = blockIdx.x; r < rows; r += gridDim.x) {
         const int b0 = d_begin[r], e0 = d_end[r];
-        for (int i = b0 + threadIdx.x; i < e0; i += blockDim.x) gin[b0 + perm[i]] = gout[i];
+        const int n = e0 - b0;
+        if (n > PERM_LDS) {
+            for (int i = b0 + t; i < e0; i += NT) gin[b0 + perm[i]] = gout[i];
+            continue;
+        }
+        for (int i0 = 0; i0 < n; i0 += NT * U) {
+            float g[U];
+            int p[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = i0 + u * NT + t;
+                const int ic = i < n ? i : 0;
+                g[u] = gout[b0 + ic];
+                p[u] = i < n ? perm[b0 + ic] : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if ((unsigned)p[u] < (unsigned)n) buf[p[u]] = g[u];
+        }
+        __syncthreads();
+        for (int j = t; j < n; j += NT) gin[b0 + j] = buf[j];
+        __syncthreads();
     }
 }
 

@@ -807,8 +807,8 @@ int segsort_perm_scatter_f32(const float* d_grad_out, const int* d_perm, const i
                              const int* d_end_offsets, int rows, float* d_grad_in, void* stream) {
     if (rows < 0) return fail(hipErrorInvalidValue, "segsort_perm_scatter_f32: negative rows");
     if (rows == 0) return ok();
-    k_perm_scatter_f32<<<std::min(rows, 65535), 256, 0, (hipStream_t)stream>>>(d_grad_out, d_perm, d_begin_offsets,
-                                                                               d_end_offsets, rows, d_grad_in);
+    k_perm_scatter_f32<256><<<std::min(rows, 65535), 256, 0, (hipStream_t)stream>>>(
+        d_grad_out, d_perm, d_begin_offsets, d_end_offsets, rows, d_grad_in);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? ok() : fail(e, "segsort_perm_scatter_f32");
 }

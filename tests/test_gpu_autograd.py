@@ -20,10 +20,13 @@ def _inputs(pkg, torch, rows, lo, hi, seed=0x5E650A7 + 4):
     return x, off, xd, od
 
 
+@pytest.mark.parametrize("rows,lo,hi", [(64, 16, 3000), (12, 6000, 30000)], ids=["short-rows", "long-rows"])
 @pytest.mark.parametrize("algo", ["batched", "plain"])
-def test_forward_backward_exact_vs_torch_sort(pkg, torch_cuda, algo):
+def test_forward_backward_exact_vs_torch_sort(pkg, torch_cuda, algo, rows, lo, hi):
+    """Rows up to 3000 items, and rows of 6000..30000 items on both sides of the
+    gradient kernel's 8192-item LDS path."""
     torch = torch_cuda
-    x, off, xd, od = _inputs(pkg, torch, rows=64, lo=16, hi=3000)
+    x, off, xd, od = _inputs(pkg, torch, rows=rows, lo=lo, hi=hi)
     y, perm = pkg.segmented_pair_sort(xd, od, algo=algo)
     g = torch.from_numpy(pkg.gen.normal(99, 1, x.size).astype(np.float32)).cuda()
     (gin,) = torch.autograd.grad(y, xd, g)

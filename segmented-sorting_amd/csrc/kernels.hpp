@@ -305,8 +305,9 @@ __global__ __launch_bounds__(256, MAXN <= 256 ? 8 : 6) void k_wave_tier(Bufs b, 
 
 // Wave-group tier: each wave sorts 64/G queued ranges of n <= G items at once,
 // one range per group of G lanes (a bitonic network confined to the group).
+// The loop of one wave over its share of queue cls; no barriers.
 template <int G, typename U, typename V, bool HAS_V>
-__global__ __launch_bounds__(256) void k_group_tier(Bufs b, Sched sc, int cls, KeyMode km) {
+__device__ __forceinline__ void group_loop(const Bufs& b, const Sched& sc, int cls, const KeyMode& km) {
     constexpr int PER_WAVE = WAVE / G;
     const int waves_per_block = blockDim.x / WAVE;
     const int wid = blockIdx.x * waves_per_block + (threadIdx.x >> 6);
@@ -315,7 +316,7 @@ __global__ __launch_bounds__(256) void k_group_tier(Bufs b, Sched sc, int cls, K
     const int g = (int)(lane_id() / G);
     const int step = nwaves * PER_WAVE;
     int base = wid * PER_WAVE;
-    if (base >= count) return;  // whole wave; the kernel has no barriers
+    if (base >= count) return;  // whole wave
     // pipeline: queue entries two batches ahead, keys and values one batch ahead
     auto entry = [&](int at) {
         TileItem it = make_ti(0, 0, BUF_IN);
@@ -336,6 +337,11 @@ __global__ __launch_bounds__(256) void k_group_tier(Bufs b, Sched sc, int cls, K
         cur = nxt;
         base = next;
     }
+}
+
+template <int G, typename U, typename V, bool HAS_V>
+__global__ __launch_bounds__(256) void k_group_tier(Bufs b, Sched sc, int cls, KeyMode km) {
+    group_loop<G, U, V, HAS_V>(b, sc, cls, km);
 }
 
 // The tile tier's loop for tiles of at most 4 items per thread: software
@@ -405,21 +411,38 @@ __global__ __launch_bounds__(NT, 4) void k_tile_tier(Bufs b, Sched sc, int cls, 
     tile_loop_pipelined<NT, IPT, U, V, HAS_V>(sm, b, sc, cls, km, persistent);
 }
 
-// Wave tier and the 256 x IPT (IPT <= 4) tile tier in one persistent launch:
-// each wave first sorts ranges of the wave-tier queue qw on its own (no
-// barriers), then the workgroup sorts ranges of the tile queue qt together.
+// Wave tier and the 256 x IPT (IPT <= 4) tile tier in one persistent launch
+// (and, when enabled, the wave-group classes before them): each wave first
+// sorts its share of the group queues and of the wave-tier queue qw on its own
+// (no barriers), then the workgroup sorts ranges of the tile queue qt together.
 // One launch instead of two, and a workgroup's tile ranges start while other
 // workgroups still sort their last wave-tier ranges. Both tiers run at 8 waves
 // per SIMD with about 15 KB of LDS per workgroup, so neither loses occupancy.
+// (For 64-bit keys the 256 x 4 tile then spills a few dozen bytes per lane;
+// measured, that beats 5 or 6 waves per SIMD without spills.)
+// Queues of the wave-group classes (G = 4, 8, 16, 32) sorted first by the same
+// launch, or -1.
+struct GroupQueues {
+    int q[4];
+};
+
 template <typename U, typename V, bool HAS_V, int MAXN, int IPT>
 __global__ __launch_bounds__(256, 8) void k_wave_tile_tier(Bufs b, Sched sc, int qw, int qt, KeyMode km,
-                                                           int rank_max) {
+                                                           int rank_max, GroupQueues gq) {
     using T = RadixTile<256, IPT, U, V, HAS_V>;
     static_assert(IPT <= 4, "the pipelined tile loop");
     __shared__ union {
         WaveSmem<U, V, MAXN> wsm[4];
         typename T::Smem tsm;
     } sm;
+    // wave-group classes first (per wave, no barriers, no LDS); 32-bit keys only
+    // (the host keeps their own launches for 64-bit keys)
+    if constexpr (sizeof(U) == 4) {
+        if (gq.q[0] >= 0) group_loop<4, U, V, HAS_V>(b, sc, gq.q[0], km);
+        if (gq.q[1] >= 0) group_loop<8, U, V, HAS_V>(b, sc, gq.q[1], km);
+        if (gq.q[2] >= 0) group_loop<16, U, V, HAS_V>(b, sc, gq.q[2], km);
+        if (gq.q[3] >= 0) group_loop<32, U, V, HAS_V>(b, sc, gq.q[3], km);
+    }
     {
         const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
         const int count = tile_count(sc, qw);

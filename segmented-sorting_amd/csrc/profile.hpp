@@ -19,7 +19,7 @@ struct ProfRecord {
     hipEvent_t a, b;
     double bytes;  // algorithmic bytes of this launch (filled after the call)
     int tag;       // tile class c, 100 + MSD level, or -1
-    int tag2;      // a second tile class sorted by the same launch, or -1
+    uint32_t more;  // bit q: tile queue q is also sorted by this launch
     double per_item;  // algorithmic bytes per element of the tagged queue
 };
 
@@ -36,13 +36,13 @@ public:
         if (e) clear_locked();
     }
     // Returns the record index (or -1 when off) after recording the start event.
-    int start(const char* name, hipStream_t st, int tag, double per_item, double bytes, int tag2 = -1) {
+    int start(const char* name, hipStream_t st, int tag, double per_item, double bytes, uint32_t more = 0) {
         if (!on_) return -1;
         std::lock_guard<std::mutex> lk(mu_);
         ProfRecord r;
         r.name = name;
         r.tag = tag;
-        r.tag2 = tag2;
+        r.more = more;
         r.per_item = per_item;
         r.bytes = bytes;
         (void)hipEventCreate(&r.a);
@@ -68,7 +68,8 @@ public:
             ProfRecord& r = recs_[i];
             if (r.tag >= 0 && r.tag < nclass) {
                 r.bytes = r.per_item * (double)tile_items[r.tag];
-                if (r.tag2 >= 0 && r.tag2 < nclass) r.bytes += r.per_item * (double)tile_items[r.tag2];
+                for (int q = 0; q < nclass && q < 32; ++q)
+                    if ((r.more >> q) & 1u) r.bytes += r.per_item * (double)tile_items[q];
             }
             else if (r.tag >= 100 && r.tag - 100 < nlevels) r.bytes = r.per_item * (double)level_items[r.tag - 100];
         }
@@ -113,10 +114,10 @@ private:
     std::vector<ProfRecord> recs_;
 };
 
-// The same for a launch that sorts the ranges of two tile queues.
-#define SEGSORT_LAUNCH2(name, st, tag, tag2, per_item, stmt)                                      \
+// The same for a launch that also sorts the tile queues in the bit mask `more`.
+#define SEGSORT_LAUNCHM(name, st, tag, more, per_item, stmt)                                      \
     do {                                                                                          \
-        const int _pi = ::ssort::Profiler::get().start(name, st, tag, per_item, 0.0, tag2);      \
+        const int _pi = ::ssort::Profiler::get().start(name, st, tag, per_item, 0.0, more);      \
         stmt;                                                                                     \
         ::ssort::Profiler::get().stop(_pi, st);                                                   \
     } while (0)

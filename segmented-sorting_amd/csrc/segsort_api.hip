@@ -401,14 +401,32 @@ struct Launcher {
 
     // The classes of one queue set (queue q0 + c for class c), smallest first.
     // A wave tier of ranges up to 256 items followed by the 256 x 4 tile class
-    // runs as one launch (k_wave_tile_tier).
+    // runs as one launch (k_wave_tile_tier), for 32-bit keys together with the
+    // wave-group classes (all enabled ones come first in the plan). Measured:
+    // 2^20 segments of U[1,32] f32 keys 6.8% faster with the groups inside;
+    // with 64-bit keys 9.6% slower, so those keep the groups' own launches.
     static void launch_classes(const Plan& p, Bufs b, Sched sc, int q0, KeyMode km, hipStream_t st, int grid_mult) {
+        int w = 0;  // the wave class, after the group classes it takes in
+        if (sizeof(U) == 4)
+            while (w < p.nclass && p.kind[w] >= KIND_GROUP) ++w;
+        else
+            while (w < p.nclass && p.kind[w] != KIND_WAVE) ++w;
+        const int g0 = sizeof(U) == 4 ? 0 : w;  // first class of the shared launch
+        const bool merged = w + 1 < p.nclass && p.kind[w] == KIND_WAVE && p.cc.cap[w] <= 256 &&
+                            p.kind[w + 1] == KIND_TILE + 0;
         for (int c = 0; c < p.nclass; ++c) {
-            if (p.kind[c] == KIND_WAVE && p.cc.cap[c] <= 256 && c + 1 < p.nclass && p.kind[c + 1] == KIND_TILE + 0) {
+            if (merged && c >= g0 && c < w) continue;  // sorted by the shared launch below
+            if (merged && c == w) {
+                GroupQueues gq = {{-1, -1, -1, -1}};
+                uint32_t more = 1u << (q0 + c + 1);
+                for (int cg = g0; cg < w; ++cg) {
+                    gq.q[p.kind[cg] - KIND_GROUP] = q0 + cg;
+                    more |= 1u << (q0 + cg);
+                }
                 auto k = k_wave_tile_tier<U, V, HAS_V, 256, C::IPT0>;
                 const int g = resident_blocks(k, 256) * grid_mult;
-                SEGSORT_LAUNCH2("wave_tile", st, q0 + c, q0 + c + 1, PAIR_BYTES,
-                                (k<<<g, 256, 0, st>>>(b, sc, q0 + c, q0 + c + 1, km, WAVE_RANK_MAX)));
+                SEGSORT_LAUNCHM("wave_tile", st, q0 + c, more, PAIR_BYTES,
+                                (k<<<g, 256, 0, st>>>(b, sc, q0 + c, q0 + c + 1, km, WAVE_RANK_MAX, gq)));
                 ++c;
                 continue;
             }

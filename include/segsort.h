@@ -130,8 +130,9 @@ int segsort_perm_scatter_f32(const float* d_grad_out, const int* d_perm,
  * concurrent: 1 sorts the ranges that fit a class on a second stream while
  * the multi-workgroup tier partitions the longer segments on the caller's
  * stream (joined back before the call's work ends); 0 (default) keeps every
- * launch on the caller's stream; honoured for direct launches (no graph
- * replay, SEGSORT_GRAPH=0) and ignored while a graph is captured.
+ * launch on the caller's stream. Under stream capture (the caller's graph or
+ * the library's own replay) the fork and join are recorded too, as two
+ * parallel branches of the graph.
  * key_width: 4 or 8; value_bytes: 0, 4 or 8. Returns a hipError_t value. */
 int segsort_set_tuning(int key_width, int value_bytes, int wave_cap, int tile_enable, int grid_mult,
                        int concurrent);

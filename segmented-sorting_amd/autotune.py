@@ -7,8 +7,8 @@ the enabled LDS tile classes (`tile_enable` bits 0..3, capacities from
 segsort_tile_caps), and the multi-workgroup MSD tier above the largest enabled
 class; `grid_mult` sizes the persistent grids.
 `concurrent` (the binned ranges sorted on a second stream while the MSD tier
-runs; direct launches only) and the larger grid_mult values are tried on the
-finalists of the screening stage.
+runs) and the larger grid_mult values are tried on the finalists of the
+screening stage.
 
 `tune()` times every candidate on a resident workload and keeps the fastest one
 whose output is bit-identical to the default tuning's. Results are stored in

@@ -500,8 +500,8 @@ struct Launcher {
     // Launch the batched sequence (bin, MSD levels, tile tiers) on st.
     static hipError_t enqueue_batched(char* base, const Layout& L, const Plan& p, const Tuning& tu, int levels,
                                       const void* ki, void* ko, const void* vi, void* vo, int S, const int* db,
-                                      const int* de, KeyMode km, hipStream_t st, bool allow_fork) {
-        // With `concurrent`, long segments possible and a forkable stream, the
+                                      const int* de, KeyMode km, hipStream_t st) {
+        // With `concurrent` and long segments possible, the
         // binned ranges are sorted on a second stream while the MSD tier (a few
         // hundred workgroups at most, latency-bound) partitions the long ones on
         // the caller's stream; the MSD tier's children are then sorted on the
@@ -511,10 +511,14 @@ struct Launcher {
         // segment is long, because the (then empty) MSD launches wait for CUs
         // behind the persistent class grids. The host cannot tell the two
         // apart, so the tuning table chooses per signature.
+        // Under stream capture the fork and join are recorded as well: the second
+        // stream joins the capture and the graph gets parallel branches.
         SideStreams* ss = nullptr;
-        if (levels > 0 && allow_fork && tu.concurrent) {
+        bool capturing = false;
+        if (levels > 0 && tu.concurrent) {
             hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-            if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) {
+            if (hipStreamIsCapturing(st, &cs) == hipSuccess) {
+                capturing = cs != hipStreamCaptureStatusNone;
                 ss = &side_streams();
                 if (!ss->ok) ss = nullptr;  // no second stream: one stream it is
             }
@@ -524,7 +528,7 @@ struct Launcher {
         const Bufs b = make_bufs(base, L, ki, ko, vi, vo);
         const size_t mark = Profiler::get().mark();
         SEGSORT_LAUNCH("init", st, -1, 0.0, 0.0, (k_init<<<1, 64, 0, st>>>(sc.ctr)));
-        if (split) {
+        if (split && !capturing) {
             // the fork event completes with the binning kernel itself (no separate marker on the caller's stream)
             SEGSORT_LAUNCH("bin", st, -1, 0.0, 8.0 * S,
                            hipExtLaunchKernelGGL(k_bin<U, V, HAS_V>, dim3(bin_grid(S)), dim3(BIN_NT), 0, st, nullptr,
@@ -536,6 +540,7 @@ struct Launcher {
         hipError_t e = hipSuccess;
         if (split) {
             if ((e = hipGetLastError()) != hipSuccess) return e;
+            if (capturing && (e = hipEventRecord(ss->fork, st)) != hipSuccess) return e;
             if ((e = hipStreamWaitEvent(ss->s, ss->fork, 0)) != hipSuccess) return e;
             launch_classes(p, b, sc, 0, km, ss->s, tu.grid_mult);
             if ((e = hipEventRecord(ss->join, ss->s)) != hipSuccess) return e;
@@ -585,7 +590,8 @@ struct Launcher {
         key.tu = tu;
         key.dev = current_device();
         hipError_t e = run_graph(key, st, [&](hipStream_t cs, bool captured) {
-            return enqueue_batched(base, L, p, tu, levels, ki, ko, vi, vo, S, db, de, km, cs, !captured);
+            (void)captured;
+            return enqueue_batched(base, L, p, tu, levels, ki, ko, vi, vo, S, db, de, km, cs);
         });
         if (e != hipSuccess) return fail(e, "segsort_sort(batched): launch");
         return ok();

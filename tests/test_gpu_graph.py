@@ -30,7 +30,8 @@ def _setup(pkg, torch):
 @pytest.mark.parametrize("concurrent", [0, 1], ids=["one-stream", "overlap"])
 def test_caller_captures_and_replays(pkg, torch_cuda, orc, concurrent):
     """With `concurrent` on, the direct call forks a second stream (its result is checked
-    too) and the captured call stays on the caller's stream."""
+    too), and so does the captured call: the second stream joins the capture and the
+    graph holds both branches."""
     torch = torch_cuda
     keys, vals, off, kout, vout, call = _setup(pkg, torch)
     ek, ev = orc.seg_sort(keys, vals, offsets=off)
@@ -58,10 +59,13 @@ def test_caller_captures_and_replays(pkg, torch_cuda, orc, concurrent):
         pkg.set_tuning(4, 4, **before)
 
 
-def test_library_graph_replay_matches_direct(pkg, torch_cuda, orc):
+@pytest.mark.parametrize("concurrent", [0, 1], ids=["one-stream", "overlap"])
+def test_library_graph_replay_matches_direct(pkg, torch_cuda, orc, concurrent):
     torch = torch_cuda
     keys, vals, off, kout, vout, call = _setup(pkg, torch)
     ek, ev = orc.seg_sort(keys, vals, offsets=off)
+    tuning = pkg.get_tuning(4, 4)
+    pkg.set_tuning(4, 4, tuning["wave_cap"], tuning["tile_enable"], tuning["grid_mult"], concurrent)
     before = pkg.set_graph_replay(True)
     try:
         st = torch.cuda.current_stream().cuda_stream
@@ -74,3 +78,4 @@ def test_library_graph_replay_matches_direct(pkg, torch_cuda, orc):
             assert np.array_equal(vout.cpu().numpy().view(np.uint32), ev), rep
     finally:
         pkg.set_graph_replay(before)
+        pkg.set_tuning(4, 4, **tuning)

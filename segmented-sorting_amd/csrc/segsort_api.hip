@@ -403,8 +403,9 @@ struct Launcher {
     // A wave tier of ranges up to 256 items followed by the 256 x 4 tile class
     // runs as one launch (k_wave_tile_tier), for 32-bit keys together with the
     // wave-group classes (all enabled ones come first in the plan). Measured:
-    // 2^20 segments of U[1,32] f32 keys 6.8% faster with the groups inside;
-    // with 64-bit keys 9.6% slower, so those keep the groups' own launches.
+    // 2^20 segments of U[1,32] f32 keys about 6% faster with the groups inside;
+    // with 64-bit keys the shared kernel's register budget made the groups
+    // slower (9.6% at 5 waves per SIMD), so those keep their own launches.
     static void launch_classes(const Plan& p, Bufs b, Sched sc, int q0, KeyMode km, hipStream_t st, int grid_mult) {
         int w = 0;  // the wave class, after the group classes it takes in
         if (sizeof(U) == 4)

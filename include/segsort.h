@@ -103,6 +103,25 @@ int segsort_bb_segsort(void* d_keys, void* d_vals, int num_items, const int* d_s
 int segsort_mgpu_segmented_sort(void* d_keys, void* d_vals, int count, const int* d_segments, int num_segments,
                                 int key_type, int value_bytes, void* stream);
 
+/* Comparators of the moderngpu adapters: mgpu::less_t / mgpu::greater_t. */
+enum segsort_comp { SEGSORT_LESS = 0, SEGSORT_GREATER = 1 };
+
+/* mgpu::segmented_sort with its comparator argument (kernel_segsort.hxx:405-416):
+ * comp SEGSORT_LESS (ascending) or SEGSORT_GREATER (descending; equal keys
+ * keep their input order, as in mgpu's stable merge sort). Keys compare in
+ * the radix order (floats: -NaN < -inf < ... < -0.0 < +0.0 < ... < +inf <
+ * +NaN), a total order where a comparator on NaN is undefined. d_vals may be
+ * NULL. Allocates and frees its own temp. Returns a hipError_t value. */
+int segsort_mgpu_segmented_sort_comp(void* d_keys, void* d_vals, int count, const int* d_segments, int num_segments,
+                                     int key_type, int value_bytes, int comp, void* stream);
+
+/* mgpu::segmented_sort_indices (kernel_segsort.hxx:418-429): sorts the keys in
+ * place and writes to d_indices[i] the input position of the key now at i
+ * (positions outside every segment get their own index). Begin offsets only,
+ * the last segment ends at count. Allocates and frees its own temp. */
+int segsort_mgpu_segmented_sort_indices(void* d_keys, int* d_indices, int count, const int* d_segments,
+                                        int num_segments, int key_type, int comp, void* stream);
+
 /* Differentiable pair sort, forward helper: d_perm[i] = i - d_begin[r] for
  * every i of row r (the row-local index vector that a stable pair sort then
  * carries as values). One workgroup per row. */
@@ -117,7 +136,15 @@ int segsort_perm_scatter_f32(const float* d_grad_out, const int* d_perm,
                              const int* d_begin_offsets, const int* d_end_offsets, int rows,
                              float* d_grad_in, void* stream);
 
-/* Tuning of the batched and plain forms (written by the autotuner).
+/* Tuning of the batched and plain forms. Every call picks its tuning inside
+ * the library, as cub picks its policy inside the call
+ * (dispatch_radix_sort.cuh:441-661): from a table built in at compile time
+ * (the autotuner's measurements, one entry per signature key width, value
+ * width, round(log2 num_items), round(log2 mean segment length); the nearest
+ * entry of the same widths when the signature is not in it), host arithmetic
+ * only, no device read. segsort_set_tuning installs an override for every call
+ * with the given key and value widths (the autotuner times candidates this
+ * way); segsort_reset_tuning removes it (key_width 0: every override).
  * wave_cap: ranges of 2..wave_cap items use the wave tier (2 <= wave_cap <= 512).
  * tile_enable: bit c (c = 0..3) enables LDS tile class c (capacities given
  * by segsort_tile_caps); bit 4 + g (g = 0..3) enables the wave-group class
@@ -129,15 +156,22 @@ int segsort_perm_scatter_f32(const float* d_grad_out, const int* d_perm,
  * grid_mult: persistent workgroups per CU per resident slot (1..8).
  * concurrent: 1 sorts the ranges that fit a class on a second stream while
  * the multi-workgroup tier partitions the longer segments on the caller's
- * stream (joined back before the call's work ends); 0 (default) keeps every
+ * stream (joined back before the call's work ends); 0 keeps every
  * launch on the caller's stream. Under stream capture (the caller's graph or
  * the library's own replay) the fork and join are recorded too, as two
  * parallel branches of the graph.
- * key_width: 4 or 8; value_bytes: 0, 4 or 8. Returns a hipError_t value. */
+ * key_width: 4 or 8; value_bytes: 0, 4 or 8. Return a hipError_t value.
+ * segsort_get_tuning writes the slot's override, or the built-in default
+ * (256, 0xEF, 1, 0) when none is installed. */
 int segsort_set_tuning(int key_width, int value_bytes, int wave_cap, int tile_enable, int grid_mult,
                        int concurrent);
+int segsort_reset_tuning(int key_width, int value_bytes);
 int segsort_get_tuning(int key_width, int value_bytes, int* wave_cap, int* tile_enable, int* grid_mult,
                        int* concurrent);
+/* The tuning a call with these sizes would run with (host only); *from_table
+ * is 1 when it comes from the built-in table, 0 when from an override. */
+int segsort_lookup_tuning(int num_items, int num_segments, int key_width, int value_bytes, int* wave_cap,
+                          int* tile_enable, int* grid_mult, int* concurrent, int* from_table);
 /* Capacities (items) of the LDS tile classes for a key width / value size;
  * writes up to 4 values and returns how many. */
 int segsort_tile_caps(int key_width, int value_bytes, int* caps);

@@ -103,23 +103,17 @@ struct DeviceSegmentedRadixSort {
                                         true, stream, debug_synchronous, algo);
     }
 
-    // DoubleBuffer forms (device_segmented_radix_sort.cuh:243-257, 446-460): the
-    // sorted data ends in the alternate buffer and the selector is flipped.
+    // DoubleBuffer forms (device_segmented_radix_sort.cuh:246-260, 445-459,
+    // 630-643, 811-825): the sorted data ends in the alternate buffer and the
+    // selector is flipped.
     template <typename KeyT, typename ValueT>
     static hipError_t SortPairs(void* d_temp_storage, size_t& temp_storage_bytes, DoubleBuffer<KeyT>& d_keys,
                                 DoubleBuffer<ValueT>& d_values, int num_items, int num_segments,
                                 const int* d_begin_offsets, const int* d_end_offsets, int begin_bit = 0,
                                 int end_bit = sizeof(KeyT) * 8, hipStream_t stream = 0,
                                 bool debug_synchronous = false, int algo = SEGSORT_ALGO_BATCHED) {
-        hipError_t e = detail::sort<KeyT, ValueT>(d_temp_storage, temp_storage_bytes, d_keys.Current(),
-                                                  d_keys.Alternate(), d_values.Current(), d_values.Alternate(),
-                                                  num_items, num_segments, d_begin_offsets, d_end_offsets, begin_bit,
-                                                  end_bit, false, stream, debug_synchronous, algo);
-        if (e == hipSuccess && d_temp_storage) {
-            d_keys.selector ^= 1;
-            d_values.selector ^= 1;
-        }
-        return e;
+        return pairs_db(d_temp_storage, temp_storage_bytes, d_keys, d_values, num_items, num_segments,
+                        d_begin_offsets, d_end_offsets, begin_bit, end_bit, false, stream, debug_synchronous, algo);
     }
 
     template <typename KeyT>
@@ -127,11 +121,57 @@ struct DeviceSegmentedRadixSort {
                                int num_items, int num_segments, const int* d_begin_offsets, const int* d_end_offsets,
                                int begin_bit = 0, int end_bit = sizeof(KeyT) * 8, hipStream_t stream = 0,
                                bool debug_synchronous = false, int algo = SEGSORT_ALGO_BATCHED) {
-        hipError_t e = detail::sort<KeyT, void>(d_temp_storage, temp_storage_bytes, d_keys.Current(),
-                                                d_keys.Alternate(), nullptr, nullptr, num_items, num_segments,
-                                                d_begin_offsets, d_end_offsets, begin_bit, end_bit, false, stream,
-                                                debug_synchronous, algo);
-        if (e == hipSuccess && d_temp_storage) d_keys.selector ^= 1;
+        return keys_db(d_temp_storage, temp_storage_bytes, d_keys, num_items, num_segments, d_begin_offsets,
+                       d_end_offsets, begin_bit, end_bit, false, stream, debug_synchronous, algo);
+    }
+
+    // device_segmented_radix_sort.cuh:445-459
+    template <typename KeyT, typename ValueT>
+    static hipError_t SortPairsDescending(void* d_temp_storage, size_t& temp_storage_bytes,
+                                          DoubleBuffer<KeyT>& d_keys, DoubleBuffer<ValueT>& d_values, int num_items,
+                                          int num_segments, const int* d_begin_offsets, const int* d_end_offsets,
+                                          int begin_bit = 0, int end_bit = sizeof(KeyT) * 8, hipStream_t stream = 0,
+                                          bool debug_synchronous = false, int algo = SEGSORT_ALGO_BATCHED) {
+        return pairs_db(d_temp_storage, temp_storage_bytes, d_keys, d_values, num_items, num_segments,
+                        d_begin_offsets, d_end_offsets, begin_bit, end_bit, true, stream, debug_synchronous, algo);
+    }
+
+    // device_segmented_radix_sort.cuh:811-825
+    template <typename KeyT>
+    static hipError_t SortKeysDescending(void* d_temp_storage, size_t& temp_storage_bytes, DoubleBuffer<KeyT>& d_keys,
+                                         int num_items, int num_segments, const int* d_begin_offsets,
+                                         const int* d_end_offsets, int begin_bit = 0, int end_bit = sizeof(KeyT) * 8,
+                                         hipStream_t stream = 0, bool debug_synchronous = false,
+                                         int algo = SEGSORT_ALGO_BATCHED) {
+        return keys_db(d_temp_storage, temp_storage_bytes, d_keys, num_items, num_segments, d_begin_offsets,
+                       d_end_offsets, begin_bit, end_bit, true, stream, debug_synchronous, algo);
+    }
+
+private:
+    template <typename KeyT, typename ValueT>
+    static hipError_t pairs_db(void* d_temp, size_t& temp_bytes, DoubleBuffer<KeyT>& d_keys,
+                               DoubleBuffer<ValueT>& d_values, int num_items, int num_segments, const int* d_begin,
+                               const int* d_end, int begin_bit, int end_bit, bool descending, hipStream_t stream,
+                               bool debug_synchronous, int algo) {
+        hipError_t e = detail::sort<KeyT, ValueT>(d_temp, temp_bytes, d_keys.Current(), d_keys.Alternate(),
+                                                  d_values.Current(), d_values.Alternate(), num_items, num_segments,
+                                                  d_begin, d_end, begin_bit, end_bit, descending, stream,
+                                                  debug_synchronous, algo);
+        if (e == hipSuccess && d_temp) {
+            d_keys.selector ^= 1;
+            d_values.selector ^= 1;
+        }
+        return e;
+    }
+
+    template <typename KeyT>
+    static hipError_t keys_db(void* d_temp, size_t& temp_bytes, DoubleBuffer<KeyT>& d_keys, int num_items,
+                              int num_segments, const int* d_begin, const int* d_end, int begin_bit, int end_bit,
+                              bool descending, hipStream_t stream, bool debug_synchronous, int algo) {
+        hipError_t e = detail::sort<KeyT, void>(d_temp, temp_bytes, d_keys.Current(), d_keys.Alternate(), nullptr,
+                                                nullptr, num_items, num_segments, d_begin, d_end, begin_bit, end_bit,
+                                                descending, stream, debug_synchronous, algo);
+        if (e == hipSuccess && d_temp) d_keys.selector ^= 1;
         return e;
     }
 };

@@ -33,7 +33,8 @@ EXPORTS = (
     "segsort_sort", "segsort_sort_pairs_u32", "segsort_sort_keys_u32", "segsort_bb_segsort",
     "segsort_mgpu_segmented_sort", "segsort_row_iota", "segsort_perm_scatter_f32", "segsort_set_tuning",
     "segsort_get_tuning", "segsort_tile_caps", "segsort_last_error", "segsort_version", "segsort_profile_enable",
-    "segsort_profile_report", "segsort_set_graph_replay",
+    "segsort_profile_report", "segsort_set_graph_replay", "segsort_reset_tuning", "segsort_lookup_tuning",
+    "segsort_mgpu_segmented_sort_comp", "segsort_mgpu_segmented_sort_indices",
 )
 
 _lib = None
@@ -64,6 +65,10 @@ def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
     lib.segsort_bb_segsort.restype = i32
     lib.segsort_mgpu_segmented_sort.argtypes = [vp, vp, i32, vp, i32, i32, i32, vp]
     lib.segsort_mgpu_segmented_sort.restype = i32
+    lib.segsort_mgpu_segmented_sort_comp.argtypes = [vp, vp, i32, vp, i32, i32, i32, i32, vp]
+    lib.segsort_mgpu_segmented_sort_comp.restype = i32
+    lib.segsort_mgpu_segmented_sort_indices.argtypes = [vp, vp, i32, vp, i32, i32, i32, vp]
+    lib.segsort_mgpu_segmented_sort_indices.restype = i32
     lib.segsort_row_iota.argtypes = [vp, vp, i32, vp, vp]
     lib.segsort_row_iota.restype = i32
     lib.segsort_perm_scatter_f32.argtypes = [vp, vp, vp, vp, i32, vp, vp]
@@ -73,6 +78,11 @@ def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
     lib.segsort_get_tuning.argtypes = [i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32),
                                        ctypes.POINTER(i32)]
     lib.segsort_get_tuning.restype = i32
+    lib.segsort_reset_tuning.argtypes = [i32, i32]
+    lib.segsort_reset_tuning.restype = i32
+    ip = ctypes.POINTER(i32)
+    lib.segsort_lookup_tuning.argtypes = [i32, i32, i32, i32, ip, ip, ip, ip, ip]
+    lib.segsort_lookup_tuning.restype = i32
     lib.segsort_tile_caps.argtypes = [i32, i32, ctypes.POINTER(i32)]
     lib.segsort_tile_caps.restype = i32
     lib.segsort_last_error.argtypes = []
@@ -210,48 +220,131 @@ def temp_bytes(num_items, num_segments, key_dtype, value_bytes=0, algo="batched"
     return tb[0]
 
 
+def _check_tensor(t, what, device, dtype=None):
+    import torch
+
+    if not isinstance(t, torch.Tensor):
+        raise SegsortError(f"{what} must be a torch tensor")
+    if t.device.type != "cuda" or t.device != device:
+        raise SegsortError(f"{what} must be a CUDA tensor on {device} (got {t.device})")
+    if not t.is_contiguous():
+        raise SegsortError(f"{what} must be contiguous")
+    if dtype is not None and t.dtype != dtype:
+        raise SegsortError(f"{what} must be {dtype} (got {t.dtype}): offsets are read as int32 begin/end values")
+
+
 def segmented_sort(keys, offsets=None, values=None, *, begin=None, end=None, descending=False, begin_bit=0,
                    end_bit=None, algo="batched", keys_out=None, values_out=None, temp=None, stream=None,
                    full=False):
     """Stable per-segment sort of cuda tensors; returns (keys_out, values_out).
 
-    Segments are [offsets[i], offsets[i+1]) or [begin[i], end[i]); positions outside
-    every segment are copied from the input (the kernels never write them), unless
-    full=True, which promises that the segments cover every position and skips
-    that copy.
+    Segments are [offsets[i], offsets[i+1]) or [begin[i], end[i]) (int32, contiguous, on
+    the keys' device); positions outside every segment are copied from the input (the
+    kernels never write them), unless full=True, which promises that the segments cover
+    every position and skips that copy.
+
+    ``stream`` (a torch.cuda.Stream; default the current stream) is the stream the sort
+    runs on. The outputs and the temp storage are allocated on it and it first waits for
+    the current stream, so the inputs are ready and a later reuse of the temp storage is
+    ordered after the sort; the inputs are marked as used on it. The caller synchronises
+    with ``stream`` before reading the outputs elsewhere, as with any torch side stream.
     """
     import torch
 
     if begin is None:
+        if offsets is None:
+            raise SegsortError("segmented_sort needs offsets or begin/end")
+        _check_tensor(offsets, "offsets", keys.device, torch.int32)
         begin, end = offsets[:-1], offsets[1:]
+    _check_tensor(keys, "keys", keys.device)
+    _check_tensor(begin, "begin", keys.device, torch.int32)
+    _check_tensor(end, "end", keys.device, torch.int32)
+    if begin.numel() != end.numel():
+        raise SegsortError("begin and end must have the same length")
+    if values is not None:
+        _check_tensor(values, "values", keys.device)
+        if values.numel() != keys.numel():
+            raise SegsortError("values must have as many elements as keys")
+    for t, what in ((keys_out, "keys_out"), (values_out, "values_out")):
+        if t is not None:
+            _check_tensor(t, what, keys.device)
     n = keys.numel()
     s = begin.numel()
-    if keys_out is None:
-        keys_out = torch.empty_like(keys) if full else keys.clone()
-    if values is not None and values_out is None:
-        values_out = torch.empty_like(values) if full else values.clone()
     kt = key_type_of(keys.dtype)
     vb = 0 if values is None else values.element_size()
     need = temp_bytes(n, s, kt, vb, algo, begin_bit, end_bit)
-    if temp is None or temp.numel() < need:
-        temp = torch.empty(need, dtype=torch.uint8, device=keys.device)
-    tb = [temp.numel()]
-    rc = DeviceSegmentedRadixSort._call(temp, tb, keys, keys_out, values, values_out, n, s, begin, end, begin_bit,
-                                        end_bit, stream, 1 if descending else 0, kt, vb, algo)
+    caller = torch.cuda.current_stream(keys.device)
+    if stream is None:
+        stream = caller
+    if isinstance(stream, int):
+        raise SegsortError("segmented_sort takes a torch.cuda.Stream (raw handles: DeviceSegmentedRadixSort)")
+    side = stream != caller
+    if side:
+        stream.wait_stream(caller)  # the inputs (and caller-given outputs) are ready on the sort's stream
+    with torch.cuda.stream(stream):
+        if keys_out is None:
+            keys_out = torch.empty_like(keys) if full else keys.clone()
+        if values is not None and values_out is None:
+            values_out = torch.empty_like(values) if full else values.clone()
+        if temp is None or temp.numel() < need:
+            temp = torch.empty(need, dtype=torch.uint8, device=keys.device)
+        tb = [temp.numel()]
+        rc = DeviceSegmentedRadixSort._call(temp, tb, keys, keys_out, values, values_out, n, s, begin, end,
+                                            begin_bit, end_bit, stream, 1 if descending else 0, kt, vb, algo)
     _check(rc, "segsort_sort")
+    if side:
+        # tensors made on another stream and used by this sort: their memory is not
+        # reused before the sort's work on `stream` is done
+        for t in (keys, values, begin, end, keys_out, values_out, temp):
+            if t is not None:
+                t.record_stream(stream)
     return keys_out, values_out
 
 
 def set_tuning(key_width, value_bytes, wave_cap, tile_enable, grid_mult, concurrent=0):
+    """Install an override used by every call with these key/value widths (instead of
+    the library's built-in table) until reset_tuning."""
     _check(load_library().segsort_set_tuning(key_width, value_bytes, wave_cap, tile_enable, grid_mult, concurrent),
            "set_tuning")
 
 
+def reset_tuning(key_width=0, value_bytes=0):
+    """Drop the override of one slot (key_width 0: every slot): calls use the built-in table."""
+    _check(load_library().segsort_reset_tuning(key_width, value_bytes), "reset_tuning")
+
+
 def get_tuning(key_width, value_bytes):
+    """The slot's override, or the built-in default when none is installed."""
     a, b, c, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     _check(load_library().segsort_get_tuning(key_width, value_bytes, ctypes.byref(a), ctypes.byref(b),
                                              ctypes.byref(c), ctypes.byref(d)), "get_tuning")
     return {"wave_cap": a.value, "tile_enable": b.value, "grid_mult": c.value, "concurrent": d.value}
+
+
+def lookup_tuning(num_items, num_segments, key_width, value_bytes):
+    """The tuning a call of this shape runs with (the override, else the built-in table's
+    entry), and whether it comes from the table."""
+    v = [ctypes.c_int() for _ in range(5)]
+    _check(load_library().segsort_lookup_tuning(int(num_items), int(num_segments), key_width, value_bytes,
+                                                *[ctypes.byref(x) for x in v]), "lookup_tuning")
+    return {"wave_cap": v[0].value, "tile_enable": v[1].value, "grid_mult": v[2].value,
+            "concurrent": v[3].value, "from_table": bool(v[4].value)}
+
+
+class tuning_override:
+    """Context manager: calls with these widths use the given tuning inside the block;
+    on exit the slot goes back to the built-in table."""
+
+    def __init__(self, key_width, value_bytes, **tuning):
+        self.kw, self.vb, self.t = key_width, value_bytes, tuning
+
+    def __enter__(self):
+        set_tuning(self.kw, self.vb, **self.t)
+        return self
+
+    def __exit__(self, *exc):
+        reset_tuning(self.kw, self.vb)
+        return False
 
 
 def set_graph_replay(on: bool) -> bool:
@@ -296,10 +389,19 @@ def _autograd():
         """
 
         @staticmethod
-        def forward(ctx, x, offsets, algo="batched"):
+        def forward(ctx, x, offsets, algo="batched", assume_cover=False):
             if x.dtype != torch.float32 or offsets.dtype != torch.int32:
                 raise SegsortError("SegmentedPairSort expects float32 values and int32 offsets")
             x = x.contiguous()
+            _check_tensor(offsets, "offsets", x.device, torch.int32)
+            if offsets.numel() < 1:
+                raise SegsortError("offsets needs at least one entry")
+            if not assume_cover:
+                # the rows must cover x exactly: outputs and gradients are written only
+                # inside rows (one device-to-host read of the two end offsets)
+                first, last = (int(v) for v in offsets[[0, -1]].tolist())
+                if first != 0 or last != x.numel():
+                    raise SegsortError(f"offsets must run from 0 to x.numel()={x.numel()} (got {first}..{last})")
             begin, end = offsets[:-1], offsets[1:]
             rows = begin.numel()
             lib = load_library()
@@ -315,7 +417,7 @@ def _autograd():
         @staticmethod
         def backward(ctx, grad_y, grad_perm):
             if grad_y is None:
-                return None, None, None
+                return None, None, None, None
             perm, offsets = ctx.saved_tensors
             grad_y = grad_y.contiguous().float()
             gin = torch.empty_like(grad_y)  # rows cover every position
@@ -323,7 +425,7 @@ def _autograd():
             _check(load_library().segsort_perm_scatter_f32(grad_y.data_ptr(), perm.data_ptr(), begin.data_ptr(),
                                                            end.data_ptr(), begin.numel(), gin.data_ptr(),
                                                            _stream(None)), "perm_scatter")
-            return gin, None, None
+            return gin, None, None, None
 
     return SegmentedPairSort
 
@@ -331,9 +433,13 @@ def _autograd():
 _SPS = None
 
 
-def segmented_pair_sort(x, offsets, algo="batched"):
-    """Differentiable stable per-row sort: returns (sorted values, row-local perm)."""
+def segmented_pair_sort(x, offsets, algo="batched", assume_cover=False):
+    """Differentiable stable per-row sort: returns (sorted values, row-local perm).
+
+    The rows [offsets[r], offsets[r+1]) must cover x exactly (offsets[0] == 0 and
+    offsets[-1] == x.numel()); this is checked with one device-to-host read unless the
+    caller promises it with assume_cover=True."""
     global _SPS
     if _SPS is None:
         _SPS = _autograd()
-    return _SPS.apply(x, offsets, algo)
+    return _SPS.apply(x, offsets, algo, assume_cover)

@@ -14,8 +14,9 @@ screening stage.
 whose output is bit-identical to the default tuning's. Results are stored in
 tuning.json keyed by a signature the host knows without reading device memory:
 key width, value width, round(log2 N) and round(log2 mean segment length).
-`apply()` looks up the nearest stored signature at plan time and installs it
-(one host-side table read, no device synchronisation).
+build.py compiles the table into the library (csrc/tuning_table.inc), which picks
+the entry of each call's signature (or the nearest one) itself, inside the call;
+`apply()` reports that choice.
 
     python autotune.py --workload ragged   # tunes on the named bench workload (GPU)
 """
@@ -85,9 +86,15 @@ def lookup(sig: str, table: dict | None = None) -> dict:
 
 
 def apply(pkg, num_items: int, num_segments: int, key_width: int, value_bytes: int, table: dict | None = None) -> dict:
-    t = lookup(signature(num_items, num_segments, key_width, value_bytes), table)
-    pkg.set_tuning(key_width, value_bytes, t["wave_cap"], t["tile_enable"], t["grid_mult"], t["concurrent"])
-    return t
+    """The tuning the library picks for a call of this shape. With `table` (a table
+    other than the built-in one), install its nearest entry as an override."""
+    if table is not None:
+        t = lookup(signature(num_items, num_segments, key_width, value_bytes), table)
+        pkg.set_tuning(key_width, value_bytes, t["wave_cap"], t["tile_enable"], t["grid_mult"], t["concurrent"])
+        return t
+    pkg.reset_tuning(key_width, value_bytes)
+    t = pkg.lookup_tuning(num_items, num_segments, key_width, value_bytes)
+    return {k: t[k] for k in DEFAULT}
 
 
 def _time(pkg, torch, args, steps, warmup):
@@ -133,7 +140,6 @@ def tune(pkg, torch, keys, vals, offsets, steps=5, warmup=2, verbose=True, quick
     kout = torch.empty_like(keys)
     vout = torch.empty_like(vals) if vals is not None else None
     args = (keys, kout, vals, vout, n, s, offsets[:-1], offsets[1:], kt, vb)
-    before = pkg.get_tuning(kw, vb)
     pkg.set_tuning(kw, vb, **DEFAULT)
     _time(pkg, torch, args, 1, 0)
     ref_k, ref_v = kout.clone(), (vout.clone() if vout is not None else None)
@@ -150,7 +156,7 @@ def tune(pkg, torch, keys, vals, offsets, steps=5, warmup=2, verbose=True, quick
             if same and (best is None or ms < best[1]):
                 best = (c, ms)
     finally:
-        pkg.set_tuning(kw, vb, **before)
+        pkg.reset_tuning(kw, vb)
     return best, results
 
 

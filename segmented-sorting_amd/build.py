@@ -1,6 +1,7 @@
 """Build libsegsort.so (gfx950) and the C++ drivers in-tree with hipcc."""
 from __future__ import annotations
 
+import json
 import os
 import subprocess
 import sys
@@ -16,6 +17,8 @@ COMMON = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-Xclang", "-target-fea
 SOURCES = [HERE / "csrc" / "segsort_api.hip"]
 LIB = HERE / "lib" / "libsegsort.so"
 DRIVERS = ["radixseg", "fixcub", "mergeseg", "bbsegsort", "facade_example"]
+GENERATORS = ["gen_equal", "gen_diff"]  # host-only input generators (g++)
+CXX = os.environ.get("CXX", "g++")
 
 
 def _run(cmd: list[str]) -> None:
@@ -25,8 +28,38 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"build step failed: {' '.join(cmd)}")
 
 
+TABLE_JSON = HERE / "tuning.json"
+TABLE_INC = HERE / "csrc" / "tuning_table.inc"
+
+
+def table_source(table: dict) -> str:
+    """C++ text of the library's built-in tuning table (segsort_api.hip includes it)
+    from the autotuner's tuning.json: one entry per signature k<K>v<V>-n<N>-m<M>."""
+    rows = []
+    for sig in sorted(table):
+        head, n, m = sig.split("-")
+        kw, vb = head[1:].split("v")
+        t = table[sig]
+        rows.append(f"    {{{int(kw)}, {int(vb)}, {int(n[1:])}, {int(m[1:])}, "
+                    f"{{{t['wave_cap']}, 0x{t['tile_enable']:02X}, {t['grid_mult']}, {t['concurrent']}}}}},  // {sig}")
+    return ("// Generated by build.py from tuning.json (the autotuner's table); do not edit.\n"
+            "// {key width, value bytes, round(log2 N), round(log2 mean length),\n"
+            "//  {wave_cap, tile_enable, grid_mult, concurrent}}\n"
+            f"constexpr int TUNING_TABLE_SIZE = {len(rows)};\n"
+            "constexpr TableEntry TUNING_TABLE[TUNING_TABLE_SIZE > 0 ? TUNING_TABLE_SIZE : 1] = {\n"
+            + "\n".join(rows) + "\n};\n")
+
+
+def write_table() -> Path:
+    text = table_source(json.loads(TABLE_JSON.read_text()))
+    if not TABLE_INC.exists() or TABLE_INC.read_text() != text:
+        TABLE_INC.write_text(text)
+    return TABLE_INC
+
+
 def build_library() -> Path:
     LIB.parent.mkdir(exist_ok=True)
+    write_table()
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, *COMMON, "-fPIC", "-shared", "-o", str(tmp), *map(str, SOURCES)])
     os.replace(tmp, LIB)
@@ -44,6 +77,17 @@ def build_drivers() -> list[Path]:
         exe = bindir / name
         _run([HIPCC, "-O2", "-std=c++17", f"-I{ROOT / 'include'}", "-o", str(exe), str(src), f"-L{LIB.parent}",
               "-lsegsort", f"-Wl,-rpath,{LIB.parent}"])
+        out.append(exe)
+    return out + build_generators()
+
+
+def build_generators() -> list[Path]:
+    out = []
+    bindir = HERE / "bin"
+    bindir.mkdir(exist_ok=True)
+    for name in GENERATORS:
+        exe = bindir / name
+        _run([CXX, "-O2", "-std=c++17", "-Wall", "-o", str(exe), str(HERE / "drivers" / f"{name}.cpp")])
         out.append(exe)
     return out
 

@@ -1234,6 +1234,11 @@ __global__ void k_fix_gather(Bufs b, const uint32_t* idx0, const uint32_t* idx1,
 
 // ---- differentiable pair sort helpers ---------------------------------------
 
+// p[i] = i for i < n (the input positions an index sort carries as values).
+__global__ void k_iota(int* p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = i;
+}
+
 // vals[i] = i - begin[r] for every i of row r (one workgroup per row).
 __global__ void k_row_iota(const int* d_begin, const int* d_end, int rows, int* vals) {
     for (int r = blockIdx.x; r < rows; r += gridDim.x) {

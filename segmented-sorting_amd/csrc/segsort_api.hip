@@ -7,6 +7,7 @@
 #include "../../include/segsort.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -50,9 +51,25 @@ struct Tuning {
     int concurrent;   // binned ranges sorted on a second stream while the MSD tier runs (direct launches only)
 };
 
-// index: (key_width == 8) * 3 + value-bytes slot (0, 4, 8 -> 0, 1, 2)
-Tuning g_tuning[6] = {{256, 0xEF, 1, 0}, {256, 0xEF, 1, 0}, {256, 0xEF, 1, 0},
-                      {256, 0xEF, 1, 0}, {256, 0xEF, 1, 0}, {256, 0xEF, 1, 0}};
+// The built-in tuning table (generated from tuning.json by build.py): one entry
+// per signature (key width, value width, round(log2 N), round(log2 mean
+// segment length)), as the autotuner measured them. A call takes the entry of
+// its own signature, else the nearest one with the same key and value widths
+// (distance |d log2 N| + 2 |d log2 mean|), else DEFAULT_TUNING, the way cub
+// picks its tuning policy inside the call (dispatch_radix_sort.cuh:441-661):
+// host arithmetic only, no device read.
+struct TableEntry {
+    int key_width, value_bytes, log2n, log2m;
+    Tuning tu;
+};
+#include "tuning_table.inc"
+
+constexpr Tuning DEFAULT_TUNING = {256, 0xEF, 1, 0};
+
+// Overrides installed by segsort_set_tuning, per (key width, value width) slot:
+// index (key_width == 8) * 3 + value-bytes slot (0, 4, 8 -> 0, 1, 2).
+Tuning g_override[6];
+bool g_override_on[6] = {false, false, false, false, false, false};
 std::mutex g_tuning_mu;
 
 int tuning_slot(int key_width, int value_bytes) {
@@ -61,9 +78,38 @@ int tuning_slot(int key_width, int value_bytes) {
     return (key_width == 8 ? 3 : 0) + vs;
 }
 
-Tuning get_tuning(int key_width, int value_bytes) {
-    std::lock_guard<std::mutex> lk(g_tuning_mu);
-    return g_tuning[tuning_slot(key_width, value_bytes)];
+int rounded_log2(double x) { return (int)std::lround(std::log2(x < 1.0 ? 1.0 : x)); }
+
+const TableEntry* table_lookup(int key_width, int value_bytes, long long num_items, long long num_segments) {
+    const int ln = rounded_log2((double)(num_items > 1 ? num_items : 1));
+    const double mean = (double)(num_items > 1 ? num_items : 1) / (double)(num_segments > 1 ? num_segments : 1);
+    const int lm = rounded_log2(mean);
+    const TableEntry* best = nullptr;
+    int dist = 0;
+    for (int i = 0; i < TUNING_TABLE_SIZE; ++i) {
+        const TableEntry& e = TUNING_TABLE[i];
+        if (e.key_width != key_width || e.value_bytes != value_bytes) continue;
+        const int d = std::abs(e.log2n - ln) + 2 * std::abs(e.log2m - lm);
+        if (!best || d < dist) {
+            best = &e;
+            dist = d;
+        }
+    }
+    return best;
+}
+
+// The tuning a call of this shape runs with: the override of its slot if one is
+// installed, else the table's.
+Tuning choose_tuning(int key_width, int value_bytes, long long num_items, long long num_segments,
+                     bool* from_override = nullptr) {
+    const int slot = tuning_slot(key_width, value_bytes);
+    {
+        std::lock_guard<std::mutex> lk(g_tuning_mu);
+        if (from_override) *from_override = g_override_on[slot];
+        if (g_override_on[slot]) return g_override[slot];
+    }
+    const TableEntry* e = table_lookup(key_width, value_bytes, num_items, num_segments);
+    return e ? e->tu : DEFAULT_TUNING;
 }
 
 // Per-device CU count and per-kernel residency, queried once and cached.
@@ -103,6 +149,72 @@ int resident_blocks(K kernel, int threads) {
     std::lock_guard<std::mutex> lk(g_occ_mu);
     g_occ[{dev, key}] = per_cu;
     return per_cu * cus;
+}
+
+// Second stream and fork/join events of the batched form's `concurrent` mode,
+// one set per (host thread, device, caller stream), so that independent
+// caller streams never share (and so serialise on) one second stream. The
+// streams keep the default priority. A thread's sets are destroyed when the
+// thread ends.
+struct SideStreams {
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr;
+    hipEvent_t join = nullptr;
+};
+
+struct SidePool {
+    std::map<std::pair<int, hipStream_t>, SideStreams> sets;
+    std::map<int, hipStream_t> capture;  // the graph-replay mode's private capture stream, per device
+    ~SidePool() {
+        for (auto& kv : sets) {
+            if (kv.second.s) (void)hipStreamDestroy(kv.second.s);
+            if (kv.second.fork) (void)hipEventDestroy(kv.second.fork);
+            if (kv.second.join) (void)hipEventDestroy(kv.second.join);
+        }
+        for (auto& kv : capture)
+            if (kv.second) (void)hipStreamDestroy(kv.second);
+    }
+};
+
+SidePool& side_pool() {
+    static thread_local SidePool pool;
+    return pool;
+}
+
+constexpr size_t SIDE_SETS_MAX = 32;  // per thread; beyond this a call runs on the caller's stream alone
+
+// The second stream for a call on `st`, or nullptr when the call must run on
+// st alone: creation failed, the thread already holds SIDE_SETS_MAX sets, or
+// the second stream is still part of a stream capture that is not st's own
+// (a stream that joined a capture stays in it until that capture ends).
+SideStreams* side_streams(hipStream_t st, bool capturing) {
+    SidePool& pool = side_pool();
+    const std::pair<int, hipStream_t> key{current_device(), st};
+    auto it = pool.sets.find(key);
+    if (it == pool.sets.end()) {
+        if (pool.sets.size() >= SIDE_SETS_MAX) return nullptr;
+        SideStreams ss;
+        const bool made = hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming) == hipSuccess &&
+                          hipEventCreateWithFlags(&ss.join, hipEventDisableTiming) == hipSuccess &&
+                          hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking) == hipSuccess;
+        if (!made) {
+            if (ss.s) (void)hipStreamDestroy(ss.s);
+            if (ss.fork) (void)hipEventDestroy(ss.fork);
+            if (ss.join) (void)hipEventDestroy(ss.join);
+            return nullptr;
+        }
+        it = pool.sets.emplace(key, ss).first;
+    }
+    SideStreams* ss = &it->second;
+    hipStreamCaptureStatus side_cs = hipStreamCaptureStatusNone;
+    unsigned long long side_id = 0;
+    if (hipStreamGetCaptureInfo(ss->s, &side_cs, &side_id) != hipSuccess) return nullptr;
+    if (side_cs == hipStreamCaptureStatusNone) return ss;
+    if (!capturing) return nullptr;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    unsigned long long id = 0;
+    if (hipStreamGetCaptureInfo(st, &cs, &id) != hipSuccess || cs != hipStreamCaptureStatusActive) return nullptr;
+    return id == side_id ? ss : nullptr;
 }
 
 // ---- hipGraph capture of the batched sequence ---------------------------------
@@ -166,8 +278,7 @@ hipError_t run_graph(const GraphKey& key, hipStream_t st, F&& enqueue) {
     }
     // one capture stream per thread and device (a stream belongs to the device
     // that was current when it was created)
-    static thread_local std::map<int, hipStream_t> caps;
-    hipStream_t& cap = caps[key.dev];
+    hipStream_t& cap = side_pool().capture[key.dev];
     if (!cap) {
         hipError_t e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
         if (e != hipSuccess) {
@@ -208,30 +319,6 @@ hipError_t run_graph(const GraphKey& key, hipStream_t st, F&& enqueue) {
     }
     g_graphs.push_back({key, graph, exec, done, ++g_graph_clock});
     return replay(g_graphs.back(), st);
-}
-
-// Per-thread, per-device second stream and fork/join events of the batched
-// form's `concurrent` mode. The stream keeps the default priority.
-struct SideStreams {
-    bool ok = false;
-    bool tried = false;
-    hipStream_t s;
-    hipEvent_t fork;
-    hipEvent_t join;
-};
-
-// Created on first use per thread and device; if creation fails the entry
-// stays !ok and the batched form runs on the caller's stream alone.
-SideStreams& side_streams() {
-    static thread_local std::map<int, SideStreams> per_dev;
-    SideStreams& ss = per_dev[current_device()];
-    if (!ss.ok && !ss.tried) {
-        ss.tried = true;
-        ss.ok = hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming) == hipSuccess &&
-                hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking) == hipSuccess &&
-                hipEventCreateWithFlags(&ss.join, hipEventDisableTiming) == hipSuccess;
-    }
-    return ss;
 }
 
 // ---- per-(key, value) configuration -------------------------------------------
@@ -520,8 +607,7 @@ struct Launcher {
             hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
             if (hipStreamIsCapturing(st, &cs) == hipSuccess) {
                 capturing = cs != hipStreamCaptureStatusNone;
-                ss = &side_streams();
-                if (!ss->ok) ss = nullptr;  // no second stream: one stream it is
+                ss = side_streams(st, capturing);  // nullptr: one stream it is
             }
         }
         const bool split = ss != nullptr;
@@ -568,7 +654,7 @@ struct Launcher {
 
     static int batched(void* d_temp, size_t* temp_bytes, const void* ki, void* ko, const void* vi, void* vo, int N,
                        int S, const int* db, const int* de, KeyMode km, hipStream_t st) {
-        const Tuning tu = get_tuning(sizeof(U), HAS_V ? sizeof(V) : 0);
+        const Tuning tu = choose_tuning(sizeof(U), HAS_V ? sizeof(V) : 0, N, S);
         const Plan p = make_plan<C>(tu);
         const int levels = levels_for(km, N, p);
         const Layout L = make_layout<C, U, V, HAS_V>(p, N, S, levels, false);
@@ -600,7 +686,7 @@ struct Launcher {
 
     static int plain(void* d_temp, size_t* temp_bytes, const void* ki, void* ko, const void* vi, void* vo, int N,
                      int S, const int* db, const int* de, KeyMode km, hipStream_t st) {
-        const Tuning tu = get_tuning(sizeof(U), HAS_V ? sizeof(V) : 0);
+        const Tuning tu = choose_tuning(sizeof(U), HAS_V ? sizeof(V) : 0, N, S);
         const Plan p = make_plan<C>(tu);
         const Layout L = make_layout<C, U, V, HAS_V>(p, N, S, 0, true);
         if (!d_temp) {
@@ -844,7 +930,20 @@ int segsort_set_tuning(int key_width, int value_bytes, int wave_cap, int tile_en
         grid_mult > 8 || concurrent < 0 || concurrent > 1)
         return fail(hipErrorInvalidValue, "segsort_set_tuning: out of range");
     std::lock_guard<std::mutex> lk(g_tuning_mu);
-    g_tuning[s] = {wave_cap, tile_enable, grid_mult, concurrent};
+    g_override[s] = {wave_cap, tile_enable, grid_mult, concurrent};
+    g_override_on[s] = true;
+    return ok();
+}
+
+int segsort_reset_tuning(int key_width, int value_bytes) {
+    std::lock_guard<std::mutex> lk(g_tuning_mu);
+    if (key_width == 0) {
+        for (bool& on : g_override_on) on = false;
+        return ok();
+    }
+    const int s = tuning_slot(key_width, value_bytes);
+    if (s < 0) return fail(hipErrorInvalidValue, "segsort_reset_tuning: bad slot");
+    g_override_on[s] = false;
     return ok();
 }
 
@@ -853,10 +952,25 @@ int segsort_get_tuning(int key_width, int value_bytes, int* wave_cap, int* tile_
     const int s = tuning_slot(key_width, value_bytes);
     if (s < 0) return fail(hipErrorInvalidValue, "segsort_get_tuning: bad slot");
     std::lock_guard<std::mutex> lk(g_tuning_mu);
-    if (wave_cap) *wave_cap = g_tuning[s].wave_cap;
-    if (tile_enable) *tile_enable = g_tuning[s].tile_enable;
-    if (grid_mult) *grid_mult = g_tuning[s].grid_mult;
-    if (concurrent) *concurrent = g_tuning[s].concurrent;
+    const Tuning t = g_override_on[s] ? g_override[s] : DEFAULT_TUNING;
+    if (wave_cap) *wave_cap = t.wave_cap;
+    if (tile_enable) *tile_enable = t.tile_enable;
+    if (grid_mult) *grid_mult = t.grid_mult;
+    if (concurrent) *concurrent = t.concurrent;
+    return ok();
+}
+
+int segsort_lookup_tuning(int num_items, int num_segments, int key_width, int value_bytes, int* wave_cap,
+                          int* tile_enable, int* grid_mult, int* concurrent, int* from_table) {
+    if (tuning_slot(key_width, value_bytes) < 0 || num_items < 0 || num_segments < 0)
+        return fail(hipErrorInvalidValue, "segsort_lookup_tuning: bad arguments");
+    bool ovr = false;
+    const Tuning t = choose_tuning(key_width, value_bytes, num_items, num_segments, &ovr);
+    if (wave_cap) *wave_cap = t.wave_cap;
+    if (tile_enable) *tile_enable = t.tile_enable;
+    if (grid_mult) *grid_mult = t.grid_mult;
+    if (concurrent) *concurrent = t.concurrent;
+    if (from_table) *from_table = ovr ? 0 : 1;
     return ok();
 }
 
@@ -876,15 +990,23 @@ int segsort_tile_caps(int key_width, int value_bytes, int* caps) {
     return 4;
 }
 
-// In-place adapters: sort into a scratch copy, then copy back.
+// In-place adapters: sort into a scratch copy, then copy back. With `iota`
+// the values are not read: d_vals receives each sorted key's input position
+// (mgpu's segmented_sort_indices, kernel_segsort.hxx:418-429), and positions
+// outside every segment their own index.
 static int inplace_sort(void* d_keys, void* d_vals, int n, const int* d_segs, int S, int key_type, int value_bytes,
-                        hipStream_t st) {
+                        int descending, bool iota, hipStream_t st) {
     if (n <= 0 || S <= 0) return ok();
+    if (key_type < SEGSORT_KEY_U32 || key_type > SEGSORT_KEY_F64)
+        return fail(hipErrorInvalidValue, "adapter: bad key_type");
+    if (!d_keys || !d_segs || (iota && !d_vals)) return fail(hipErrorInvalidValue, "adapter: NULL buffer");
     const int kw = key_type >= SEGSORT_KEY_U64 ? 8 : 4;
-    const int vb = d_vals ? value_bytes : 0;
+    const int vb = iota ? 4 : (d_vals ? value_bytes : 0);
+    if (vb != 0 && vb != 4 && vb != 8) return fail(hipErrorInvalidValue, "adapter: value_bytes must be 0, 4 or 8");
     // end offsets: d_segs[1..S-1] then n
     int* d_end = nullptr;
-    void *kout = nullptr, *vout = nullptr, *temp = nullptr;
+    void *kout = nullptr, *vout = nullptr, *vin = nullptr, *temp = nullptr;
+    const void* vsrc = d_vals;
     size_t tb = 0;
     int rc = 0;
     hipError_t e;
@@ -906,7 +1028,20 @@ static int inplace_sort(void* d_keys, void* d_vals, int n, const int* d_segs, in
         rc = fail(e, "adapter: alloc values");
         goto done;
     }
-    rc = segsort_sort(nullptr, &tb, d_keys, kout, d_vals, vout, n, S, d_segs, d_end, 0, kw * 8, key_type, vb, 0,
+    if (iota) {
+        // the values to carry: 0..n-1 (one row covering everything)
+        if ((e = hipMalloc(&vin, (size_t)n * 4)) != hipSuccess) {
+            rc = fail(e, "adapter: alloc indices");
+            goto done;
+        }
+        k_iota<<<std::min((n + 1023) / 1024, 4096), 256, 0, st>>>((int*)vin, n);
+        if ((e = hipGetLastError()) != hipSuccess) {
+            rc = fail(e, "adapter: iota");
+            goto done;
+        }
+        vsrc = vin;
+    }
+    rc = segsort_sort(nullptr, &tb, d_keys, kout, vsrc, vout, n, S, d_segs, d_end, 0, kw * 8, key_type, vb, descending,
                       SEGSORT_ALGO_BATCHED, st);
     if (rc) goto done;
     if ((e = hipMalloc(&temp, tb)) != hipSuccess) {
@@ -918,11 +1053,11 @@ static int inplace_sort(void* d_keys, void* d_vals, int n, const int* d_segs, in
         rc = fail(e, "adapter: prefill keys");
         goto done;
     }
-    if (vb && (e = hipMemcpyAsync(vout, d_vals, (size_t)n * vb, hipMemcpyDeviceToDevice, st)) != hipSuccess) {
+    if (vb && (e = hipMemcpyAsync(vout, vsrc, (size_t)n * vb, hipMemcpyDeviceToDevice, st)) != hipSuccess) {
         rc = fail(e, "adapter: prefill values");
         goto done;
     }
-    rc = segsort_sort(temp, &tb, d_keys, kout, d_vals, vout, n, S, d_segs, d_end, 0, kw * 8, key_type, vb, 0,
+    rc = segsort_sort(temp, &tb, d_keys, kout, vsrc, vout, n, S, d_segs, d_end, 0, kw * 8, key_type, vb, descending,
                       SEGSORT_ALGO_BATCHED, st);
     if (rc) goto done;
     if ((e = hipMemcpyAsync(d_keys, kout, (size_t)n * kw, hipMemcpyDeviceToDevice, st)) != hipSuccess) {
@@ -935,17 +1070,18 @@ static int inplace_sort(void* d_keys, void* d_vals, int n, const int* d_segs, in
     }
     if ((e = hipStreamSynchronize(st)) != hipSuccess) rc = fail(e, "adapter: sync");
 done:
-    hipStreamSynchronize(st);
-    if (temp) hipFree(temp);
-    if (vout) hipFree(vout);
-    if (kout) hipFree(kout);
-    if (d_end) hipFree(d_end);
+    (void)hipStreamSynchronize(st);
+    if (temp) (void)hipFree(temp);
+    if (vin) (void)hipFree(vin);
+    if (vout) (void)hipFree(vout);
+    if (kout) (void)hipFree(kout);
+    if (d_end) (void)hipFree(d_end);
     return rc;
 }
 
 int segsort_bb_segsort(void* d_keys, void* d_vals, int num_items, const int* d_segs, int num_segments, int key_type,
                        int value_bytes, void* stream) {
-    return inplace_sort(d_keys, d_vals, num_items, d_segs, num_segments, key_type, value_bytes,
+    return inplace_sort(d_keys, d_vals, num_items, d_segs, num_segments, key_type, value_bytes, 0, false,
                         (hipStream_t)stream) == 0
                ? 1
                : 0;
@@ -953,7 +1089,25 @@ int segsort_bb_segsort(void* d_keys, void* d_vals, int num_items, const int* d_s
 
 int segsort_mgpu_segmented_sort(void* d_keys, void* d_vals, int count, const int* d_segments, int num_segments,
                                 int key_type, int value_bytes, void* stream) {
-    return inplace_sort(d_keys, d_vals, count, d_segments, num_segments, key_type, value_bytes, (hipStream_t)stream);
+    return inplace_sort(d_keys, d_vals, count, d_segments, num_segments, key_type, value_bytes, 0, false,
+                        (hipStream_t)stream);
+}
+
+int segsort_mgpu_segmented_sort_comp(void* d_keys, void* d_vals, int count, const int* d_segments, int num_segments,
+                                     int key_type, int value_bytes, int comp, void* stream) {
+    if (comp != SEGSORT_LESS && comp != SEGSORT_GREATER)
+        return fail(hipErrorInvalidValue, "segsort_mgpu_segmented_sort_comp: comp must be SEGSORT_LESS or SEGSORT_GREATER");
+    return inplace_sort(d_keys, d_vals, count, d_segments, num_segments, key_type, value_bytes,
+                        comp == SEGSORT_GREATER ? 1 : 0, false, (hipStream_t)stream);
+}
+
+int segsort_mgpu_segmented_sort_indices(void* d_keys, int* d_indices, int count, const int* d_segments,
+                                        int num_segments, int key_type, int comp, void* stream) {
+    if (comp != SEGSORT_LESS && comp != SEGSORT_GREATER)
+        return fail(hipErrorInvalidValue,
+                    "segsort_mgpu_segmented_sort_indices: comp must be SEGSORT_LESS or SEGSORT_GREATER");
+    return inplace_sort(d_keys, d_indices, count, d_segments, num_segments, key_type, 4,
+                        comp == SEGSORT_GREATER ? 1 : 0, true, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -47,8 +47,13 @@ int main(int argc, char** argv) {
         total_ms += ms;
     }
     drv::check(hipMemcpy(out.data(), key_d, sizeof(int) * (size_t)N, hipMemcpyDeviceToHost), "D2H");
+    std::vector<double> vout((size_t)N);
+    drv::check(hipMemcpy(vout.data(), val_d, sizeof(double) * (size_t)N, hipMemcpyDeviceToHost), "D2H");
     if (o.time) printf("%f\n", total_ms / o.executions);
-    else drv::print_keys(out.data(), out.size());
+    else {
+        drv::print_keys(out.data(), out.size());
+        if (o.values) drv::print_values(vout.data(), vout.size());  // the values are the keys, as doubles
+    }
     (void)hipFree(key_d);
     (void)hipFree(val_d);
     (void)hipFree(seg_d);

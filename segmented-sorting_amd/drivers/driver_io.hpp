@@ -16,6 +16,7 @@ namespace drv {
 
 struct Options {
     bool time = false;     // print mean ms instead of the keys (-DELAPSED_TIME=1 in the reference)
+    bool values = false;   // also print the sorted values, as a second line in the keys' format
     int executions = 1;    // timed repetitions (-DEXECUTIONS)
     int algo = 0;          // segsort_algo
 };
@@ -25,6 +26,7 @@ inline Options parse(int argc, char** argv, int default_algo) {
     o.algo = default_algo;
     for (int i = 1; i < argc; ++i) {
         if (!strcmp(argv[i], "--time")) o.time = true;
+        else if (!strcmp(argv[i], "--values")) o.values = true;
         else if (!strcmp(argv[i], "--executions") && i + 1 < argc) o.executions = atoi(argv[++i]);
         else if (!strcmp(argv[i], "--algo") && i + 1 < argc) {
             const char* a = argv[++i];
@@ -57,10 +59,10 @@ inline bool read_input(std::vector<int>& offsets, std::vector<unsigned>& keys) {
 }
 
 template <typename T>
-inline void print_keys(const T* h, size_t n) {
+inline void print_line(const T* h, size_t n, bool lead) {
     std::string out;
     out.reserve(n * 11 + 2);
-    out += "\n";
+    if (lead) out += "\n";
     char buf[32];
     for (size_t i = 0; i < n; ++i) {
         int len = snprintf(buf, sizeof(buf), "%lld ", (long long)h[i]);
@@ -68,6 +70,18 @@ inline void print_keys(const T* h, size_t n) {
     }
     out += "\n";
     fwrite(out.data(), 1, out.size(), stdout);
+}
+
+// "\n" + "k " per key + "\n", the reference's output (src/cub.out).
+template <typename T>
+inline void print_keys(const T* h, size_t n) {
+    print_line(h, n, true);
+}
+
+// With --values: the sorted values follow as one more line, "v " per value + "\n".
+template <typename T>
+inline void print_values(const T* h, size_t n) {
+    print_line(h, n, false);
 }
 
 inline void check(hipError_t e, const char* what) {

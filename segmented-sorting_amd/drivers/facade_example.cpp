@@ -1,7 +1,9 @@
 // The cub-style C++ facade (include/segsort.hpp) on the example printed in cub's
 // documentation (lib/cub/device/device_segmented_radix_sort.cuh:104-129, 302-327,
-// 506-526, 685-705): every overload is called the way cub's is and its output
-// compared with the documented answer. Exit status 0 when all match.
+// 506-526, 685-705): all eight overloads (SortPairs, SortPairsDescending,
+// SortKeys, SortKeysDescending, each with separate in/out pointers and with
+// DoubleBuffers) are called the way cub's are, for every algorithm, and their
+// output compared with the documented answer. Exit status 0 when all match.
 #include <cstdio>
 #include <vector>
 
@@ -62,18 +64,52 @@ int main() {
         DeviceSegmentedRadixSort::SortKeysDescending(d_temp, temp_bytes, d_keys_in, d_keys_out, num_items,
                                                      num_segments, d_offsets, d_offsets + 1, 0, 32, 0, true, algo);
         expect("SortKeysDescending", fetch(d_keys_out, 7), {8, 7, 6, 9, 5, 3, 0});
-        // DoubleBuffer form: the result ends in the alternate buffer and the selector flips
-        (void)hipMemcpy(d_keys_out, keys.data(), 7 * sizeof(int), hipMemcpyHostToDevice);
-        (void)hipMemcpy(d_values_out, vals.data(), 7 * sizeof(int), hipMemcpyHostToDevice);
+        // DoubleBuffer forms: the result ends in the alternate buffer and the selector flips
         int *d_kalt, *d_valt;
         if (hipMalloc((void**)&d_kalt, 7 * sizeof(int)) || hipMalloc((void**)&d_valt, 7 * sizeof(int))) return 2;
-        segsort::DoubleBuffer<int> dk(d_keys_out, d_kalt);
-        segsort::DoubleBuffer<int> dv(d_values_out, d_valt);
-        DeviceSegmentedRadixSort::SortPairs(d_temp, temp_bytes, dk, dv, num_items, num_segments, d_offsets,
-                                            d_offsets + 1, 0, 32, 0, true, algo);
-        if (dk.selector != 1 || dv.selector != 1) ++failures;
-        expect("DoubleBuffer keys", fetch(dk.Current(), 7), {6, 7, 8, 0, 3, 5, 9});
-        expect("DoubleBuffer values", fetch(dv.Current(), 7), {1, 2, 0, 5, 4, 3, 6});
+        auto reset = [&] {
+            (void)hipMemcpy(d_keys_out, keys.data(), 7 * sizeof(int), hipMemcpyHostToDevice);
+            (void)hipMemcpy(d_values_out, vals.data(), 7 * sizeof(int), hipMemcpyHostToDevice);
+        };
+        auto flipped = [&](int sel) {
+            if (sel != 1) ++failures;
+        };
+        {
+            reset();
+            segsort::DoubleBuffer<int> dk(d_keys_out, d_kalt), dv(d_values_out, d_valt);
+            DeviceSegmentedRadixSort::SortPairs(d_temp, temp_bytes, dk, dv, num_items, num_segments, d_offsets,
+                                                d_offsets + 1, 0, 32, 0, true, algo);
+            flipped(dk.selector);
+            flipped(dv.selector);
+            expect("DoubleBuffer SortPairs keys", fetch(dk.Current(), 7), {6, 7, 8, 0, 3, 5, 9});
+            expect("DoubleBuffer SortPairs values", fetch(dv.Current(), 7), {1, 2, 0, 5, 4, 3, 6});
+        }
+        {
+            reset();
+            segsort::DoubleBuffer<int> dk(d_keys_out, d_kalt), dv(d_values_out, d_valt);
+            DeviceSegmentedRadixSort::SortPairsDescending(d_temp, temp_bytes, dk, dv, num_items, num_segments,
+                                                          d_offsets, d_offsets + 1, 0, 32, 0, true, algo);
+            flipped(dk.selector);
+            flipped(dv.selector);
+            expect("DoubleBuffer SortPairsDescending keys", fetch(dk.Current(), 7), {8, 7, 6, 9, 5, 3, 0});
+            expect("DoubleBuffer SortPairsDescending values", fetch(dv.Current(), 7), {0, 2, 1, 6, 3, 4, 5});
+        }
+        {
+            reset();
+            segsort::DoubleBuffer<int> dk(d_keys_out, d_kalt);
+            DeviceSegmentedRadixSort::SortKeys(d_temp, temp_bytes, dk, num_items, num_segments, d_offsets,
+                                               d_offsets + 1, 0, 32, 0, true, algo);
+            flipped(dk.selector);
+            expect("DoubleBuffer SortKeys", fetch(dk.Current(), 7), {6, 7, 8, 0, 3, 5, 9});
+        }
+        {
+            reset();
+            segsort::DoubleBuffer<int> dk(d_keys_out, d_kalt);
+            DeviceSegmentedRadixSort::SortKeysDescending(d_temp, temp_bytes, dk, num_items, num_segments, d_offsets,
+                                                         d_offsets + 1, 0, 32, 0, true, algo);
+            flipped(dk.selector);
+            expect("DoubleBuffer SortKeysDescending", fetch(dk.Current(), 7), {8, 7, 6, 9, 5, 3, 0});
+        }
         (void)hipFree(d_kalt);
         (void)hipFree(d_valt);
         (void)hipFree(d_temp);

@@ -52,8 +52,12 @@ int main(int argc, char** argv) {
     }
     std::vector<unsigned> out((size_t)N);
     drv::check(hipMemcpy(out.data(), d_vec_out, sizeof(unsigned) * (size_t)N, hipMemcpyDeviceToHost), "D2H");
+    drv::check(hipMemcpy(vals.data(), d_val_out, sizeof(unsigned) * (size_t)N, hipMemcpyDeviceToHost), "D2H");
     if (o.time) printf("%f\n", total_ms / o.executions);
-    else drv::print_keys(out.data(), out.size());
+    else {
+        drv::print_keys(out.data(), out.size());
+        if (o.values) drv::print_values(vals.data(), vals.size());
+    }
     (void)hipFree(d_temp);
     (void)hipFree(d_seg);
     (void)hipFree(d_vec);

@@ -78,7 +78,6 @@ def test_temp_too_small_is_invalid_value(pkg):
 
 
 def test_tuning_table(pkg):
-    before = pkg.get_tuning(4, 4)
     pkg.set_tuning(4, 4, 16, 0x5, 2, 0)
     assert pkg.get_tuning(4, 4) == {"wave_cap": 16, "tile_enable": 5, "grid_mult": 2, "concurrent": 0}
     with pytest.raises(pkg.SegsortError):
@@ -89,6 +88,11 @@ def test_tuning_table(pkg):
         pkg.set_tuning(4, 4, 32, 0x1FF, 1)
     pkg.set_tuning(4, 4, 64, 0xFF, 1)  # every wave-group and tile class
     assert pkg.get_tuning(4, 4)["tile_enable"] == 0xFF
-    pkg.set_tuning(4, 4, **before)
+    assert pkg.lookup_tuning(1 << 20, 1 << 10, 4, 4)["tile_enable"] == 0xFF  # the override wins
+    pkg.reset_tuning(4, 4)
+    assert pkg.get_tuning(4, 4) == {"wave_cap": 256, "tile_enable": 0xEF, "grid_mult": 1, "concurrent": 0}
+    assert pkg.lookup_tuning(1 << 20, 1 << 10, 4, 4)["from_table"]
+    with pytest.raises(pkg.SegsortError):
+        pkg.reset_tuning(5, 4)
     caps = pkg.tile_caps(4, 4)
     assert caps == sorted(caps) and caps[-1] >= 8192

@@ -22,11 +22,35 @@ def test_signature_and_lookup(pkg, tmp_path):
 
 
 def test_apply_sets_library_tuning(pkg):
-    before = pkg.get_tuning(4, 4)
     table = {"k4v4-n24-m8": {"wave_cap": 128, "tile_enable": 12, "grid_mult": 2, "concurrent": 1}}
     t = pkg.autotune.apply(pkg, 1 << 24, 1 << 16, 4, 4, table)
     assert pkg.get_tuning(4, 4) == t == {"wave_cap": 128, "tile_enable": 12, "grid_mult": 2, "concurrent": 1}
-    pkg.set_tuning(4, 4, **before)
+    assert pkg.lookup_tuning(1 << 24, 1 << 16, 4, 4)["from_table"] is False
+    pkg.reset_tuning(4, 4)
+    assert pkg.lookup_tuning(1 << 24, 1 << 16, 4, 4)["from_table"] is True
+
+
+def test_library_table_matches_tuning_json(pkg):
+    """The table compiled into the library (csrc/tuning_table.inc, generated from
+    tuning.json by build.py) gives, for every stored signature and for shapes between
+    them, the same tuning as the autotuner's own nearest-signature lookup of tuning.json."""
+    import sys
+    sys.path.insert(0, str(pkg.HERE))
+    try:
+        import build
+    finally:
+        sys.path.pop(0)
+    table = pkg.autotune.load_table()
+    assert (pkg.HERE / "csrc" / "tuning_table.inc").read_text() == build.table_source(table)
+    pkg.reset_tuning(0, 0)
+    keys = ("wave_cap", "tile_enable", "grid_mult", "concurrent")
+    shapes = [(1 << 24, 1 << k) for k in range(0, 25, 1)] + [(1 << 20, 1024), (17301504, 1 << 20), (5000, 7),
+                                                             (1 << 26, 1 << 15), (16809984, 4096)]
+    for kw, vb in ((4, 4), (4, 0), (8, 4), (4, 8), (8, 8), (8, 0)):
+        for n, s in shapes:
+            want = pkg.autotune.lookup(pkg.autotune.signature(n, s, kw, vb), table)
+            got = pkg.lookup_tuning(n, s, kw, vb)
+            assert got["from_table"] and {k: got[k] for k in keys} == want, (kw, vb, n, s)
 
 
 @pytest.mark.gpu
@@ -38,7 +62,6 @@ def test_quick_tune_is_exact(pkg, torch_cuda, orc):
     do = torch.from_numpy(off).cuda()
     best, results = pkg.autotune.tune(pkg, torch, dk, dv, do, steps=1, warmup=0, verbose=False, quick=True)
     assert best is not None and all(v is not None for v in results.values())
-    before = pkg.get_tuning(4, 4)
     pkg.set_tuning(4, 4, **best[0])
     try:
         ko, vo = pkg.segmented_sort(dk, do, dv)
@@ -46,4 +69,4 @@ def test_quick_tune_is_exact(pkg, torch_cuda, orc):
         assert np.array_equal(ko.view(torch.int32).cpu().numpy().view(np.uint32), ek)
         assert np.array_equal(vo.view(torch.int32).cpu().numpy().view(np.uint32), ev)
     finally:
-        pkg.set_tuning(4, 4, **before)
+        pkg.reset_tuning(4, 4)

@@ -106,11 +106,11 @@ def test_equal_segments_pairs_reduced(pkg, torch_cuda, orc, k):
     check(pkg, torch_cuda, orc, keys, vals, off)
 
 
-@pytest.mark.parametrize("k", [0, 2, 6, 10, 14, 18, 20])
+@pytest.mark.parametrize("k", list(range(0, 21, 2)))
 def test_equal_segments_pairs_2e24(pkg, torch_cuda, orc, k):
-    """2^24 u32 pairs (values = index) in 2^k equal segments, against the oracle."""
+    """2^24 u32 pairs (values = index) in 2^k equal segments, every form against the oracle."""
     keys, vals, off = pkg.gen.equal_segments(1 << 24, 1 << k, seed=0x5E650A7 + 1)
-    check(pkg, torch_cuda, orc, keys, vals, off, algos=("batched", "plain") if k != 10 else ALGOS)
+    check(pkg, torch_cuda, orc, keys, vals, off)
 
 
 @pytest.mark.parametrize("dup", [False, True])
@@ -130,14 +130,13 @@ def test_table_tuning_2e24(pkg, torch_cuda, orc, which):
         keys, vals, off = pkg.gen.equal_segments(1 << 24, 1 << 16, seed=0x5E650A7 + 1)
     else:
         keys, vals, off = pkg.gen.ragged_lognormal(0x5E650A7 + 2, dup_heavy=which == "ragged_dup")
-    before = pkg.get_tuning(4, 4)
     try:
-        t = pkg.autotune.apply(pkg, keys.size, off.size - 1, 4, 4)
+        t = pkg.autotune.apply(pkg, keys.size, off.size - 1, 4, 4)  # the library's own table choice
         print(f"[case] table tuning {t}", flush=True)
         for _ in range(2):
             check(pkg, torch_cuda, orc, keys, vals, off, algos=("batched",))
     finally:
-        pkg.set_tuning(4, 4, **before)
+        pkg.reset_tuning(4, 4)
 
 
 @pytest.mark.parametrize("key", ["f32", "u64"])
@@ -149,9 +148,9 @@ def test_tiny_segments_reduced(pkg, torch_cuda, orc, key):
 
 @pytest.mark.parametrize("key", ["f32", "u64"])
 def test_tiny_segments_2e20(pkg, torch_cuda, orc, key):
-    """2^20 segments of length U[1,32] (about 17.3M items)."""
+    """2^20 segments of length U[1,32] (about 17.3M items), every form (fix-pass: f32 only)."""
     keys, vals, off = pkg.gen.tiny_segments(0x5E650A7 + 3, key=key)
-    check(pkg, torch_cuda, orc, keys, vals, off, algos=("batched",))
+    check(pkg, torch_cuda, orc, keys, vals, off)
 
 
 def test_key_types_and_value_widths(pkg, torch_cuda, orc):
@@ -165,10 +164,7 @@ def test_key_types_and_value_widths(pkg, torch_cuda, orc):
             keys = (raw >> np.uint64(32)).astype(np.uint32).view(dt)
         else:
             keys = raw.view(dt)
-        if np.dtype(dt).kind == "f":
-            keys = keys.copy()
-            keys[np.isnan(keys)] = 0.5  # NaN order is covered by the twiddle closed form
-        keys = keys.copy()
+        keys = keys.copy()  # random bit patterns: float keys include NaNs of both signs
         keys[n // 2:] = keys[: n - n // 2]  # duplicates across and within segments
         for vdt in (None, np.uint32, np.uint64):
             vals = None if vdt is None else np.arange(n, dtype=vdt)
@@ -201,6 +197,137 @@ def test_bit_ranges_every_tier(pkg, torch_cuda, orc):
             for desc in (False, True):
                 check(pkg, torch_cuda, orc, keys, vals, off, begin_bit=bb, end_bit=eb, descending=desc,
                       algos=("batched", "plain"))
+
+
+def _float_specials_keys(pkg, dt, n, seed):
+    """n keys of dtype dt: N(0,1) values with a few percent of quiet/signalling NaNs of
+    both signs and several payloads, +-inf, +-0.0, denormals of both signs, and
+    repeated values (ties)."""
+    rng = np.random.default_rng(seed)
+    x = pkg.gen.normal(seed, 7, n).astype(dt)
+    ut = np.uint32 if dt == np.float32 else np.uint64
+    bits = x.view(ut).copy()
+    if dt == np.float32:
+        nan_pats = [0x7FC00000, 0x7F800001, 0x7FFFFFFF, 0x7FA00000, 0xFFC00000, 0xFF800001, 0xFFFFFFFF,
+                    0xFFA00000]
+        inf, ninf, nz, den = 0x7F800000, 0xFF800000, 0x80000000, 0x00000003
+        sign = 0x80000000
+    else:
+        nan_pats = [0x7FF8000000000000, 0x7FF0000000000001, 0x7FFFFFFFFFFFFFFF, 0xFFF8000000000000,
+                    0xFFF0000000000001, 0xFFFFFFFFFFFFFFFF]
+        inf, ninf, nz, den = 0x7FF0000000000000, 0xFFF0000000000000, 0x8000000000000000, 3
+        sign = 0x8000000000000000
+    k = max(8, n // 40)
+    idx = rng.integers(0, n, (6, k))
+    bits[idx[0]] = np.array(nan_pats, dtype=ut)[rng.integers(0, len(nan_pats), k)]
+    bits[idx[1]] = np.array([inf, ninf], dtype=ut)[rng.integers(0, 2, k)]
+    bits[idx[2]] = np.array([0, nz], dtype=ut)[rng.integers(0, 2, k)]
+    bits[idx[3]] = (rng.integers(1, 1 << 20, k).astype(ut) | np.where(rng.integers(0, 2, k) == 1, ut(sign), ut(0)))
+    bits[idx[4]] = ut(den)
+    bits[idx[5]] = bits[rng.integers(0, n, k)]  # ties
+    return bits.view(dt)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64], ids=["f32", "f64"])
+def test_float_specials_every_tier(pkg, torch_cuda, orc, dt):
+    """Float keys with NaNs (-NaN first, +NaN last, by bit pattern), +-inf, -0.0 < +0.0 and
+    denormals, ascending and descending, on lengths that reach every class (wave groups,
+    the wave tier, each LDS tile, the MSD tiers), pairs and keys only, every form
+    (fix-pass: 32-bit keys)."""
+    rng = np.random.default_rng(55)
+    lengths = np.concatenate([rng.integers(2, 40, 400), rng.integers(40, 600, 100), rng.integers(600, 5000, 20),
+                              [0, 1, 1024, 4096, 16384, 20000, 70000]])
+    rng.shuffle(lengths)
+    off = pkg.gen.offsets_from_lengths(lengths)
+    n = int(off[-1])
+    keys = _float_specials_keys(pkg, dt, n, 55)
+    assert np.isnan(keys).any() and (np.signbit(keys) & np.isnan(keys)).any()
+    vals = np.arange(n, dtype=np.uint32)
+    for desc in (False, True):
+        check(pkg, torch_cuda, orc, keys, vals, off, descending=desc)
+        check(pkg, torch_cuda, orc, keys, None, off, descending=desc)
+
+
+def test_inplace_adapters_keys_and_values(pkg, torch_cuda, orc):
+    """The in-place adapters called straight through the C-ABI: bb_segsort's
+    int bb_segsort(K*, T*, n, d_segs, S) (bb_segsort.h:36, returns 1) and
+    mgpu::segmented_sort(keys, vals, count, segments, S) (kernel_segsort.hxx:405-416):
+    begin offsets only, the last segment ends at n; sorted keys AND values equal the
+    oracle's stable result, for u32 keys with u32 values, i32 keys with f64 values (the
+    bb driver's double values), f32 keys with u64 values, and keys only."""
+    import ctypes
+    torch = torch_cuda
+    lib = pkg.load_library()
+    rng = np.random.default_rng(77)
+    lengths = np.concatenate([rng.integers(0, 300, 500), [5000, 20000, 1, 0, 70000]])
+    off = pkg.gen.offsets_from_lengths(lengths)
+    n = int(off[-1])
+    raw = pkg.gen.u64(77, 1, n)
+    cases = [((raw >> np.uint64(52)).astype(np.uint32), np.arange(n, dtype=np.uint32)),   # 12-bit keys: ties
+             ((raw >> np.uint64(44)).astype(np.int32) - (1 << 19), np.arange(n, dtype=np.float64)),
+             (_float_specials_keys(pkg, np.float32, n, 78), np.arange(n, dtype=np.uint64)),
+             ((raw >> np.uint64(40)).astype(np.uint32), None)]
+    segs = np.ascontiguousarray(off[:-1], dtype=np.int32)
+    for keys, vals in cases:
+        ek, ev = orc.seg_sort(keys, vals, offsets=off)
+        kt = KEY_TYPES[keys.dtype]
+        vb = 0 if vals is None else vals.dtype.itemsize
+        for name in ("bb", "mgpu"):
+            print(f"[case] adapter {name} keys={keys.dtype} vals={None if vals is None else vals.dtype}", flush=True)
+            dk, ds = dev(torch, keys), dev(torch, segs)
+            dv = dev(torch, vals) if vals is not None else None
+            vp = ctypes.c_void_p(dv.data_ptr()) if dv is not None else None
+            if name == "bb":
+                rc = lib.segsort_bb_segsort(dk.data_ptr(), vp, n, ds.data_ptr(), len(segs), kt, vb, None)
+                assert rc == 1, pkg.last_error()
+            else:
+                rc = lib.segsort_mgpu_segmented_sort(dk.data_ptr(), vp, n, ds.data_ptr(), len(segs), kt, vb, None)
+                assert rc == 0, pkg.last_error()
+            torch.cuda.synchronize()
+            assert np.array_equal(host(dk, keys.dtype).view(np.uint8), ek.view(np.uint8)), name
+            if vals is not None:
+                assert np.array_equal(host(dv, vals.dtype).view(np.uint8), ev.view(np.uint8)), name
+
+
+def test_mgpu_comparator_and_indices(pkg, torch_cuda, orc):
+    """mgpu::segmented_sort with less_t / greater_t (kernel_segsort.hxx:405-416) and
+    mgpu::segmented_sort_indices (:418-429) through the C-ABI: keys sorted in place in
+    the comparator's direction (equal keys in input order), values carried, and for the
+    index form each sorted key's input position; positions before the first segment
+    keep their key and get their own index."""
+    import ctypes
+    torch = torch_cuda
+    lib = pkg.load_library()
+    rng = np.random.default_rng(79)
+    lengths = np.concatenate([[0, 3], rng.integers(0, 400, 300), [9000, 30000, 1]])
+    off = pkg.gen.offsets_from_lengths(lengths) + 5  # 5 leading positions outside every segment
+    n = int(off[-1])
+    keys = (pkg.gen.u64(79, 1, n) >> np.uint64(50)).astype(np.uint32)  # 14-bit keys: ties
+    segs = np.ascontiguousarray(off[:-1], dtype=np.int32)
+    idx = np.arange(n, dtype=np.int32)
+    for comp in (0, 1):
+        ek, ev = orc.seg_sort(keys, idx, offsets=off, descending=bool(comp))
+        print(f"[case] mgpu indices comp={comp}", flush=True)
+        dk, ds = dev(torch, keys), dev(torch, segs)
+        di = torch.full((n,), -7, dtype=torch.int32, device="cuda")
+        rc = lib.segsort_mgpu_segmented_sort_indices(dk.data_ptr(), di.data_ptr(), n, ds.data_ptr(), len(segs), 0,
+                                                     comp, None)
+        assert rc == 0, pkg.last_error()
+        torch.cuda.synchronize()
+        assert np.array_equal(host(dk, np.uint32), ek), comp
+        assert np.array_equal(di.cpu().numpy(), ev), comp
+        print(f"[case] mgpu comparator comp={comp}", flush=True)
+        vals = np.arange(n, dtype=np.uint64) * 3
+        ek2, ev2 = orc.seg_sort(keys, vals, offsets=off, descending=bool(comp))
+        dk, dv = dev(torch, keys), dev(torch, vals)
+        rc = lib.segsort_mgpu_segmented_sort_comp(dk.data_ptr(), dv.data_ptr(), n, ds.data_ptr(), len(segs), 0, 8,
+                                                  comp, None)
+        assert rc == 0, pkg.last_error()
+        torch.cuda.synchronize()
+        assert np.array_equal(host(dk, np.uint32), ek2), comp
+        assert np.array_equal(host(dv, np.uint64), ev2), comp
+    assert lib.segsort_mgpu_segmented_sort_comp(dk.data_ptr(), None, n, ds.data_ptr(), len(segs), 0, 0, 2,
+                                                None) != 0
 
 
 def _runs_segment(rng, n, run_len, head_uniform):
@@ -347,7 +474,7 @@ def test_tuning_variants(pkg, torch_cuda, orc, tuning, concurrent):
     off = pkg.gen.offsets_from_lengths(lengths)
     n = int(off[-1])
     raw = pkg.gen.u64(31, 1, n)
-    befores = {(kw, vb): pkg.get_tuning(kw, vb) for kw, vb in ((4, 4), (8, 4), (4, 0))}
+    befores = ((4, 4), (8, 4), (4, 0))
     try:
         for kw, vb in befores:
             pkg.set_tuning(kw, vb, tuning["wave_cap"], tuning["tile_enable"], tuning["grid_mult"], concurrent)
@@ -356,5 +483,5 @@ def test_tuning_variants(pkg, torch_cuda, orc, tuning, concurrent):
         check(pkg, torch_cuda, orc, k32.view(np.float32), None, off, algos=("batched", "plain"))
         check(pkg, torch_cuda, orc, raw, np.arange(n, dtype=np.uint32), off, algos=("batched", "plain"))
     finally:
-        for (kw, vb), t in befores.items():
-            pkg.set_tuning(kw, vb, **t)
+        for kw, vb in befores:
+            pkg.reset_tuning(kw, vb)

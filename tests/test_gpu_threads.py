@@ -25,7 +25,6 @@ def test_concurrent_host_threads(pkg, torch_cuda, orc):
         vals = np.arange(n, dtype=np.uint32)
         ek, ev = orc.seg_sort(keys, vals, offsets=off)
         cases.append((keys, vals, off, ek, ev))
-    before = pkg.get_tuning(4, 4)
     errors = []
 
     D = pkg.DeviceSegmentedRadixSort
@@ -66,5 +65,5 @@ def test_concurrent_host_threads(pkg, torch_cuda, orc):
             th.join(timeout=120)
         assert not any(th.is_alive() for th in ths), "a worker thread did not finish"
     finally:
-        pkg.set_tuning(4, 4, **before)
+        pkg.reset_tuning(4, 4)
     assert not errors, errors

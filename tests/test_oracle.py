@@ -94,6 +94,53 @@ def test_float_order_closed_form(orc):
             assert orc.order_key(b, dt) == want
 
 
+def _nan_specials(dt):
+    """Quiet and signalling NaNs of both signs with several payloads, as raw bit patterns."""
+    if dt == np.float32:
+        pats = [0x7FC00000, 0x7F800001, 0x7FFFFFFF, 0x7FA00000, 0x7FC00123]
+        sign = 0x80000000
+        ut = np.uint32
+    else:
+        pats = [0x7FF8000000000000, 0x7FF0000000000001, 0x7FFFFFFFFFFFFFFF, 0x7FF4000000000000,
+                0x7FF8000000000ABC]
+        sign = 0x8000000000000000
+        ut = np.uint64
+    return np.array(pats + [p | sign for p in pats], dtype=ut)
+
+
+def test_nan_order_closed_form(orc):
+    """NaN keys in the radix order (util_type.cuh:1017-1027): every NaN with the sign bit
+    set comes before -inf (the largest negative-NaN bit pattern first), every NaN with the
+    sign bit clear after +inf (by increasing bit pattern), each NaN exactly where its
+    bit pattern puts it; descending order reverses the whole sequence with ties kept in
+    input order. The expected order is built from IEEE classes (numpy's isnan/signbit
+    and value order for non-NaN), not from the twiddle."""
+    for dt in (np.float32, np.float64):
+        ut = np.uint32 if dt == np.float32 else np.uint64
+        finite = np.array([0.0, -0.0, 1.0, -1.0, np.inf, -np.inf, 1e-45, -1e-45, 2.5, -2.5], dtype=dt).view(ut)
+        nans = _nan_specials(dt)
+        bits = np.concatenate([finite, nans, nans[:3]])  # repeated NaNs: ties by input position
+        x = bits.view(dt)
+        n = len(x)
+        neg_nan = [i for i in range(n) if np.isnan(x[i]) and np.signbit(x[i])]
+        pos_nan = [i for i in range(n) if np.isnan(x[i]) and not np.signbit(x[i])]
+        num = [i for i in range(n) if not np.isnan(x[i])]
+        # negative NaNs: larger magnitude payload first; positive NaNs: smaller first
+        neg_nan.sort(key=lambda i: (-int(bits[i]), i))
+        pos_nan.sort(key=lambda i: (int(bits[i]), i))
+        num.sort(key=lambda i: (float(x[i]), not np.signbit(x[i]), i))
+        want = neg_nan + num + pos_nan
+        _, vo = orc.seg_sort(x, np.arange(n, dtype=np.uint32), offsets=[0, n])
+        assert vo.tolist() == want, dt
+        # descending: the reverse order of distinct patterns, equal patterns still in input order
+        rank = {}
+        for r, i in enumerate(want):
+            rank.setdefault(int(bits[i]), r)
+        want_desc = sorted(range(n), key=lambda i: (-rank[int(bits[i])], i))
+        _, vd = orc.seg_sort(x, np.arange(n, dtype=np.uint32), offsets=[0, n], descending=True)
+        assert vd.tolist() == want_desc, dt
+
+
 def test_signed_order_closed_form(orc):
     for dt in (np.int32, np.int64):
         x = np.array([0, -1, 1, np.iinfo(dt).min, np.iinfo(dt).max, -5, 5, 5, -5], dtype=dt)

@@ -40,13 +40,16 @@ struct RadixTile {
     static_assert(RADIX % NDT == 0, "digit split");
     static_assert(TILE <= 65536, "16-bit source indices");
 
-    struct Smem {
+    struct alignas(16) Smem {
         U keys[TILE];
         union {
             V vals[HAS_V ? TILE : 1];                       // partition(): values ride along
             // sort(): source index per item, ping-pong; the top-16 finish keeps
-            // each slot's final place in the last array (keys only: the only one)
-            uint16_t idx[HAS_V ? 2 : 1][(HAS_V || TOP2_FINISH) ? TILE : 1];
+            // each slot's final place in the last array (keys only: the only one).
+            // The bucket finish of small tiles reads keys[] and this array as one
+            // array of TILE 64-bit sort words (32-bit keys), or keeps each
+            // slot's original position in idx[0] (64-bit keys).
+            uint16_t idx[(HAS_V || MSD_FINISH) ? 2 : 1][(HAS_V || TOP2_FINISH || MSD_FINISH) ? TILE : 1];
         };
         uint32_t wcnt[NW * RADIX];  // per-wave digit counts, then scatter bases
         uint32_t dummy[NT];         // store target of the lanes that do not advance a count
@@ -606,48 +609,105 @@ struct RadixTile {
         const int lo = lowest_bit<U>(varying);
         const int hi = bit_length<U>(varying);
         if constexpr (MSD_FINISH) {
-            // One stable pass on the top digit; when every bucket then holds at
-            // most MSD_GUARD items, each item's rank inside its bucket comes from
-            // comparing it with the bucket's items (smaller key, or equal key at
-            // an earlier position: the pass is stable) instead of three more passes.
+            // One pass places every item in its bucket of the top 8 varying bits
+            // (LDS atomics, unordered inside a bucket) as a sort word (masked
+            // order key, original position); when no bucket holds more than
+            // MSD_GUARD items, the item at each slot is ranked inside its bucket by
+            // comparing sort words -- the count of smaller words is its stable
+            // rank -- and written to its final place, instead of three more passes.
             if (hi - lo > RADIX_BITS) {
+                static_assert(NW <= 4, "one wave scans the bucket counts");
                 const int shift = hi - RADIX_BITS;
-                uint32_t dst[IPT];
-                rank_digits(sm, k, n, shift, RADIX_BITS, dst);
+                const U rm = range_mask<U>(km.begin_bit, km.end_bit);
+                uint32_t* cnt = sm.wcnt;  // RADIX counters (any wave's count area is free here)
+                const int t = threadIdx.x;
+                for (int d = t; d < RADIX; d += NT) cnt[d] = 0;
+                __syncthreads();
+                uint32_t rk[IPT];
+                if (act) {
+#pragma unroll
+                    for (int j = 0; j < IPT; ++j) {
+                        const int p = (lb + j * WAVE);
+                        if (p < n) rk[j] = atomicAdd(&cnt[(uint32_t)(k[j] >> shift) & (RADIX - 1)], 1u);
+                    }
+                }
+                __syncthreads();
+                if (wave() == 0) {
+                    const uint32_t l = t & 63;
+                    uint32_t c[RADIX / WAVE];
+#pragma unroll
+                    for (int i = 0; i < RADIX / WAVE; ++i) c[i] = cnt[i * WAVE + (int)l];
+                    uint32_t carry = 0, big = 0;
+#pragma unroll
+                    for (int i = 0; i < RADIX / WAVE; ++i) {
+                        const uint32_t incl = wave_incl_scan(c[i]);
+                        sm.dstart[i * WAVE + (int)l] = carry + incl - c[i];
+                        carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+                        big = c[i] > big ? c[i] : big;
+                    }
+                    big = wave_max(big);
+                    if (l == 0) {
+                        sm.dstart[RADIX] = (uint32_t)n;
+                        sm.maxcnt = big;
+                    }
+                }
+                __syncthreads();
                 if (sm.maxcnt <= (uint32_t)MSD_GUARD) {
                     if (act) {
 #pragma unroll
                         for (int j = 0; j < IPT; ++j) {
                             const int p = (lb + j * WAVE);
+                            const uint32_t d = (uint32_t)(k[j] >> shift) & (RADIX - 1);
                             if (p < n) {
-                                sm.keys[dst[j]] = k[j];
-                                if constexpr (HAS_V) sm.idx[0][dst[j]] = (uint16_t)p;
+                                const uint32_t at = sm.dstart[d] + rk[j];
+                                if constexpr (sizeof(U) == 4) {
+                                    reinterpret_cast<uint64_t*>(sm.keys)[at] =
+                                        (uint64_t(U(k[j] & rm)) << 32) | uint64_t(p);
+                                } else {
+                                    sm.keys[at] = U(k[j] & rm);
+                                    sm.idx[0][at] = (uint16_t)p;
+                                }
                             }
                         }
                     }
                     __syncthreads();
                     prefetch();
+                    // each thread finishes the items at its slots of the bucket
+                    // order (neighbouring lanes share buckets: coherent loops and
+                    // broadcast reads); the sort word names the item's input position
+                    const bool full = rm == Bits<U>::ALL;
                     if (act) {
-                        // the order covers only the key bits [begin_bit, end_bit)
-                        const U rm = range_mask<U>(km.begin_bit, km.end_bit);
 #pragma unroll
                         for (int j = 0; j < IPT; ++j) {
-                            const int p = (lb + j * WAVE);
-                            if (p < n) {
-                                const U kk = sm.keys[p];
-                                const U km_ = U(kk & rm);
-                                const uint32_t d = (uint32_t)(kk >> shift) & (RADIX - 1);
-                                const int s0 = (int)sm.dstart[d];
-                                const int e0 = d + 1 < RADIX ? (int)sm.dstart[d + 1] : n;
-                                const int e = e0 < n ? e0 : n;
+                            const int q0 = (lb + j * WAVE);
+                            if (q0 < n) {
+                                U mk;
+                                uint32_t p;
                                 uint32_t c = 0;
-                                for (int q = s0; q < e; ++q) {
-                                    const U kq = U(sm.keys[q] & rm);
-                                    c += (kq < km_) | ((kq == km_) & (q < p));
+                                int s0;
+                                if constexpr (sizeof(U) == 4) {
+                                    const uint64_t* w = reinterpret_cast<const uint64_t*>(sm.keys);
+                                    const uint64_t mine = w[q0];
+                                    mk = U(mine >> 32);
+                                    p = (uint32_t)mine;
+                                    const uint32_t d = (uint32_t)(mk >> shift) & (RADIX - 1);
+                                    s0 = (int)sm.dstart[d];
+                                    const int e0 = (int)sm.dstart[d + 1];
+                                    for (int q = s0; q < e0; ++q) c += w[q] < mine;
+                                } else {
+                                    mk = sm.keys[q0];
+                                    p = sm.idx[0][q0];
+                                    const uint32_t d = (uint32_t)(mk >> shift) & (RADIX - 1);
+                                    s0 = (int)sm.dstart[d];
+                                    const int e0 = (int)sm.dstart[d + 1];
+                                    for (int q = s0; q < e0; ++q) {
+                                        const U kq = sm.keys[q];
+                                        c += (kq < mk) | ((kq == mk) & ((uint32_t)sm.idx[0][q] < p));
+                                    }
                                 }
                                 const int r = s0 + (int)c;
-                                ko[begin + r] = twiddle_out<U>(kk, km.kind, km.desc);
-                                if constexpr (HAS_V) vo[begin + r] = vs[begin + sm.idx[0][p]];
+                                ko[begin + r] = full ? twiddle_out<U>(mk, km.kind, km.desc) : ks[begin + p];
+                                if constexpr (HAS_V) vo[begin + r] = vs[begin + p];
                             }
                         }
                     }
@@ -1100,15 +1160,26 @@ __device__ __forceinline__ void wave_bitonic_sort(const U* ks, const V* vs, U* k
 // ---- one wave, one counting pass and a compare-count inside the buckets ----------
 
 constexpr int WAVE_SORT_MAX = 512;  // items of one wave-tier range
+// Wave bucket finish: lane l ranks the items at slots r*64 + l of the bucket
+// order (neighbouring lanes share buckets: coherent loops, broadcast reads; the
+// keys and values are then gathered by input position) when the range has more
+// than two items per lane, else its own input items (keys and values straight
+// from registers). Measured on MI355X: 2^16 segments of 256 items 0.116 ->
+// 0.107 ms by slot, 2^18 segments of 64 items 0.116 -> 0.125 ms by slot.
+template <int R>
+constexpr bool wave_finish_by_slot() { return R >= 4; }
 
 // The wave's LDS region for wave_msd_sort, for ranges of at most MAXN items.
+// The bucket path of 32-bit keys reads keys[] and vals[] together as one
+// array of MAXN 64-bit sort words (they are adjacent and 16-byte aligned).
 template <typename U, typename V, int MAXN = WAVE_SORT_MAX>
-struct WaveSmem {
-    U keys[MAXN];        // raw keys in bucket order
+struct alignas(16) WaveSmem {
+    U keys[MAXN];        // raw keys in bucket order; 64-bit keys: order keys of the bucket path
     V vals[MAXN];        // values by original position
     uint16_t pos[MAXN];  // original position of each bucket-ordered key
-    uint32_t cnt[RADIX + 1];      // running digit counts, then bucket starts (cnt[RADIX] = n)
+    uint32_t cnt[RADIX + 1];      // running digit counts, then bucket starts (cnt[buckets] = n)
     uint32_t dummy[WAVE];         // store target of the lanes that do not advance a count
+    static_assert(sizeof(U) == 8 || sizeof(U) * MAXN + sizeof(V) * MAXN >= 8 * MAXN, "sort words overlay");
 };
 
 // Lanes whose `bits`-bit digit d equals the calling lane's (see
@@ -1125,14 +1196,42 @@ __device__ __forceinline__ uint64_t wave_match_rt(uint32_t d, int bits) {
     return (uint64_t(acc_hi) << 32) | uint64_t(acc_lo);
 }
 
-// One wave sorts a range of n <= 64*R items: a stable counting pass on the
-// highest (up to) 8 varying bits of the order keys, then, unless that digit
-// held every varying bit, each item's rank inside its bucket by comparing it
-// with the bucket's other items (smaller key, or equal key earlier in the
-// bucket, which after the stable pass is earlier in the input). Returns false,
-// having written nothing, when a bucket exceeds `guard` items (then a network
-// sort is cheaper). Lanes of one wave need no barrier: its LDS operations
-// complete in order.
+// Exclusive scan of the wave's first `nbk` (<= RADIX) counters in place: lane l
+// takes counters l, l+64, l+128, l+192; cnt[nbk] = n afterwards. Returns the
+// largest count to every lane. LDS operations of one wave complete in order,
+// so every lane's reads precede every lane's writes.
+template <typename U, typename V, int MAXN>
+__device__ __forceinline__ uint32_t wave_scan_counts(WaveSmem<U, V, MAXN>& ws, int nbk, int n) {
+    const int l = (int)lane_id();
+    uint32_t c[RADIX / WAVE];
+#pragma unroll
+    for (int i = 0; i < RADIX / WAVE; ++i) c[i] = i * WAVE + l < nbk ? ws.cnt[i * WAVE + l] : 0u;
+    uint32_t carry = 0, big = 0;
+#pragma unroll
+    for (int i = 0; i < RADIX / WAVE; ++i) {
+        if (i * WAVE >= nbk) break;  // wave-uniform
+        const uint32_t incl = wave_incl_scan(c[i]);
+        if (i * WAVE + l < nbk) ws.cnt[i * WAVE + l] = carry + incl - c[i];
+        carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+        big = c[i] > big ? c[i] : big;
+    }
+    if (l == 0) ws.cnt[nbk] = (uint32_t)n;
+    return wave_max(big);
+}
+
+// One wave sorts a range of n <= 64*R items. Order keys are the twiddled keys
+// masked to the bit range; the range's varying bits are [lo, hi).
+//  * hi - lo <= 8: one stable counting pass on those bits orders the range
+//    (digit ranks from 64-bit-ballot peer masks), written straight out;
+//  * else: a bucket pass on the top `nb` varying bits (nb about log2(2n), at
+//    most 8) places every item in its bucket through LDS atomics (unordered
+//    inside the bucket), then each lane ranks its own items inside their
+//    bucket by comparing sort words (order key, original position): the count
+//    of smaller words is the item's stable rank. Keys and values go from the
+//    lane's registers straight to their final places.
+// Returns false, having written nothing, when a bucket exceeds `guard` items
+// (then a network sort is cheaper). Lanes of one wave need no barrier: its LDS
+// operations complete in order.
 template <int R, typename U, typename V, bool HAS_V, int MAXN>
 __device__ bool wave_msd_sort(const U* ks, const V* vs, U* ko, V* vo, int64_t begin, int n, const KeyMode& km,
                               WaveSmem<U, V, MAXN>& ws, int guard) {
@@ -1168,44 +1267,99 @@ __device__ bool wave_msd_sort(const U* ks, const V* vs, U* ko, V* vo, int64_t be
         return true;
     }
     const int hi = bit_length<U>(var), lo = lowest_bit<U>(var);
-    const int shift = hi - RADIX_BITS > lo ? hi - RADIX_BITS : lo;
-    const uint32_t dm = (1u << (hi - shift)) - 1u;
+    if (hi - lo <= RADIX_BITS) {
+        // one stable counting pass holds every varying bit
+        const int bits = hi - lo;
+        const uint32_t dm = (1u << bits) - 1u;
 #pragma unroll
-    for (int i = 0; i < RADIX / WAVE; ++i) ws.cnt[i * WAVE + l] = 0;
-    uint32_t rk[R], dg[R];
+        for (int i = 0; i < RADIX / WAVE; ++i) ws.cnt[i * WAVE + l] = 0;
+        uint32_t rk[R], dg[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const bool valid = r * WAVE + l < n;
-        dg[r] = (uint32_t)(ok[r] >> shift) & dm;
-        const uint32_t base = ws.cnt[dg[r]];
-        const uint64_t peers = wave_match_rt(dg[r], hi - shift) & __ballot(valid);
-        const uint32_t below = lanes_below(peers);
-        uint32_t* dst = (valid && below == 0) ? &ws.cnt[dg[r]] : &ws.dummy[l];
-        *dst = base + (uint32_t)__popcll(peers);
-        rk[r] = base + below;
-    }
-    // bucket starts: lane l scans digits l, l+64, l+128, l+192
-    uint32_t carry = 0, big = 0;
-#pragma unroll
-    for (int i = 0; i < RADIX / WAVE; ++i) {
-        const uint32_t c = ws.cnt[i * WAVE + l];
-        const uint32_t incl = wave_incl_scan(c);
-        ws.cnt[i * WAVE + l] = carry + incl - c;
-        carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-        big = c > big ? c : big;
-    }
-    if (l == 0) ws.cnt[RADIX] = (uint32_t)n;
-    const bool single = shift == lo;  // this digit held every varying bit
-    if (!single && wave_max(big) > (uint32_t)guard) return false;
-    uint32_t dst[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) dst[r] = ws.cnt[dg[r]] + rk[r];
-    if (single) {
+        for (int r = 0; r < R; ++r) {
+            const bool valid = r * WAVE + l < n;
+            dg[r] = (uint32_t)(ok[r] >> lo) & dm;
+            const uint32_t base = ws.cnt[dg[r]];
+            const uint64_t peers = wave_match_rt(dg[r], bits) & __ballot(valid);
+            const uint32_t below = lanes_below(peers);
+            uint32_t* dst = (valid && below == 0) ? &ws.cnt[dg[r]] : &ws.dummy[l];
+            *dst = base + (uint32_t)__popcll(peers);
+            rk[r] = base + below;
+        }
+        wave_scan_counts(ws, 1 << bits, n);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             if (r * WAVE + l < n) {
-                ko[begin + dst[r]] = raw[r];
-                if constexpr (HAS_V) vo[begin + dst[r]] = v[r];
+                const uint32_t d = ws.cnt[dg[r]] + rk[r];
+                ko[begin + d] = raw[r];
+                if constexpr (HAS_V) vo[begin + d] = v[r];
+            }
+        }
+        return true;
+    }
+    // bucket pass on the top nb varying bits: about two buckets per item
+    int nb = bit_length<uint32_t>((uint32_t)n) + 1;
+    nb = nb < RADIX_BITS ? nb : RADIX_BITS;
+    const int shift = hi - nb;
+    const int nbk = 1 << nb;
+    const uint32_t dm = (uint32_t)nbk - 1u;
+#pragma unroll
+    for (int i = 0; i < RADIX / WAVE; ++i)
+        if (i * WAVE + l < nbk) ws.cnt[i * WAVE + l] = 0;
+    uint32_t rk[R], dg[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        dg[r] = (uint32_t)(ok[r] >> shift) & dm;
+        if (r * WAVE + l < n) rk[r] = atomicAdd(&ws.cnt[dg[r]], 1u);  // place inside the bucket (unordered)
+    }
+    if (wave_scan_counts(ws, nbk, n) > (uint32_t)guard) return false;
+    // sort words in bucket order: (order key, original position)
+    uint32_t st[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int p = r * WAVE + l;
+        st[r] = ws.cnt[dg[r]];
+        if (p < n) {
+            const uint32_t at = st[r] + rk[r];
+            if constexpr (sizeof(U) == 4) {
+                reinterpret_cast<uint64_t*>(ws.keys)[at] = (uint64_t(ok[r]) << 32) | uint64_t(p);
+            } else {
+                ws.keys[at] = ok[r];
+                ws.pos[at] = (uint16_t)p;
+            }
+        }
+    }
+    if constexpr (wave_finish_by_slot<R>()) {
+        // lane l finishes the items at slots r*64 + l of the bucket order
+        const bool full = rm == Bits<U>::ALL;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int q0 = r * WAVE + l;
+            if (q0 < n) {
+                U mk;
+                uint32_t p, c = 0;
+                int s;
+                if constexpr (sizeof(U) == 4) {
+                    const uint64_t* w = reinterpret_cast<const uint64_t*>(ws.keys);
+                    const uint64_t mine = w[q0];
+                    mk = U(mine >> 32);
+                    p = (uint32_t)mine;
+                    const uint32_t d = (uint32_t)(mk >> shift) & dm;
+                    s = (int)ws.cnt[d];
+                    const int e = (int)ws.cnt[d + 1];
+                    for (int q = s; q < e; ++q) c += w[q] < mine;
+                } else {
+                    mk = ws.keys[q0];
+                    p = ws.pos[q0];
+                    const uint32_t d = (uint32_t)(mk >> shift) & dm;
+                    s = (int)ws.cnt[d];
+                    const int e = (int)ws.cnt[d + 1];
+                    for (int q = s; q < e; ++q) {
+                        const U kq = ws.keys[q];
+                        c += (kq < mk) | ((kq == mk) & ((uint32_t)ws.pos[q] < p));
+                    }
+                }
+                ko[begin + s + (int)c] = full ? twiddle_out<U>(mk, km.kind, km.desc) : ks[begin + p];
+                if constexpr (HAS_V) vo[begin + s + (int)c] = vs[begin + p];
             }
         }
         return true;
@@ -1214,26 +1368,20 @@ __device__ bool wave_msd_sort(const U* ks, const V* vs, U* ko, V* vo, int64_t be
     for (int r = 0; r < R; ++r) {
         const int p = r * WAVE + l;
         if (p < n) {
-            ws.keys[dst[r]] = raw[r];
-            ws.pos[dst[r]] = (uint16_t)p;
-            if constexpr (HAS_V) ws.vals[p] = v[r];
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int q0 = r * WAVE + l;
-        if (q0 < n) {
-            const U kr = ws.keys[q0];
-            const U kk = U(twiddle_in<U>(kr, km.kind, km.desc) & rm);
-            const uint32_t d = (uint32_t)(kk >> shift) & dm;
-            const int s = (int)ws.cnt[d], e = (int)ws.cnt[d + 1];
+            const int s = (int)st[r], e = (int)ws.cnt[dg[r] + 1];
             uint32_t c = 0;
-            for (int q = s; q < e; ++q) {
-                const U kq = U(twiddle_in<U>(ws.keys[q], km.kind, km.desc) & rm);
-                c += (kq < kk) | ((kq == kk) & (q < q0));
+            if constexpr (sizeof(U) == 4) {
+                const uint64_t* w = reinterpret_cast<const uint64_t*>(ws.keys);
+                const uint64_t mine = (uint64_t(ok[r]) << 32) | uint64_t(p);
+                for (int q = s; q < e; ++q) c += w[q] < mine;
+            } else {
+                for (int q = s; q < e; ++q) {
+                    const U kq = ws.keys[q];
+                    c += (kq < ok[r]) | ((kq == ok[r]) & ((int)ws.pos[q] < p));
+                }
             }
-            ko[begin + s + (int)c] = kr;
-            if constexpr (HAS_V) vo[begin + s + (int)c] = ws.vals[ws.pos[q0]];
+            ko[begin + s + (int)c] = raw[r];
+            if constexpr (HAS_V) vo[begin + s + (int)c] = v[r];
         }
     }
     return true;
@@ -1269,10 +1417,12 @@ template <typename U, typename V, bool HAS_V, int MAXN>
 __device__ __forceinline__ void wave_sort(const U* ks, const V* vs, U* ko, V* vo, int64_t begin, int n,
                                           const KeyMode& km, int rank_max, WaveSmem<U, V, MAXN>& ws) {
     static_assert(MAXN == 256 || MAXN == 512, "wave-tier range limit");
-    constexpr int GUARD = 16;  // largest bucket the compare-count finish takes on
+    constexpr int GUARD = 24;  // largest bucket the compare-count finish takes on
     if (n <= rank_max) wave_rank_sort<U, V, HAS_V>(ks, vs, ko, vo, begin, n, km);
-    else if (n <= 64) wave_bitonic_sort<1, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km, ws.vals);
-    else if (n <= 128) {
+    else if (n <= 64) {
+        if (!wave_msd_sort<1, U, V, HAS_V, MAXN>(ks, vs, ko, vo, begin, n, km, ws, GUARD))
+            wave_bitonic_sort<1, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km, ws.vals);
+    } else if (n <= 128) {
         if (!wave_msd_sort<2, U, V, HAS_V, MAXN>(ks, vs, ko, vo, begin, n, km, ws, GUARD))
             wave_bitonic_sort<2, U, V, HAS_V>(ks, vs, ko, vo, begin, n, km, ws.vals);
     } else if (MAXN == 256 || n <= 256) {

@@ -24,15 +24,27 @@ struct RadixTile {
     static constexpr int NDT = NT < RADIX ? NT : RADIX;  // threads in the digit scan
     static constexpr int DPT = RADIX / NDT;               // digits per scanning thread
     static constexpr int NDW = (NDT + WAVE - 1) / WAVE;   // waves in the digit scan
-    // top-digit pass plus in-bucket compare-count for tiles of up to 1024 items
-    // (measured slower than three more passes at 4096)
-    static constexpr bool MSD_FINISH = NW <= 4 && TILE <= 1024;
-    static constexpr int MSD_GUARD = 24;
-    // larger tiles: two passes on the top 16 varying bits, then a compare-count
-    // inside each run of equal top bits (runs of at most TOP_GUARD items)
-    // (keys only: when a 16-bit place per slot still fits in LDS)
+    // Bucket finish (every tile size): one pass puts each item in its bucket of
+    // the top varying bits (up to NW*256 buckets, about four items each), then
+    // a compare-count of sort words inside the bucket gives its stable rank;
+    // taken while no bucket exceeds BUCKET_GUARD items.
+    static constexpr int BUCKET_GUARD = 32;
+    // tiles of at most 4 waves and 1024 items: 8-bit buckets, counted and
+    // scanned by one wave, no sample check (measured faster there than wider
+    // buckets, whose larger code made the shared wave/tile kernel spill)
+    static constexpr bool SMALL_BUCKET = NW <= 4 && TILE <= 1024;
+    static constexpr int SMALL_GUARD = 24;
+    // (when the sort words -- 8 bytes per item over keys[] and idx[] -- fit in LDS)
+    static constexpr bool BUCKET_FINISH = size_t(TILE) * (sizeof(U) + 4) + size_t(NW) * RADIX * 4 +
+                                              size_t(NT) * 4 + 4096 <= size_t(160 * 1024);
+    static constexpr int NBK_LOG = NW <= 1 ? 8 : NW <= 2 ? 9 : NW <= 4 ? 10 : NW <= 8 ? 11 : 12;  // log2(NW*RADIX)
+    static_assert((1 << NBK_LOG) == NW * RADIX, "bucket counters are the wave-count words");
+    // larger tiles whose buckets overflow (skewed keys): two passes on the top
+    // 16 varying bits, then a compare-count inside each run of equal top bits
+    // (runs of at most TOP_GUARD items); keys only: when a 16-bit place per
+    // slot still fits in LDS
     static constexpr bool TOP2_FINISH =
-        !MSD_FINISH && (HAS_V || size_t(TILE) * (sizeof(U) + 2) + size_t(NW) * RADIX * 4 + size_t(NT) * 4 + 4096 <=
+        TILE > 1024 && (HAS_V || size_t(TILE) * (sizeof(U) + 2) + size_t(NW) * RADIX * 4 + size_t(NT) * 4 + 4096 <=
                                      size_t(160 * 1024));
     static constexpr int TOP_GUARD = 16;
     static_assert(IPT <= 32, "one bit per slot");
@@ -45,11 +57,11 @@ struct RadixTile {
         union {
             V vals[HAS_V ? TILE : 1];                       // partition(): values ride along
             // sort(): source index per item, ping-pong; the top-16 finish keeps
-            // each slot's final place in the last array (keys only: the only one).
-            // The bucket finish of small tiles reads keys[] and this array as one
-            // array of TILE 64-bit sort words (32-bit keys), or keeps each
-            // slot's original position in idx[0] (64-bit keys).
-            uint16_t idx[(HAS_V || MSD_FINISH) ? 2 : 1][(HAS_V || TOP2_FINISH || MSD_FINISH) ? TILE : 1];
+            // each slot's final place in the last array. The bucket finish reads
+            // keys[] and this array as one array of TILE 64-bit sort words
+            // (32-bit keys), or keeps each slot's input position in idx[0]
+            // (64-bit keys).
+            uint16_t idx[(HAS_V || BUCKET_FINISH) ? 2 : 1][(HAS_V || TOP2_FINISH || BUCKET_FINISH) ? TILE : 1];
         };
         uint32_t wcnt[NW * RADIX];  // per-wave digit counts, then scatter bases
         uint32_t dummy[NT];         // store target of the lanes that do not advance a count
@@ -57,9 +69,13 @@ struct RadixTile {
         uint32_t maxcnt;            // largest digit count of the last pass over real items (NW <= 4)
         uint32_t goff[RADIX];       // global destination of each digit (partition)
         uint32_t wsum[NDW];
+        uint32_t bsum[NW];          // bucket scan: per-wave totals
+        uint32_t bmax[NW];          // bucket scan: per-wave largest bucket
         U red_or[NW];
         U red_and[NW];
     };
+    static_assert(!BUCKET_FINISH || sizeof(U) == 8 || (sizeof(U) * TILE) % 8 == 0,
+                  "sort words overlay keys[] and idx[]");
 
     __device__ static __forceinline__ int wave() { return threadIdx.x >> 6; }
     __device__ static __forceinline__ int pos(int j) { return wave() * WITEMS + j * WAVE + (threadIdx.x & 63); }
@@ -319,6 +335,159 @@ struct RadixTile {
                 raw[j] = ks[begin + (p < n ? p : n - 1)];
             }
         }
+    }
+
+    // Buckets of a range of n items with varying order-key bits [lo, hi): the
+    // top nb varying bits, about 2^nb >= n / 2 buckets, at most NW*256.
+    __device__ static __forceinline__ int bucket_bits(int n, int hi, int lo) {
+        int nb = bit_length<uint32_t>((uint32_t)n) - 1;
+        nb = nb < NBK_LOG ? nb : NBK_LOG;
+        nb = nb > RADIX_BITS ? nb : RADIX_BITS;
+        return nb < hi - lo ? nb : hi - lo;
+    }
+
+    // Whether the bucket finish is likely to fit: among the range's first 64
+    // keys (the sample in sm.keys), the pairs that share their bucket estimate
+    // how many items share an item's bucket, (pairs / (64*63/2)) * n; the
+    // finish is tried while that stays at most BUCKET_GUARD / 4. Skewed keys
+    // (floats of a normal distribution) fail it and go to the top-16 passes.
+    __device__ static __forceinline__ bool bucket_promising(Smem& sm, int n, const KeyMode& km, int hi, int lo) {
+        const int nb = bucket_bits(n, hi, lo);
+        const U rm = range_mask<U>(km.begin_bit, km.end_bit);
+        const uint32_t l = threadIdx.x & 63;
+        const int ns = n < WAVE ? n : WAVE;
+        const U kk = sm.keys[l < (uint32_t)ns ? l : 0];
+        const uint32_t b = (uint32_t)(U((kk & rm) >> (hi - nb)) & U((1u << nb) - 1));
+        const uint64_t peers = match_digit_rt(b, nb) & __ballot(l < (uint32_t)ns);
+        const uint32_t pairs = wave_sum(l < (uint32_t)ns ? (uint32_t)__popcll(peers) - 1u : 0u) / 2u;
+        const uint32_t all_pairs = (uint32_t)(ns * (ns - 1) / 2);
+        return (uint64_t)pairs * (uint64_t)n * 4u <= (uint64_t)BUCKET_GUARD * all_pairs + 4u * (uint64_t)n;
+    }
+
+    // The bucket finish. Every item goes to a slot of its bucket (the top nb
+    // varying order-key bits) through LDS atomics on the block's bucket
+    // counters (unordered inside a bucket), as a sort word (masked order key,
+    // input position); a scan of the counters gives the bucket starts. When
+    // no bucket holds more than BUCKET_GUARD items, the item at each slot is
+    // ranked inside its bucket by counting the smaller sort words -- its stable
+    // rank -- and written to its final place (threads take consecutive slots,
+    // so the lanes of a wave share buckets: coherent loops, broadcast reads);
+    // keys and values are gathered by input position. Returns false, with
+    // nothing written to global memory and the prefetch hook not run, when a
+    // bucket overflows.
+    template <typename Prefetch>
+    __device__ static __forceinline__ bool bucket_finish(Smem& sm, const U (&k)[IPT], const U* ks, const V* vs,
+                                                         U* ko, V* vo, int64_t begin, int n, const KeyMode& km,
+                                                         int hi, int lo, Prefetch& prefetch) {
+        constexpr int CPT = (NW * RADIX) / NT;  // bucket counters per thread in the scan (4)
+        const int lb = lane_base(n);
+        const bool act = wave() < active_waves(n);
+        const int nb = bucket_bits(n, hi, lo);
+        const int shift = hi - nb;
+        const uint32_t bm = (1u << nb) - 1u;
+        const U rm = range_mask<U>(km.begin_bit, km.end_bit);
+        uint32_t* cnt = sm.wcnt;  // NW * RADIX counters
+        const int t = threadIdx.x;
+        const int w = wave();
+        const uint32_t l = t & 63;
+#pragma unroll
+        for (int i = 0; i < CPT; ++i) cnt[i * NT + t] = 0;
+        __syncthreads();
+        uint32_t rk[IPT];
+        if (act) {
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int p = (lb + j * WAVE);
+                if (p < n) rk[j] = atomicAdd(&cnt[(uint32_t)(k[j] >> shift) & bm], 1u);
+            }
+        }
+        __syncthreads();
+        // exclusive scan of the counters: thread t owns counters t*CPT .. t*CPT+CPT-1
+        {
+            uint32_t c[CPT], sum = 0, big = 0;
+#pragma unroll
+            for (int i = 0; i < CPT; ++i) {
+                c[i] = cnt[t * CPT + i];
+                sum += c[i];
+                big = c[i] > big ? c[i] : big;
+            }
+            const uint32_t incl = wave_incl_scan(sum);
+            big = wave_max(big);
+            if (l == 63) sm.bsum[w] = incl;
+            if (l == 0) sm.bmax[w] = big;
+            __syncthreads();
+            uint32_t base = incl - sum;
+#pragma unroll
+            for (int ww = 0; ww < NW; ++ww) base += ww < w ? sm.bsum[ww] : 0u;
+#pragma unroll
+            for (int i = 0; i < CPT; ++i) {
+                cnt[t * CPT + i] = base;
+                base += c[i];
+            }
+            if (t == 0) {
+                uint32_t m = 0;
+#pragma unroll
+                for (int ww = 0; ww < NW; ++ww) m = sm.bmax[ww] > m ? sm.bmax[ww] : m;
+                sm.maxcnt = m;
+            }
+        }
+        __syncthreads();
+        if (sm.maxcnt > (uint32_t)BUCKET_GUARD) return false;
+        if (act) {
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int p = (lb + j * WAVE);
+                if (p < n) {
+                    const uint32_t at = cnt[(uint32_t)(k[j] >> shift) & bm] + rk[j];
+                    if constexpr (sizeof(U) == 4) {
+                        reinterpret_cast<uint64_t*>(sm.keys)[at] = (uint64_t(U(k[j] & rm)) << 32) | uint64_t(p);
+                    } else {
+                        sm.keys[at] = U(k[j] & rm);
+                        sm.idx[0][at] = (uint16_t)p;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        prefetch();
+        const bool full = rm == Bits<U>::ALL;
+        const int nbk = 1 << nb;
+        if (act) {
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int q0 = (lb + j * WAVE);
+                if (q0 < n) {
+                    U mk;
+                    uint32_t p, c = 0;
+                    int s0, e0;
+                    if constexpr (sizeof(U) == 4) {
+                        const uint64_t* wd = reinterpret_cast<const uint64_t*>(sm.keys);
+                        const uint64_t mine = wd[q0];
+                        mk = U(mine >> 32);
+                        p = (uint32_t)mine;
+                        const uint32_t d = (uint32_t)(mk >> shift) & bm;
+                        s0 = (int)cnt[d];
+                        e0 = (int)d + 1 < nbk ? (int)cnt[d + 1] : n;
+                        for (int q = s0; q < e0; ++q) c += wd[q] < mine;
+                    } else {
+                        mk = sm.keys[q0];
+                        p = sm.idx[0][q0];
+                        const uint32_t d = (uint32_t)(mk >> shift) & bm;
+                        s0 = (int)cnt[d];
+                        e0 = (int)d + 1 < nbk ? (int)cnt[d + 1] : n;
+                        for (int q = s0; q < e0; ++q) {
+                            const U kq = sm.keys[q];
+                            c += (kq < mk) | ((kq == mk) & ((uint32_t)sm.idx[0][q] < p));
+                        }
+                    }
+                    const int r = s0 + (int)c;
+                    ko[begin + r] = full ? twiddle_out<U>(mk, km.kind, km.desc) : ks[begin + p];
+                    if constexpr (HAS_V) vo[begin + r] = vs[begin + p];
+                }
+            }
+        }
+        __syncthreads();
+        return true;
     }
 
     // Whether runs of equal top 16 varying bits are likely short enough for the
@@ -585,7 +754,7 @@ struct RadixTile {
         }
         // the range's first 64 keys, read back by every wave after the barrier
         // in reduce_or_and (the key area is not used before the first pass)
-        if constexpr (TOP2_FINISH) {
+        if constexpr (TOP2_FINISH || (BUCKET_FINISH && !SMALL_BUCKET)) {
             if (wave() == 0) sm.keys[threadIdx.x] = k[0];
         }
         U o, a;
@@ -608,11 +777,11 @@ struct RadixTile {
         }
         const int lo = lowest_bit<U>(varying);
         const int hi = bit_length<U>(varying);
-        if constexpr (MSD_FINISH) {
+        if constexpr (SMALL_BUCKET) {
             // One pass places every item in its bucket of the top 8 varying bits
             // (LDS atomics, unordered inside a bucket) as a sort word (masked
             // order key, original position); when no bucket holds more than
-            // MSD_GUARD items, the item at each slot is ranked inside its bucket by
+            // SMALL_GUARD items, the item at each slot is ranked inside its bucket by
             // comparing sort words -- the count of smaller words is its stable
             // rank -- and written to its final place, instead of three more passes.
             if (hi - lo > RADIX_BITS) {
@@ -652,7 +821,7 @@ struct RadixTile {
                     }
                 }
                 __syncthreads();
-                if (sm.maxcnt <= (uint32_t)MSD_GUARD) {
+                if (sm.maxcnt <= (uint32_t)SMALL_GUARD) {
                     if (act) {
 #pragma unroll
                         for (int j = 0; j < IPT; ++j) {
@@ -714,6 +883,11 @@ struct RadixTile {
                     __syncthreads();
                     return;
                 }
+            }
+        }
+        if constexpr (BUCKET_FINISH && !SMALL_BUCKET) {
+            if (hi - lo > RADIX_BITS && bucket_promising(sm, n, km, hi, lo)) {
+                if (bucket_finish(sm, k, ks, vs, ko, vo, begin, n, km, hi, lo, prefetch)) return;
             }
         }
         if constexpr (TOP2_FINISH) {

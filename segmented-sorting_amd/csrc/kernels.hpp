@@ -61,9 +61,19 @@ struct Sched {
     int child_q;  // queue of class c's MSD children: child_q + c (0: the binned ranges' queue)
 };
 
+// Every lane of a wave reads the same count and the same queue entry: taken
+// into scalar registers, the range's bounds and its buffer pointers are
+// wave-uniform, so global addresses are a scalar base plus a 32-bit lane offset.
 __device__ __forceinline__ int tile_count(const Sched& sc, int cls) {
     const uint32_t c = sc.ctr->tile_count[cls];
-    return (int)(c < sc.tq_cap[cls] ? c : sc.tq_cap[cls]);
+    return __builtin_amdgcn_readfirstlane((int)(c < sc.tq_cap[cls] ? c : sc.tq_cap[cls]));
+}
+
+__device__ __forceinline__ TileItem queue_item(const TileItem* q, int i) {
+    TileItem t = q[i];
+    t.begin = __builtin_amdgcn_readfirstlane(t.begin);
+    t.n_src = __builtin_amdgcn_readfirstlane(t.n_src);
+    return t;
 }
 
 __device__ __forceinline__ void level_counts(const Sched& sc, int level, int& count, int& chunks) {
@@ -289,13 +299,13 @@ template <typename U, typename V, bool HAS_V, int MAXN>
 __global__ __launch_bounds__(256, MAXN <= 256 ? 8 : 6) void k_wave_tier(Bufs b, Sched sc, int cls, KeyMode km, int persistent,
                                                    int rank_max) {
     const int waves_per_block = blockDim.x / WAVE;
-    const int wid = blockIdx.x * waves_per_block + (threadIdx.x >> 6);
+    const int wid = __builtin_amdgcn_readfirstlane(blockIdx.x * waves_per_block + (threadIdx.x >> 6));
     const int stride = persistent ? gridDim.x * waves_per_block : 0x7FFFFFFF;
     const int count = tile_count(sc, cls);
     __shared__ WaveSmem<U, V, MAXN> wsm[4];  // one region per wave (blocks of 4 waves)
     WaveSmem<U, V, MAXN>& my = wsm[threadIdx.x >> 6];
     for (int i = wid; i < count; i += stride) {
-        const TileItem it = sc.tq[cls][i];
+        const TileItem it = queue_item(sc.tq[cls], i);
         const int src = ti_src(it);
         wave_sort<U, V, HAS_V, MAXN>((const U*)b.k[src], (const V*)b.v[src], buf_ptr<U>(b.k[BUF_OUT]),
                                buf_ptr<V>(b.v[BUF_OUT]), it.begin, ti_n(it), km, rank_max, my);
@@ -353,7 +363,7 @@ __device__ __forceinline__ void tile_loop_pipelined(typename RadixTile<NT, IPT, 
     const int count = tile_count(sc, cls);
     int i = blockIdx.x;
     if (i >= count) return;
-    TileItem it = sc.tq[cls][i];
+    TileItem it = queue_item(sc.tq[cls], i);
     U raw[IPT];
     T::load_raw((const U*)b.k[ti_src(it)], it.begin, ti_n(it), raw);
     for (;;) {
@@ -366,7 +376,7 @@ __device__ __forceinline__ void tile_loop_pipelined(typename RadixTile<NT, IPT, 
         T::sort_loaded(sm, keys, (const U*)b.k[src], (const V*)b.v[src], buf_ptr<U>(b.k[BUF_OUT]),
                        buf_ptr<V>(b.v[BUF_OUT]), cur.begin, ti_n(cur), km, [&] {
                            if (next < count) {
-                               it = sc.tq[cls][next];
+                               it = queue_item(sc.tq[cls], next);
                                T::load_raw((const U*)b.k[ti_src(it)], it.begin, ti_n(it), raw);
                            }
                        });
@@ -395,7 +405,7 @@ __global__ __launch_bounds__(NT, 4) void k_tile_tier(Bufs b, Sched sc, int cls, 
         } sm;
         const int count = tile_count(sc, cls);
         for (int i = blockIdx.x; i < count; i += persistent ? (int)gridDim.x : count) {
-            const TileItem cur = sc.tq[cls][i];
+            const TileItem cur = queue_item(sc.tq[cls], i);
             const int src = ti_src(cur);
             const int n = ti_n(cur);
             const U* ks = (const U*)b.k[src];
@@ -444,11 +454,11 @@ __global__ __launch_bounds__(256, 8) void k_wave_tile_tier(Bufs b, Sched sc, int
         if (gq.q[3] >= 0) group_loop<32, U, V, HAS_V>(b, sc, gq.q[3], km);
     }
     {
-        const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+        const int wid = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
         const int count = tile_count(sc, qw);
         WaveSmem<U, V, MAXN>& my = sm.wsm[threadIdx.x >> 6];
         for (int i = wid; i < count; i += (int)gridDim.x * 4) {
-            const TileItem it = sc.tq[qw][i];
+            const TileItem it = queue_item(sc.tq[qw], i);
             const int src = ti_src(it);
             wave_sort<U, V, HAS_V, MAXN>((const U*)b.k[src], (const V*)b.v[src], buf_ptr<U>(b.k[BUF_OUT]),
                                          buf_ptr<V>(b.v[BUF_OUT]), it.begin, ti_n(it), km, rank_max, my);

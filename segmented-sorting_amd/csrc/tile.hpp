@@ -9,7 +9,19 @@
 
 #include "common.hpp"
 
+#include <type_traits>
+
 namespace ssort {
+
+// Element i of an array at a wave-uniform base: the byte offset is formed in 32
+// bits, so the access is a scalar base plus a 32-bit lane offset (one VALU
+// operation for the address instead of a 64-bit add). For accesses inside one
+// range, chunk or tile (i * sizeof(T) < 2^32).
+template <typename T>
+__device__ __forceinline__ T& at32(T* base, uint32_t i) {
+    using C = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+    return *reinterpret_cast<T*>(reinterpret_cast<C*>(base) + (uint32_t)(i * (uint32_t)sizeof(T)));
+}
 
 // NT threads hold IPT items each in "wave-striped" order: item j of lane l of
 // wave w is tile position w*64*IPT + j*64 + l, which is the input order, so
@@ -273,7 +285,7 @@ struct RadixTile {
 #pragma unroll
         for (int j = 0; j < IPT; ++j) {
             const int p = (lb + j * WAVE);
-            const U raw = ks[begin + (p < n ? p : n - 1)];
+            const U raw = at32(ks + begin, (p < n ? p : n - 1));
             k[j] = p < n ? twiddle_in<U>(raw, km.kind, km.desc) : Bits<U>::ALL;
         }
     }
@@ -332,7 +344,7 @@ struct RadixTile {
 #pragma unroll
             for (int j = 0; j < IPT; ++j) {
                 const int p = (lb + j * WAVE);
-                raw[j] = ks[begin + (p < n ? p : n - 1)];
+                raw[j] = at32(ks + begin, (p < n ? p : n - 1));
             }
         }
     }
@@ -481,8 +493,8 @@ struct RadixTile {
                         }
                     }
                     const int r = s0 + (int)c;
-                    ko[begin + r] = full ? twiddle_out<U>(mk, km.kind, km.desc) : ks[begin + p];
-                    if constexpr (HAS_V) vo[begin + r] = vs[begin + p];
+                    at32(ko + begin, r) = full ? twiddle_out<U>(mk, km.kind, km.desc) : at32(ks + begin, p);
+                    if constexpr (HAS_V) at32(vo + begin, r) = at32(vs + begin, p);
                 }
             }
         }
@@ -695,7 +707,7 @@ struct RadixTile {
 #pragma unroll
             for (int j = 0; j < IPT; ++j) {
                 const int p = lb + j * WAVE;
-                v[j] = vs[begin + (p < n ? p : n - 1)];  // unconditional: the loads issue together
+                v[j] = at32(vs + begin, (p < n ? p : n - 1));  // unconditional: the loads issue together
             }
         }
         if (act) {
@@ -705,7 +717,7 @@ struct RadixTile {
                 const int pc = p < n ? p : n - 1;
                 const U kk = sm.keys[pc];
                 const uint32_t r = place[pc];
-                if (p < n) ko[begin + r] = twiddle_out<U>(kk, km.kind, km.desc);
+                if (p < n) at32(ko + begin, r) = twiddle_out<U>(kk, km.kind, km.desc);
             }
         }
         if constexpr (STAGE) {
@@ -725,7 +737,7 @@ struct RadixTile {
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
                     const int p = lb + j * WAVE;
-                    if (p < n) vo[begin + place[p]] = stage[sm.idx[0][p]];
+                    if (p < n) at32(vo + begin, place[p]) = stage[sm.idx[0][p]];
                 }
             }
         } else if constexpr (HAS_V) {
@@ -734,7 +746,7 @@ struct RadixTile {
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
                     const int p = lb + j * WAVE;
-                    if (p < n) vo[begin + place[p]] = vs[begin + sm.idx[0][p]];
+                    if (p < n) at32(vo + begin, place[p]) = at32(vs + begin, sm.idx[0][p]);
                 }
             }
         }
@@ -767,8 +779,8 @@ struct RadixTile {
                 for (int j = 0; j < IPT; ++j) {
                     const int p = (lb + j * WAVE);
                     if (p < n) {
-                        ko[begin + p] = twiddle_out<U>(k[j], km.kind, km.desc);
-                        if constexpr (HAS_V) vo[begin + p] = vs[begin + p];
+                        at32(ko + begin, p) = twiddle_out<U>(k[j], km.kind, km.desc);
+                        if constexpr (HAS_V) at32(vo + begin, p) = at32(vs + begin, p);
                     }
                 }
             }
@@ -875,8 +887,8 @@ struct RadixTile {
                                     }
                                 }
                                 const int r = s0 + (int)c;
-                                ko[begin + r] = full ? twiddle_out<U>(mk, km.kind, km.desc) : ks[begin + p];
-                                if constexpr (HAS_V) vo[begin + r] = vs[begin + p];
+                                at32(ko + begin, r) = full ? twiddle_out<U>(mk, km.kind, km.desc) : at32(ks + begin, p);
+                                if constexpr (HAS_V) at32(vo + begin, r) = at32(vs + begin, p);
                             }
                         }
                     }
@@ -935,14 +947,14 @@ struct RadixTile {
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
                     const int p = (lb + j * WAVE);
-                    v[j] = vs[begin + (p < n ? p : n - 1)];  // unconditional: the loads issue together
+                    v[j] = at32(vs + begin, (p < n ? p : n - 1));  // unconditional: the loads issue together
                 }
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
                     const int p = (lb + j * WAVE);
                     if (p < n) {
                         srcs[j] = sm.idx[cur][p];
-                        ko[begin + p] = twiddle_out<U>(sm.keys[p], km.kind, km.desc);
+                        at32(ko + begin, p) = twiddle_out<U>(sm.keys[p], km.kind, km.desc);
                     }
                 }
             }
@@ -958,7 +970,7 @@ struct RadixTile {
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
                     const int p = (lb + j * WAVE);
-                    if (p < n) vo[begin + p] = stage[srcs[j]];
+                    if (p < n) at32(vo + begin, p) = stage[srcs[j]];
                 }
             }
         } else {
@@ -967,8 +979,8 @@ struct RadixTile {
                 for (int j = 0; j < IPT; ++j) {
                     const int p = (lb + j * WAVE);
                     if (p < n) {
-                        ko[begin + p] = twiddle_out<U>(sm.keys[p], km.kind, km.desc);
-                        if constexpr (HAS_V) vo[begin + p] = vs[begin + sm.idx[cur][p]];
+                        at32(ko + begin, p) = twiddle_out<U>(sm.keys[p], km.kind, km.desc);
+                        if constexpr (HAS_V) at32(vo + begin, p) = at32(vs + begin, sm.idx[cur][p]);
                     }
                 }
             }
@@ -1023,7 +1035,7 @@ struct RadixTile {
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
                     const int p = (lb + j * WAVE);
-                    v[j] = vs[begin + (p < n ? p : n - 1)];
+                    v[j] = at32(vs + begin, (p < n ? p : n - 1));
                 }
             }
         }
@@ -1070,7 +1082,7 @@ struct RadixTile {
 #pragma unroll
             for (int j = 0; j < IPT; ++j) {
                 const int i = c0 + j * NT + t;
-                key[j] = i < n ? ks[begin + i] : U(0);
+                key[j] = i < n ? at32(ks + begin + c0, j * NT + t) : U(0);  // offsets inside one tile
             }
 #pragma unroll
             for (int j = 0; j < IPT; ++j) {
@@ -1310,8 +1322,8 @@ __device__ __forceinline__ void wave_bitonic_sort(const U* ks, const V* vs, U* k
     for (int r = 0; r < R; ++r) {
         const int p = r * WAVE + (int)l;
         const int pc = p < n ? p : n - 1;
-        const U raw = ks[begin + pc];
-        if constexpr (HAS_V) v[r] = vs[begin + pc];
+        const U raw = at32(ks + begin, pc);
+        if constexpr (HAS_V) v[r] = at32(vs + begin, pc);
         w[r] = p < n ? W::make(U(twiddle_in<U>(raw, km.kind, km.desc) & rm), (uint32_t)p) : W::inf();
     }
     bitonic_wave<R>(w);
@@ -1325,8 +1337,8 @@ __device__ __forceinline__ void wave_bitonic_sort(const U* ks, const V* vs, U* k
         const int q = (int)l * R + r;
         if (q < n) {
             const uint32_t src = w[r].pos();
-            ko[begin + q] = full ? twiddle_out<U>(U(w[r].key()), km.kind, km.desc) : ks[begin + src];
-            if constexpr (HAS_V) vo[begin + q] = stage[src];
+            at32(ko + begin, q) = full ? twiddle_out<U>(U(w[r].key()), km.kind, km.desc) : at32(ks + begin, src);
+            if constexpr (HAS_V) at32(vo + begin, q) = stage[src];
         }
     }
 }
@@ -1418,8 +1430,8 @@ __device__ bool wave_msd_sort(const U* ks, const V* vs, U* ko, V* vo, int64_t be
     for (int r = 0; r < R; ++r) {
         const int p = r * WAVE + l;
         const int pc = p < n ? p : n - 1;
-        raw[r] = ks[begin + pc];
-        if constexpr (HAS_V) v[r] = vs[begin + pc];
+        raw[r] = at32(ks + begin, pc);
+        if constexpr (HAS_V) v[r] = at32(vs + begin, pc);
         ok[r] = U(twiddle_in<U>(raw[r], km.kind, km.desc) & rm);
         if (p < n) {
             o |= ok[r];
@@ -1434,8 +1446,8 @@ __device__ bool wave_msd_sort(const U* ks, const V* vs, U* ko, V* vo, int64_t be
         for (int r = 0; r < R; ++r) {
             const int p = r * WAVE + l;
             if (p < n) {
-                ko[begin + p] = raw[r];
-                if constexpr (HAS_V) vo[begin + p] = v[r];
+                at32(ko + begin, p) = raw[r];
+                if constexpr (HAS_V) at32(vo + begin, p) = v[r];
             }
         }
         return true;
@@ -1464,8 +1476,8 @@ __device__ bool wave_msd_sort(const U* ks, const V* vs, U* ko, V* vo, int64_t be
         for (int r = 0; r < R; ++r) {
             if (r * WAVE + l < n) {
                 const uint32_t d = ws.cnt[dg[r]] + rk[r];
-                ko[begin + d] = raw[r];
-                if constexpr (HAS_V) vo[begin + d] = v[r];
+                at32(ko + begin, d) = raw[r];
+                if constexpr (HAS_V) at32(vo + begin, d) = v[r];
             }
         }
         return true;
@@ -1532,8 +1544,8 @@ __device__ bool wave_msd_sort(const U* ks, const V* vs, U* ko, V* vo, int64_t be
                         c += (kq < mk) | ((kq == mk) & ((uint32_t)ws.pos[q] < p));
                     }
                 }
-                ko[begin + s + (int)c] = full ? twiddle_out<U>(mk, km.kind, km.desc) : ks[begin + p];
-                if constexpr (HAS_V) vo[begin + s + (int)c] = vs[begin + p];
+                at32(ko + begin, s + (int)c) = full ? twiddle_out<U>(mk, km.kind, km.desc) : at32(ks + begin, p);
+                if constexpr (HAS_V) at32(vo + begin, s + (int)c) = at32(vs + begin, p);
             }
         }
         return true;
@@ -1554,8 +1566,8 @@ __device__ bool wave_msd_sort(const U* ks, const V* vs, U* ko, V* vo, int64_t be
                     c += (kq < ok[r]) | ((kq == ok[r]) & ((int)ws.pos[q] < p));
                 }
             }
-            ko[begin + s + (int)c] = raw[r];
-            if constexpr (HAS_V) vo[begin + s + (int)c] = v[r];
+            at32(ko + begin, s + (int)c) = raw[r];
+            if constexpr (HAS_V) at32(vo + begin, s + (int)c) = v[r];
         }
     }
     return true;
@@ -1568,9 +1580,9 @@ __device__ __forceinline__ void wave_rank_sort(const U* ks, const V* vs, U* ko, 
                                                const KeyMode& km) {
     const int l = (int)lane_id();
     const bool valid = l < n;
-    const U raw = valid ? ks[begin + l] : U(0);
+    const U raw = valid ? at32(ks + begin, l) : U(0);
     V val = V(0);
-    if constexpr (HAS_V) val = valid ? vs[begin + l] : V(0);
+    if constexpr (HAS_V) val = valid ? at32(vs + begin, l) : V(0);
     const U okey = U(twiddle_in<U>(raw, km.kind, km.desc) & range_mask<U>(km.begin_bit, km.end_bit));
     uint32_t rank = 0;
     for (int j = 0; j < n; ++j) {
@@ -1578,8 +1590,8 @@ __device__ __forceinline__ void wave_rank_sort(const U* ks, const V* vs, U* ko, 
         rank += (kj < okey) | ((kj == okey) & (j < l));
     }
     if (valid) {
-        ko[begin + rank] = raw;
-        if constexpr (HAS_V) vo[begin + rank] = val;
+        at32(ko + begin, rank) = raw;
+        if constexpr (HAS_V) at32(vo + begin, rank) = val;
     }
 }
 

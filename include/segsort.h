@@ -72,6 +72,25 @@ int segsort_sort(void* d_temp, size_t* temp_bytes,
                  int key_type, int value_bytes, int descending, int algo,
                  void* stream);
 
+/* segsort_sort with the batched algorithm and no temp argument, for callers
+ * that cannot allocate device memory (code under graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). No counterpart in the reference
+ * (cub's calls always take caller temp). The temp storage is a static device
+ * array of segsort_noalloc_max_bytes() bytes (64 MiB), one per GPU, held until
+ * the process ends; its address is looked up once per device on the first
+ * call (hipGetSymbolAddress). A call whose temp storage would exceed it
+ * returns hipErrorInvalidValue (segsort_last_error says how much it needs;
+ * about 2^21 u32 pairs fit). The call allocates nothing and creates no
+ * stream, event or graph: kernels are launched directly on `stream`, on that
+ * stream alone. Every call on one GPU uses the same array, which keeps the
+ * last call's contents (each call clears what must start at zero): calls that
+ * overlap on one GPU (other streams, other threads) race. Arguments as in
+ * segsort_sort. */
+int segsort_sort_noalloc(const void* d_keys_in, void* d_keys_out, const void* d_vals_in, void* d_vals_out,
+                         int num_items, int num_segments, const int* d_begin_offsets, const int* d_end_offsets,
+                         int begin_bit, int end_bit, int key_type, int value_bytes, int descending, void* stream);
+size_t segsort_noalloc_max_bytes(void);
+
 /* Typed shorthands of segsort_sort for the reference's u32/u32 drivers
  * (src/radixseg.cu:116-124 calls SortPairs<uint, uint>). */
 int segsort_sort_pairs_u32(void* d_temp, size_t* temp_bytes,

@@ -34,7 +34,8 @@ EXPORTS = (
     "segsort_mgpu_segmented_sort", "segsort_row_iota", "segsort_perm_scatter_f32", "segsort_set_tuning",
     "segsort_get_tuning", "segsort_tile_caps", "segsort_last_error", "segsort_version", "segsort_profile_enable",
     "segsort_profile_report", "segsort_set_graph_replay", "segsort_reset_tuning", "segsort_lookup_tuning",
-    "segsort_mgpu_segmented_sort_comp", "segsort_mgpu_segmented_sort_indices",
+    "segsort_mgpu_segmented_sort_comp", "segsort_mgpu_segmented_sort_indices", "segsort_sort_noalloc",
+    "segsort_noalloc_max_bytes",
 )
 
 _lib = None
@@ -57,6 +58,10 @@ def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
     lib.segsort_sort.argtypes = [vp, ctypes.POINTER(sz), vp, vp, vp, vp, i32, i32, vp, vp, i32, i32, i32, i32, i32,
                                  i32, vp]
     lib.segsort_sort.restype = i32
+    lib.segsort_sort_noalloc.argtypes = [vp, vp, vp, vp, i32, i32, vp, vp, i32, i32, i32, i32, i32, vp]
+    lib.segsort_sort_noalloc.restype = i32
+    lib.segsort_noalloc_max_bytes.argtypes = []
+    lib.segsort_noalloc_max_bytes.restype = sz
     lib.segsort_sort_pairs_u32.argtypes = [vp, ctypes.POINTER(sz), vp, vp, vp, vp, i32, i32, vp, vp, i32, vp]
     lib.segsort_sort_pairs_u32.restype = i32
     lib.segsort_sort_keys_u32.argtypes = [vp, ctypes.POINTER(sz), vp, vp, i32, i32, vp, vp, i32, vp]

@@ -22,6 +22,11 @@ namespace {
 
 thread_local std::string g_last_error = "no error";
 
+// Set while segsort_sort_noalloc runs: the call launches its kernels directly on
+// the caller's stream (no graph capture, no second stream), so that it creates
+// and allocates nothing.
+thread_local bool t_no_alloc = false;
+
 int fail(hipError_t e, const char* what) {
     g_last_error = std::string(what) + ": " + hipGetErrorString(e);
     return (int)e;
@@ -676,10 +681,17 @@ struct Launcher {
         key.vb = HAS_V ? (int)sizeof(V) : 0;
         key.tu = tu;
         key.dev = current_device();
-        hipError_t e = run_graph(key, st, [&](hipStream_t cs, bool captured) {
-            (void)captured;
-            return enqueue_batched(base, L, p, tu, levels, ki, ko, vi, vo, S, db, de, km, cs);
-        });
+        hipError_t e;
+        if (t_no_alloc) {
+            Tuning one = tu;
+            one.concurrent = 0;
+            e = enqueue_batched(base, L, p, one, levels, ki, ko, vi, vo, S, db, de, km, st);
+        } else {
+            e = run_graph(key, st, [&](hipStream_t cs, bool captured) {
+                (void)captured;
+                return enqueue_batched(base, L, p, tu, levels, ki, ko, vi, vo, S, db, de, km, cs);
+            });
+        }
         if (e != hipSuccess) return fail(e, "segsort_sort(batched): launch");
         return ok();
     }
@@ -890,6 +902,72 @@ int segsort_sort(void* d_temp, size_t* temp_bytes, const void* d_keys_in, void* 
     if (value_bytes == 4) return dispatch_algo<uint64_t, uint32_t, true>(SEGSORT_ARGS);
     return dispatch_algo<uint64_t, uint64_t, true>(SEGSORT_ARGS);
 #undef SEGSORT_ARGS
+}
+
+}  // extern "C"
+
+namespace {
+
+// Scratch of segsort_sort_noalloc: one static array per GPU (each device holds
+// its own copy for the life of the process). In host code `scratch` names a
+// host-side stand-in, not device memory: only the address that
+// hipGetSymbolAddress returns for the current device is passed to kernels.
+constexpr size_t SCRATCH_BYTES = size_t(64) << 20;
+__device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_ptr;  // device -> device address of `scratch`
+
+hipError_t scratch_address(void** out) {
+    const int dev = current_device();
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it == g_scratch_ptr.end()) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!ptr) return hipErrorInvalidSymbol;
+        it = g_scratch_ptr.emplace(dev, ptr).first;
+    }
+    *out = it->second;
+    return hipSuccess;
+}
+
+struct NoAllocScope {
+    NoAllocScope() { t_no_alloc = true; }
+    ~NoAllocScope() { t_no_alloc = false; }
+};
+
+}  // namespace
+
+extern "C" {
+
+size_t segsort_noalloc_max_bytes(void) { return SCRATCH_BYTES; }
+
+int segsort_sort_noalloc(const void* d_keys_in, void* d_keys_out, const void* d_vals_in, void* d_vals_out,
+                         int num_items, int num_segments, const int* d_begin_offsets, const int* d_end_offsets,
+                         int begin_bit, int end_bit, int key_type, int value_bytes, int descending, void* stream) {
+    NoAllocScope scope;
+    size_t need = 0;
+    int rc = segsort_sort(nullptr, &need, d_keys_in, d_keys_out, d_vals_in, d_vals_out, num_items, num_segments,
+                          d_begin_offsets, d_end_offsets, begin_bit, end_bit, key_type, value_bytes, descending,
+                          SEGSORT_ALGO_BATCHED, stream);
+    if (rc) return rc;
+    if (need > SCRATCH_BYTES) {
+        char msg[160];
+        snprintf(msg, sizeof(msg), "segsort_sort_noalloc: the call needs %zu bytes of scratch, more than the %zu of "
+                 "the static scratch array", need, SCRATCH_BYTES);
+        g_last_error = msg;
+        return (int)hipErrorInvalidValue;
+    }
+    if (num_items == 0 || num_segments == 0) return ok();
+    void* ptr = nullptr;
+    const hipError_t e = scratch_address(&ptr);
+    if (e != hipSuccess) return fail(e, "segsort_sort_noalloc: hipGetSymbolAddress(scratch)");
+    size_t tb = SCRATCH_BYTES;
+    return segsort_sort(ptr, &tb, d_keys_in, d_keys_out, d_vals_in, d_vals_out, num_items, num_segments,
+                        d_begin_offsets, d_end_offsets, begin_bit, end_bit, key_type, value_bytes, descending,
+                        SEGSORT_ALGO_BATCHED, stream);
 }
 
 int segsort_sort_pairs_u32(void* d_temp, size_t* temp_bytes, const uint32_t* d_keys_in, uint32_t* d_keys_out,

@@ -49,6 +49,7 @@ WORKLOADS = {
     "tiny_f32": "2^20 segments of U[1,32] f32 N(0,1) keys (+-0, denormals), u32 values",
     "tiny_u64": "2^20 segments of U[1,32] u64 keys, u32 values",
     "diff": "differentiable pair sort: 4096 rows of U[16,8192] f32 N(0,1), forward + backward",
+    "skew_f32": "one segment of 2^24 f32 N(0,1) keys (skewed top digit), u32 values",
 }
 
 
@@ -65,6 +66,10 @@ def make_workload(pkg, name):
         return g.tiny_segments(0x5E650A7 + 3, key="f32")
     if name == "tiny_u64":
         return g.tiny_segments(0x5E650A7 + 3, key="u64")
+    if name == "skew_f32":  # one 2^24 segment of N(0,1) f32 keys (skewed top digit), values = index
+        n = 1 << 24
+        return g.normal(0x5E650A7 + 14, 1, n).astype(np.float32), np.arange(n, dtype=np.uint32), \
+            np.array([0, n], dtype=np.int32)
     if name == "diff_rows":  # the differentiable sort's forward pass: f32 rows, row-local indices as values
         x, off = g.diff_rows(0x5E650A7 + 4)
         local = np.concatenate([np.arange(b - a, dtype=np.uint32) for a, b in zip(off[:-1], off[1:])])

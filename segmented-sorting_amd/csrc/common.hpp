@@ -205,24 +205,20 @@ __device__ __forceinline__ TileItem make_ti(int begin, int n, int src) {
     return t;
 }
 
-// A range larger than one tile, partitioned by its top varying radix digit.
-enum : int { LMODE_PARTITION = 0, LMODE_FINAL = 1, LMODE_COPY = 2 };
-
+// A range larger than one tile. In the multi-workgroup MSD levels it is cut
+// into nch chunks of chunk_len items (a multiple of the partition tile, so
+// that a range has at most a few hundred chunks), numbered from chunk_base in
+// its level's chunk arrays; the depth-first tier and the plain form's
+// per-segment LSD use only the first four fields.
 struct LargeItem {
     int begin;       // first element (same index in every buffer)
     int n;           // element count
     int src;         // buffer holding the range now
     int hi_bit;      // bits at and above hi_bit are equal within the range
     int chunk_base;  // first chunk id of this range in its level
-    int shift;       // chosen digit position      (written by the scan)
-    int bits;        // chosen digit width         (written by the scan)
-    int mode;        // LMODE_*                    (written by the scan)
-    int parent;      // index of the parent range in the previous level, -1 at level 0
-    uint32_t hist_rem;  // chunks not yet histogrammed
-    uint32_t scat_rem;  // chunks not yet scattered
-    uint32_t flags;     // LFLAG_*
+    int chunk_len;   // items per chunk (the last one may be shorter)
+    int nch;         // chunks of the range
+    int pad;
 };
-
-enum : uint32_t { LFLAG_SCANNED = 1u, LFLAG_SCATTERED = 2u };
 
 }  // namespace ssort

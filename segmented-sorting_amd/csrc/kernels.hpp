@@ -18,13 +18,18 @@ constexpr int MAX_LEVELS = 8;        // MSD levels of the large tier (64-bit key
 struct Counters {
     uint32_t tile_count[MAX_QUEUES];
     uint32_t pad0[2];
-    unsigned long long level[MAX_LEVELS + 1];  // (items << 32) | chunks, per MSD level
+    unsigned long long level[MAX_LEVELS + 1];  // (ranges << 32) | chunks, per MSD level
     unsigned long long tile_items[MAX_QUEUES];  // elements queued per queue
     unsigned long long level_items[MAX_LEVELS + 1];   // elements queued per MSD level
-    uint32_t claim;                                   // next task of the large tier
-    uint32_t scans_done[MAX_LEVELS + 1];              // ranges scanned, per MSD level
-    uint32_t error;                                   // a bounded wait gave up
+    uint32_t claim;                                   // next range of the depth-first tier
 };
+
+// MSD levels of the batched form: two multi-workgroup levels (the long
+// segments, then their children of more than MSD_BIG_TILES partition tiles),
+// then the depth-first tier for every other child larger than a tile.
+constexpr int MSD_LEVEL0 = 0, MSD_LEVEL1 = 1, MSD_LOCAL = 2;
+constexpr int MSD_BIG_TILES = 8;
+constexpr int MSD_MAX_CHUNKS = 256;  // chunks per range of a multi-workgroup level
 
 // Length caps of the classes of one call: class c holds ranges with
 // cap[c-1] < n <= cap[c] (cap[-1] = 0); n > cap[nclass-1] is large.
@@ -49,10 +54,9 @@ struct Sched {
     Counters* ctr;
     TileItem* tq[MAX_QUEUES];
     LargeItem* lq[MAX_LEVELS + 1];
-    uint32_t* hist;  // [chunk][RADIX], the histogram of the chunk's highest varying digit
-    ChunkInfo* cinfo;  // [chunk]
-    uint32_t* off;   // [chunk][RADIX]
-    int chunk;       // items per large-tier chunk
+    uint32_t* hist[2];      // per multi-workgroup level: [chunk][RADIX], the histogram of the chunk's highest varying digit
+    ChunkInfo* cinfo[2];    // per multi-workgroup level: [chunk]
+    int chunk;       // partition tile of the MSD levels (items); chunks are multiples of it
     // Capacities: a push beyond them is dropped and readers clamp their counts,
     // so overlapping (invalid) segments cannot write outside the temp storage.
     uint32_t tq_cap[MAX_QUEUES];
@@ -141,10 +145,19 @@ __device__ __forceinline__ void push_tiles_block(const Sched& sc, BlockPush& bp,
     }
 }
 
+// Queue a range for MSD level `level`. For the multi-workgroup levels
+// (chunked) the range is cut into at most MSD_MAX_CHUNKS chunks of a whole
+// number of partition tiles, which take consecutive ids of the level's chunk
+// arrays.
 __device__ __forceinline__ void push_large(const Sched& sc, int level, int begin, int n, int src, int hi_bit,
-                                           int parent = -1) {
-    const unsigned long long nch = (unsigned long long)((n + sc.chunk - 1) / sc.chunk);
-    const unsigned long long old = atomicAdd(&sc.ctr->level[level], (1ull << 32) | nch);
+                                           bool chunked) {
+    int chunk_len = 0, nch = 0;
+    if (chunked) {
+        const int tiles = (n + sc.chunk - 1) / sc.chunk;
+        chunk_len = ((tiles + MSD_MAX_CHUNKS - 1) / MSD_MAX_CHUNKS) * sc.chunk;
+        nch = (n + chunk_len - 1) / chunk_len;
+    }
+    const unsigned long long old = atomicAdd(&sc.ctr->level[level], (1ull << 32) | (unsigned long long)nch);
     atomicAdd(&sc.ctr->level_items[level], (unsigned long long)n);
     LargeItem li;
     li.begin = begin;
@@ -152,14 +165,15 @@ __device__ __forceinline__ void push_large(const Sched& sc, int level, int begin
     li.src = src;
     li.hi_bit = hi_bit;
     li.chunk_base = (int)(old & 0xFFFFFFFFull);
-    li.shift = 0;
-    li.bits = 0;
-    li.mode = 0;
-    li.parent = parent;
-    li.hist_rem = (uint32_t)nch;
-    li.scat_rem = (uint32_t)nch;
-    li.flags = 0;
-    if ((old >> 32) < sc.lq_cap && (old & 0xFFFFFFFFull) + nch <= sc.chunk_cap) sc.lq[level][old >> 32] = li;
+    li.chunk_len = chunk_len;
+    li.nch = nch;
+    li.pad = 0;
+    if ((old >> 32) < sc.lq_cap) {
+        // past the chunk capacity (only overlapping segments can get there) the
+        // record stays, empty, so that no later range maps its chunks to a stale one
+        if (chunked && (old & 0xFFFFFFFFull) + nch > sc.chunk_cap) li.n = li.nch = 0;
+        sc.lq[level][old >> 32] = li;
+    }
 }
 
 // Bin every segment by length: length 1 is copied here, the rest go to the
@@ -206,7 +220,7 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
             if constexpr (HAS_V) buf_ptr<V>(b.v[BUF_OUT])[beg[j]] = ((const V*)b.v[BUF_IN])[beg[j]];
         }
         cls[j] = n[j] >= 2 ? class_of(cc, n[j]) : -2;
-        if (cls[j] == -1 && large_to_level0) push_large(sc, 0, beg[j], n[j], BUF_IN, end_bit);
+        if (cls[j] == -1 && large_to_level0) push_large(sc, MSD_LEVEL0, beg[j], n[j], BUF_IN, end_bit, true);
     }
     // per class: the wave's slots counted with ballots in registers, then one LDS
     // atomic per class and wave for the wave's base inside the block
@@ -279,10 +293,7 @@ __global__ void k_bin_large_ids(const int* __restrict__ d_begin, const int* __re
         li.n = n;
         li.src = BUF_IN;
         li.hi_bit = 0;
-        li.chunk_base = 0;
-        li.shift = li.bits = li.mode = 0;
-        li.parent = -1;
-        li.hist_rem = li.scat_rem = li.flags = 0;
+        li.chunk_base = li.chunk_len = li.nch = li.pad = 0;
         const uint32_t slot = base + lanes_below(m);
         if (slot < sc.lq_cap) sc.lq[MAX_LEVELS][slot] = li;
     }
@@ -480,6 +491,24 @@ __device__ __forceinline__ int find_item(const LargeItem* items, int count, int 
     return lo;
 }
 
+// The range record owning chunk g of a multi-workgroup level, in scalar registers.
+__device__ __forceinline__ LargeItem chunk_owner(const LargeItem* items, int count, int g) {
+    LargeItem it = items[find_item(items, count, g)];
+    it.begin = __builtin_amdgcn_readfirstlane(it.begin);
+    it.n = __builtin_amdgcn_readfirstlane(it.n);
+    it.src = __builtin_amdgcn_readfirstlane(it.src);
+    it.hi_bit = __builtin_amdgcn_readfirstlane(it.hi_bit);
+    it.chunk_base = __builtin_amdgcn_readfirstlane(it.chunk_base);
+    it.chunk_len = __builtin_amdgcn_readfirstlane(it.chunk_len);
+    it.nch = __builtin_amdgcn_readfirstlane(it.nch);
+    return it;
+}
+
+__device__ __forceinline__ int chunk_items(const LargeItem& it, int c) {
+    const int left = it.n - c * it.chunk_len;
+    return left < it.chunk_len ? left : it.chunk_len;
+}
+
 __device__ __forceinline__ int npos_of(int hi_bit, int begin_bit) {
     return (hi_bit - begin_bit + RADIX_BITS - 1) / RADIX_BITS;
 }
@@ -489,10 +518,11 @@ __device__ __forceinline__ int npos_of(int hi_bit, int begin_bit) {
 // highest varying digit position. The level's digit is the range's highest
 // varying position, which no chunk exceeds; a chunk whose own top is lower
 // holds a single digit value there, known from its AND, so one histogram per
-// chunk is all the scan needs.
+// chunk is all the partition needs. A chunk of one tile is read once (its
+// keys stay in registers); a longer chunk is read again for the histogram.
 template <int NT, int IPT, typename U, typename V, bool HAS_V>
 __global__ __launch_bounds__(NT) void k_large_hist(Bufs b, Sched sc, int level, KeyMode km) {
-    constexpr int NW = NT / WAVE;
+    constexpr int NW = NT / WAVE, TILE = NT * IPT;
     __shared__ uint32_t hist[RADIX];
     __shared__ U red_or[NW];
     __shared__ U red_and[NW];
@@ -501,25 +531,33 @@ __global__ __launch_bounds__(NT) void k_large_hist(Bufs b, Sched sc, int level, 
     level_counts(sc, level, count, chunks);
     const LargeItem* items = sc.lq[level];
     for (int g = blockIdx.x; g < chunks; g += gridDim.x) {
-        const LargeItem it = items[find_item(items, count, g)];
+        const LargeItem it = chunk_owner(items, count, g);
         const int c = g - it.chunk_base;
-        const int64_t start = (int64_t)it.begin + (int64_t)c * sc.chunk;
-        const int n = (it.n - c * sc.chunk) < sc.chunk ? (it.n - c * sc.chunk) : sc.chunk;
+        const int len = chunk_items(it, c);
         const U rmask = range_mask<U>(km.begin_bit, it.hi_bit);
-        const U* ks = (const U*)b.k[it.src];
+        const U* ks = (const U*)b.k[it.src] + ((int64_t)it.begin + (int64_t)c * it.chunk_len);
         U key[IPT];
+        auto load_tile = [&](int t0) {
+            const int tn = (len - t0) < TILE ? (len - t0) : TILE;
 #pragma unroll
-        for (int j = 0; j < IPT; ++j) {
-            const int i = j * NT + t;
-            key[j] = i < n ? ks[start + i] : U(0);
-        }
+            for (int j = 0; j < IPT; ++j) {
+                const int i = j * NT + t;
+                key[j] = i < tn ? at32(ks + t0, i) : U(0);
+            }
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) key[j] = U(twiddle_in<U>(key[j], km.kind, km.desc) & rmask);
+            return tn;
+        };
         U o = 0, a = Bits<U>::ALL;
+        int tn = 0;
+        for (int t0 = 0; t0 < len; t0 += TILE) {
+            tn = load_tile(t0);
 #pragma unroll
-        for (int j = 0; j < IPT; ++j) {
-            if (j * NT + t < n) {
-                key[j] = U(twiddle_in<U>(key[j], km.kind, km.desc) & rmask);
-                o |= key[j];
-                a &= key[j];
+            for (int j = 0; j < IPT; ++j) {
+                if (j * NT + t < tn) {
+                    o |= key[j];
+                    a &= key[j];
+                }
             }
         }
         o = wave_or(o);
@@ -541,72 +579,87 @@ __global__ __launch_bounds__(NT) void k_large_hist(Bufs b, Sched sc, int level, 
         if (lp >= 0) {
             const int sh = km.begin_bit + lp * RADIX_BITS;
             const U dm = U((U(1) << ((it.hi_bit - sh) < RADIX_BITS ? (it.hi_bit - sh) : RADIX_BITS)) - 1);
+            for (int t0 = 0; t0 < len; t0 += TILE) {
+                if (len > TILE) tn = load_tile(t0);  // a one-tile chunk still holds its keys
 #pragma unroll
-            for (int j = 0; j < IPT; ++j) {
-                if (j * NT + (t & ~63) >= n) break;  // this and later rows: whole wave past the end
-                const bool v = j * NT + t < n;
-                const uint32_t d = (uint32_t)((key[j] >> sh) & dm);
-                const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);
-                const uint64_t act = __ballot(v);
-                if (__ballot(v && d != d0) == 0) {
-                    if ((t & 63) == 0) atomicAdd(&hist[d0], (uint32_t)__popcll(act));
-                } else if (v) {
-                    atomicAdd(&hist[d], 1u);
+                for (int j = 0; j < IPT; ++j) {
+                    if (j * NT + (t & ~63) >= tn) break;  // this and later rows: whole wave past the end
+                    const bool v = j * NT + t < tn;
+                    const uint32_t d = (uint32_t)((key[j] >> sh) & dm);
+                    const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);
+                    const uint64_t act = __ballot(v);
+                    if (__ballot(v && d != d0) == 0) {
+                        if ((t & 63) == 0) atomicAdd(&hist[d0], (uint32_t)__popcll(act));
+                    } else if (v) {
+                        atomicAdd(&hist[d], 1u);
+                    }
                 }
             }
         }
         __syncthreads();
-        if (t == 0) sc.cinfo[g] = ChunkInfo{(unsigned long long)o, (unsigned long long)a, lp, 0};
+        if (t == 0) sc.cinfo[level][g] = ChunkInfo{(unsigned long long)o, (unsigned long long)a, lp, 0};
         if (lp >= 0)
-            for (int i = t; i < RADIX; i += NT) sc.hist[(size_t)g * RADIX + i] = hist[i];
+            for (int i = t; i < RADIX; i += NT) sc.hist[level][(size_t)g * RADIX + i] = hist[i];
         __syncthreads();
     }
 }
-
-// Pass 2 of a level: one workgroup per range. The range's OR/AND (from its
-// chunks') give the highest digit position whose bits vary; its SCAN_G groups
-// of 256 threads (one thread per digit) then total that digit over a
-// contiguous share of the chunks each, write every chunk's destination
-// offsets (each group for its own chunks, after the groups before it), and
-// queue the child ranges.
-constexpr int SCAN_NT = 1024;
-constexpr int SCAN_G = SCAN_NT / RADIX;
 
 // Items of digit d in chunk gc at the level's position: the chunk's histogram
 // when that is its own top varying position, else all of its n items or none.
 // The histogram word is read whether or not it is used (it may be stale when
 // the chunk's top is lower), so that both loads of a chunk are in flight at once.
-__device__ __forceinline__ uint32_t chunk_digit_count(const Sched& sc, size_t gc, int pstar, int shift,
+__device__ __forceinline__ uint32_t chunk_digit_count(const Sched& sc, int level, size_t gc, int pstar, int shift,
                                                       unsigned long long dmask, uint32_t d, int n) {
-    const uint32_t h = sc.hist[gc * RADIX + d];
-    const ChunkInfo ci = sc.cinfo[gc];
+    const uint32_t h = sc.hist[level][gc * RADIX + d];
+    const ChunkInfo ci = sc.cinfo[level][gc];
     if (ci.lp == pstar) return h;
     return (uint32_t)((ci.kand >> shift) & dmask) == d ? (uint32_t)n : 0u;
 }
 
-__global__ __launch_bounds__(SCAN_NT) void k_large_scan(Sched sc, int level, KeyMode km, ClassCaps cc,
-                                                        int tile_max) {
-    __shared__ uint32_t part[SCAN_G][RADIX];
-    __shared__ unsigned long long red_or[SCAN_NT / WAVE];
-    __shared__ unsigned long long red_and[SCAN_NT / WAVE];
-    __shared__ BlockPush bp;
+// Pass 2 of a level: the stable partition of every chunk by the range's
+// highest varying digit. Each workgroup derives its chunk's destinations
+// itself from the range's chunk records (no separate scan launch): the range's
+// OR/AND give the digit, and NT/256 groups of 256 threads (one per digit) total
+// the digit over a share of the range's chunks each, separately for the chunks
+// before this one; the digit bases are the exclusive scan of the totals. The
+// chunk is then partitioned tile by tile into the child buffer (or straight
+// into the output when no lower digit varies), with running destinations in
+// LDS. The range's first chunk queues the children: to the tile classes, to
+// the next multi-workgroup level (children of level 0 with more than `big`
+// items), or to the depth-first tier. A range whose keys are all equal is
+// copied to the output.
+template <int NT, int IPT, typename U, typename V, bool HAS_V>
+__global__ __launch_bounds__(NT) void k_large_scatter(Bufs b, Sched sc, int level, KeyMode km, ClassCaps cc,
+                                                      int tile_max, int big) {
+    using T = RadixTile<NT, IPT, U, V, HAS_V>;
+    constexpr int NW = NT / WAVE;
+    constexpr int G = NT / RADIX;  // digit groups of the offset scan
+    static_assert(NT % RADIX == 0 && 2 * G * RADIX <= NW * RADIX, "partial totals fit the wave-count words");
+    __shared__ typename T::Smem sm;
+    __shared__ uint32_t run[RADIX];     // running destination of each digit
+    __shared__ uint32_t cstart[RADIX + 1];  // range-relative start of each digit's child
     __shared__ uint32_t wsum[RADIX / WAVE];
+    __shared__ unsigned long long red_or[NW];
+    __shared__ unsigned long long red_and[NW];
+    __shared__ BlockPush bp;
     const int t = threadIdx.x;
     const int d = t & (RADIX - 1);
-    const int g = t / RADIX;
+    const int grp = t / RADIX;
     int count, chunks;
     level_counts(sc, level, count, chunks);
-    LargeItem* items = sc.lq[level];
-    const int child_buf = 1 + (level & 1);  // BUF_TMP_A on even levels, BUF_TMP_B on odd
-    for (int i = blockIdx.x; i < count; i += gridDim.x) {
-        LargeItem it = items[i];
-        const int nch = (it.n + sc.chunk - 1) / sc.chunk;
-        const int c0 = (nch * g) / SCAN_G, c1 = (nch * (g + 1)) / SCAN_G;
-        auto chunk_n = [&](int c) { return (it.n - c * sc.chunk) < sc.chunk ? (it.n - c * sc.chunk) : sc.chunk; };
+    const LargeItem* items = sc.lq[level];
+    const int child_buf = 1 + (level & 1);  // BUF_TMP_A below level 0, BUF_TMP_B below level 1
+    for (int g = blockIdx.x; g < chunks; g += gridDim.x) {
+        const LargeItem it = chunk_owner(items, count, g);
+        const int c = g - it.chunk_base;
+        const int len = chunk_items(it, c);
+        const int64_t start = (int64_t)it.begin + (int64_t)c * it.chunk_len;
+        const U* ks = (const U*)b.k[it.src];
+        const V* vs = (const V*)b.v[it.src];
         // the range's OR / AND over its chunks
         unsigned long long o = 0, a = ~0ull;
-        for (int c = t; c < nch; c += SCAN_NT) {
-            const ChunkInfo ci = sc.cinfo[it.chunk_base + c];
+        for (int x = t; x < it.nch; x += NT) {
+            const ChunkInfo ci = sc.cinfo[level][it.chunk_base + x];
             o |= ci.kor;
             a &= ci.kand;
         }
@@ -619,15 +672,17 @@ __global__ __launch_bounds__(SCAN_NT) void k_large_scan(Sched sc, int level, Key
         __syncthreads();
         o = 0;
         a = ~0ull;
-        for (int w = 0; w < SCAN_NT / WAVE; ++w) {
+        for (int w = 0; w < NW; ++w) {
             o |= red_or[w];
             a &= red_and[w];
         }
         const unsigned long long var = o ^ a;  // already inside [begin_bit, hi_bit)
         if (var == 0) {
-            if (t == 0) {
-                it.mode = LMODE_COPY;
-                items[i] = it;
+            U* ko = buf_ptr<U>(b.k[BUF_OUT]);
+            V* vo = buf_ptr<V>(b.v[BUF_OUT]);
+            for (int i = t; i < len; i += NT) {
+                ko[start + i] = ks[start + i];
+                if constexpr (HAS_V) vo[start + i] = vs[start + i];
             }
             __syncthreads();
             continue;
@@ -637,81 +692,63 @@ __global__ __launch_bounds__(SCAN_NT) void k_large_scan(Sched sc, int level, Key
         const int bits = (it.hi_bit - shift) < RADIX_BITS ? (it.hi_bit - shift) : RADIX_BITS;
         const unsigned long long dmask = (1ull << bits) - 1;
         const bool final_ = (var & ((1ull << shift) - 1)) == 0;
-        uint32_t tot = 0;
-#pragma unroll 8
-        for (int c = c0; c < c1; ++c) {
-            const size_t gc = (size_t)(it.chunk_base + c);
-            tot += chunk_digit_count(sc, gc, pstar, shift, dmask, (uint32_t)d, chunk_n(c));
-        }
-        part[g][d] = tot;
-        __syncthreads();
-        uint32_t mine = 0;
-        for (int gg = 0; gg < SCAN_G; ++gg) mine += part[gg][d];
-        const uint32_t incl = wave_incl_scan(mine);
-        if (g == 0 && lane_id() == 63) wsum[d >> 6] = incl;
-        __syncthreads();
-        uint32_t base = incl - mine;
-        for (int ww = 0; ww < (d >> 6); ++ww) base += wsum[ww];
-        // this group's chunks start after the digit-d items of the groups before it
-        uint32_t run = (uint32_t)it.begin + base;
-        for (int gg = 0; gg < g; ++gg) run += part[gg][d];
-#pragma unroll 8
-        for (int c = c0; c < c1; ++c) {
-            const size_t gc = (size_t)(it.chunk_base + c);
-            sc.off[gc * RADIX + d] = run;
-            run += chunk_digit_count(sc, gc, pstar, shift, dmask, (uint32_t)d, chunk_n(c));
-        }
+        // digit d's total over the range and over the chunks before this one
+        uint32_t tot = 0, pre = 0;
         {
-            const int cb = it.begin + (int)base;
-            const int cn = (g == 0 && !final_) ? (int)mine : 0;
-            const int cls = (cn >= 1 && cn <= tile_max) ? class_of(cc, cn) : -1;
-            push_tiles_block(sc, bp, cls, make_ti(cb, cls >= 0 ? cn : 0, child_buf));
-            if (cn > tile_max) push_large(sc, level + 1, cb, cn, child_buf, shift);
+            const int x0 = (it.nch * grp) / G, x1 = (it.nch * (grp + 1)) / G;
+#pragma unroll 4
+            for (int x = x0; x < x1; ++x) {
+                const uint32_t k = chunk_digit_count(sc, level, (size_t)(it.chunk_base + x), pstar, shift, dmask,
+                                                     (uint32_t)d, chunk_items(it, x));
+                tot += k;
+                pre += x < c ? k : 0u;
+            }
         }
-        if (t == 0) {
-            it.shift = shift;
-            it.bits = bits;
-            it.mode = final_ ? LMODE_FINAL : LMODE_PARTITION;
-            items[i] = it;
+        uint32_t* part = sm.wcnt;  // free until the partition's first ranking pass
+        part[grp * RADIX + d] = tot;
+        part[(G + grp) * RADIX + d] = pre;
+        __syncthreads();
+        uint32_t total = 0, before = 0, incl = 0;
+        if (t < RADIX) {
+#pragma unroll
+            for (int q = 0; q < G; ++q) {
+                total += part[q * RADIX + t];
+                before += part[(G + q) * RADIX + t];
+            }
+            incl = wave_incl_scan(total);
+            if ((t & 63) == 63) wsum[t >> 6] = incl;
         }
         __syncthreads();
-    }
-}
-
-// Pass 3 of a level: stable scatter of every chunk by the chosen digit into
-// the child buffer (or straight into the output when the digit is the last
-// varying one); ranges whose keys are all equal are copied to the output.
-template <int NT, int IPT, typename U, typename V, bool HAS_V>
-__global__ __launch_bounds__(NT) void k_large_scatter(Bufs b, Sched sc, int level, KeyMode km) {
-    using T = RadixTile<NT, IPT, U, V, HAS_V>;
-    __shared__ typename T::Smem sm;
-    int count, chunks;
-    level_counts(sc, level, count, chunks);
-    const LargeItem* items = sc.lq[level];
-    const int child_buf = 1 + (level & 1);
-    for (int g = blockIdx.x; g < chunks; g += gridDim.x) {
-        const LargeItem it = items[find_item(items, count, g)];
-        const int c = g - it.chunk_base;
-        const int64_t start = (int64_t)it.begin + (int64_t)c * sc.chunk;
-        const int n = (it.n - c * sc.chunk) < sc.chunk ? (it.n - c * sc.chunk) : sc.chunk;
-        const U* ks = (const U*)b.k[it.src];
-        const V* vs = (const V*)b.v[it.src];
-        if (it.mode == LMODE_COPY) {
-            U* ko = buf_ptr<U>(b.k[BUF_OUT]);
-            V* vo = buf_ptr<V>(b.v[BUF_OUT]);
-            for (int i = threadIdx.x; i < n; i += NT) {
-                ko[start + i] = ks[start + i];
-                if constexpr (HAS_V) vo[start + i] = vs[start + i];
-            }
-            continue;
+        if (t < RADIX) {
+            uint32_t base = incl - total;
+            for (int w = 0; w < (t >> 6); ++w) base += wsum[w];
+            cstart[t] = base;
+            if (t == RADIX - 1) cstart[RADIX] = base + total;
+            run[t] = (uint32_t)it.begin + base + before;
         }
-        const int dbuf = it.mode == LMODE_FINAL ? BUF_OUT : child_buf;
-        T::partition(sm, ks, vs, buf_ptr<U>(b.k[dbuf]), buf_ptr<V>(b.v[dbuf]), start, n, km, it.shift, it.bits,
-                     sc.off + (size_t)g * RADIX);
+        __syncthreads();
+        if (c == 0 && !final_) {
+            const uint32_t cb = t < RADIX ? cstart[t] : 0u;
+            const int cn = t < RADIX ? (int)(cstart[t + 1] - cb) : 0;
+            const int cls = (cn >= 1 && cn <= tile_max) ? class_of(cc, cn) : -1;
+            push_tiles_block(sc, bp, cls, make_ti(it.begin + (int)cb, cls >= 0 ? cn : 0, child_buf));
+            if (cn > tile_max) {
+                if (level == MSD_LEVEL0 && cn > big)
+                    push_large(sc, MSD_LEVEL1, it.begin + (int)cb, cn, child_buf, shift, true);
+                else
+                    push_large(sc, MSD_LOCAL, it.begin + (int)cb, cn, child_buf, shift, false);
+            }
+        }
+        const int dbuf = final_ ? BUF_OUT : child_buf;
+        for (int t0 = 0; t0 < len; t0 += T::TILE) {
+            const int tn = (len - t0) < T::TILE ? (len - t0) : T::TILE;
+            T::partition_run(sm, ks, vs, buf_ptr<U>(b.k[dbuf]), buf_ptr<V>(b.v[dbuf]), start + t0, tn, km, shift,
+                             bits, run);
+        }
     }
 }
 
-// Levels below the first: one workgroup takes a range that the level-0 scan
+// The depth-first tier: one workgroup takes a range that a multi-workgroup level
 // queued and partitions it, and every descendant still larger than a tile,
 // itself, depth first, so that no workgroup ever waits for another. Each
 // partition histograms the range for every digit position below its hi_bit,

@@ -39,6 +39,11 @@ int ok() {
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
+// Workgroups of the second multi-workgroup MSD level: it has work only for a
+// child of a long segment that holds more than MSD_BIG_TILES tiles (skewed
+// keys), and a small grid keeps its launches short when it has none.
+constexpr int MSD_LEVEL1_GRID = 128;
+
 // Segments per binning thread: one while that keeps >= 64 blocks, at most BIN_SPT.
 int bin_spt(int S) { return std::min(BIN_SPT, std::max(1, (S + BIN_NT * 64 - 1) / (BIN_NT * 64))); }
 int bin_grid(int S) { return (S + BIN_NT * bin_spt(S) - 1) / (BIN_NT * bin_spt(S)); }
@@ -388,7 +393,7 @@ Plan make_plan(const Tuning& tu) {
 }
 
 struct Layout {
-    size_t ctr, tq[MAX_TILE_CLASSES], lq[MAX_LEVELS + 1], hist, cinfo, off, tk[2], tv[2];
+    size_t ctr, tq[MAX_TILE_CLASSES], lq[MAX_LEVELS + 1], hist[2], cinfo[2], tk[2], tv[2];
     size_t total;
     size_t tq_cap[MAX_TILE_CLASSES];
     size_t lq_cap;
@@ -421,14 +426,18 @@ Layout make_layout(const Plan& p, int num_items, int num_segments, int levels, b
     if (plain) {
         L.lq_cap = std::max<size_t>(std::min(S, L.lq_cap), 1);
         L.lq[MAX_LEVELS] = take(L.lq_cap * sizeof(LargeItem));
-    } else {
-        if (levels > 0) {
-            L.lq[0] = take(L.lq_cap * sizeof(LargeItem));  // multi-workgroup level 0
-            L.lq[1] = take(L.lq_cap * sizeof(LargeItem));  // ranges for the depth-first tier
+    } else if (levels > 0) {
+        // the two multi-workgroup levels and the depth-first tier; a level's
+        // ranges are disjoint, so its chunks (a whole number of partition tiles
+        // each, at most MSD_MAX_CHUNKS per range) number at most N / CHUNK plus one
+        // per range
+        L.lq[MSD_LEVEL0] = take(L.lq_cap * sizeof(LargeItem));
+        L.lq[MSD_LEVEL1] = take(L.lq_cap * sizeof(LargeItem));
+        L.lq[MSD_LOCAL] = take(L.lq_cap * sizeof(LargeItem));
+        for (int l = 0; l < 2; ++l) {
+            L.hist[l] = take(L.chunks * RADIX * sizeof(uint32_t));
+            L.cinfo[l] = take(L.chunks * sizeof(ChunkInfo));
         }
-        L.hist = take(L.chunks * RADIX * sizeof(uint32_t));
-        L.cinfo = take(L.chunks * sizeof(ChunkInfo));
-        L.off = take(L.chunks * RADIX * sizeof(uint32_t));
     }
     const bool need_tmp = plain ? (N > (size_t)p.tile_max) : (levels > 0);
     for (int i = 0; i < 2; ++i) {
@@ -550,9 +559,10 @@ struct Launcher {
             sc.lq[MAX_LEVELS] = (LargeItem*)(base + L.lq[MAX_LEVELS]);
         } else {
             for (int l = 0; l < MAX_LEVELS; ++l) sc.lq[l] = L.lq[l] ? (LargeItem*)(base + L.lq[l]) : nullptr;
-            sc.hist = (uint32_t*)(base + L.hist);
-            sc.cinfo = (ChunkInfo*)(base + L.cinfo);
-            sc.off = (uint32_t*)(base + L.off);
+            for (int l = 0; l < 2; ++l) {
+                sc.hist[l] = L.hist[l] ? (uint32_t*)(base + L.hist[l]) : nullptr;
+                sc.cinfo[l] = L.cinfo[l] ? (ChunkInfo*)(base + L.cinfo[l]) : nullptr;
+            }
         }
         sc.chunk = C::CHUNK;
         for (int c = 0; c < p.nclass; ++c) sc.tq_cap[c] = (uint32_t)std::min<size_t>(L.tq_cap[c], 0xFFFFFFFFu);
@@ -583,11 +593,21 @@ struct Launcher {
         return b;
     }
 
-    // 1 when a segment may exceed the largest tile: then the multi-workgroup
-    // level 0 and the depth-first tier below it are launched.
-    static int levels_for(const KeyMode& km, int num_items, const Plan& p) {
+    // MSD levels a call launches: 0 when no segment can exceed the largest tile,
+    // 1 when one can (multi-workgroup level 0 and the depth-first tier), 2 when
+    // in addition the mean segment length is at least a quarter of MSD_BIG_TILES
+    // partition tiles, so that a child of level 0 holding more than MSD_BIG_TILES
+    // tiles (keys skewed in their top digit) is plausible. Below that mean such a
+    // child is still sorted, by the depth-first tier, and calls whose long
+    // segments are a small share (the ragged mixes) do not pay for two launches
+    // that would find nothing to do (measured: 2^24 items in 2^16 lognormal
+    // segments 0.203 -> 0.219 ms with them).
+    static int levels_for(const KeyMode& km, int num_items, int num_segments, const Plan& p) {
         (void)km;
-        return num_items > p.tile_max ? 1 : 0;
+        if (num_items <= p.tile_max) return 0;
+        const long long big = (long long)MSD_BIG_TILES * C::CHUNK;
+        const long long s = num_segments > 0 ? num_segments : 1;
+        return (num_items > big && 4ll * num_items >= big * s) ? 2 : 1;
     }
 
     // Launch the batched sequence (bin, MSD levels, tile tiers) on st.
@@ -638,18 +658,34 @@ struct Launcher {
             if ((e = hipEventRecord(ss->join, ss->s)) != hipSuccess) return e;
         }
         if (levels > 0) {
+            // level 0 over the long segments, level 1 over their children of more
+            // than MSD_BIG_TILES tiles (launched only when such a child can exist;
+            // a smaller grid, most calls find it empty), then the depth-first tier
             auto kh = k_large_hist<C::NTC, C::IPTC, U, V, HAS_V>;
             auto ks = k_large_scatter<C::NTC, C::IPTC, U, V, HAS_V>;
             auto kl = k_large_local<C::NT3, C::IPT3, U, V, HAS_V>;
-            const int gh = resident_blocks(kh, C::NTC) * 2;
-            const int gs = resident_blocks(ks, C::NTC) * tu.grid_mult;
+            // children of level 0 above `big` items go to level 1, which runs only
+            // when levels > 1; otherwise every child larger than a tile goes to the
+            // depth-first tier
+            const int big = levels > 1 ? MSD_BIG_TILES * C::CHUNK : 0x7FFFFFFF;
+            // one workgroup per resident slot (measured: twice or three times as
+            // many, mostly finding no chunk, only wait for CUs under the overlap)
+            const int gh = resident_blocks(kh, C::NTC);
+            const int gs = resident_blocks(ks, C::NTC);
             const int gl = resident_blocks(kl, C::NT3);
-            SEGSORT_LAUNCH("large_hist", st, 100, (double)sizeof(U), 0.0, (kh<<<gh, C::NTC, 0, st>>>(b, sc, 0, km)));
-            SEGSORT_LAUNCH("large_scan", st, -1, 0.0, 0.0,
-                           (k_large_scan<<<device_cus(), SCAN_NT, 0, st>>>(sc, 0, km, p.cc, p.tile_max)));
-            SEGSORT_LAUNCH("large_scatter", st, 100, PAIR_BYTES, 0.0, (ks<<<gs, C::NTC, 0, st>>>(b, sc, 0, km)));
-            SEGSORT_LAUNCH("large_local", st, 101, PAIR_BYTES + sizeof(U), 0.0,
-                           (kl<<<gl, C::NT3, 0, st>>>(b, sc, 1, km, p.cc, p.tile_max)));
+            SEGSORT_LAUNCH("large_hist", st, 100 + MSD_LEVEL0, (double)sizeof(U), 0.0,
+                           (kh<<<gh, C::NTC, 0, st>>>(b, sc, MSD_LEVEL0, km)));
+            SEGSORT_LAUNCH("large_scatter", st, 100 + MSD_LEVEL0, PAIR_BYTES, 0.0,
+                           (ks<<<gs, C::NTC, 0, st>>>(b, sc, MSD_LEVEL0, km, p.cc, p.tile_max, big)));
+            if (levels > 1) {
+                const int g1 = std::min(gs, MSD_LEVEL1_GRID);
+                SEGSORT_LAUNCH("large_hist", st, 100 + MSD_LEVEL1, (double)sizeof(U), 0.0,
+                               (kh<<<g1, C::NTC, 0, st>>>(b, sc, MSD_LEVEL1, km)));
+                SEGSORT_LAUNCH("large_scatter", st, 100 + MSD_LEVEL1, PAIR_BYTES, 0.0,
+                               (ks<<<g1, C::NTC, 0, st>>>(b, sc, MSD_LEVEL1, km, p.cc, p.tile_max, big)));
+            }
+            SEGSORT_LAUNCH("large_local", st, 100 + MSD_LOCAL, PAIR_BYTES + sizeof(U), 0.0,
+                           (kl<<<gl, C::NT3, 0, st>>>(b, sc, MSD_LOCAL, km, p.cc, p.tile_max)));
         }
         launch_classes(p, b, sc, sc.child_q, km, st, tu.grid_mult);
         if (split && (e = hipStreamWaitEvent(st, ss->join, 0)) != hipSuccess) return e;
@@ -661,7 +697,7 @@ struct Launcher {
                        int S, const int* db, const int* de, KeyMode km, hipStream_t st) {
         const Tuning tu = choose_tuning(sizeof(U), HAS_V ? sizeof(V) : 0, N, S);
         const Plan p = make_plan<C>(tu);
-        const int levels = levels_for(km, N, p);
+        const int levels = levels_for(km, N, S, p);
         const Layout L = make_layout<C, U, V, HAS_V>(p, N, S, levels, false);
         if (!d_temp) {
             *temp_bytes = L.total;

@@ -990,18 +990,11 @@ struct RadixTile {
 
     // Stable partition of the n <= TILE items at ks[begin..) by digit
     // (twiddled >> shift) & (2^bits - 1). Item with digit d and rank r among
-    // the chunk's digit-d items goes to global index goff[d] + r. The chunk is
+    // the chunk's digit-d items goes to global index run[d] + r. The chunk is
     // ordered by digit in LDS first, so each digit run is written contiguously.
-    // goff comes from global memory (one offset table per chunk).
-    __device__ static void partition(Smem& sm, const U* ks, const V* vs, U* kd, V* vd, int64_t begin, int n,
-                                     const KeyMode& km, int shift, int bits, const uint32_t* goff_global) {
-        for (int d = threadIdx.x; d < RADIX; d += NT) sm.goff[d] = goff_global[d];
-        partition_body(sm, ks, vs, kd, vd, begin, n, km, shift, bits, sm.goff);  // barriers before goff is read
-    }
-
-    // The same with destinations from an LDS table run[] that the workgroup
-    // keeps across the chunks of one range: afterwards run[d] has advanced by
-    // the chunk's count of digit d.
+    // The destinations come from an LDS table run[] that the workgroup keeps
+    // across the tiles of one range: afterwards run[d] has advanced by the
+    // tile's count of digit d.
     __device__ static void partition_run(Smem& sm, const U* ks, const V* vs, U* kd, V* vd, int64_t begin, int n,
                                          const KeyMode& km, int shift, int bits, uint32_t* run) {
         partition_body(sm, ks, vs, kd, vd, begin, n, km, shift, bits, run);

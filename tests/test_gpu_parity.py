@@ -330,6 +330,52 @@ def test_mgpu_comparator_and_indices(pkg, torch_cuda, orc):
                                                 None) != 0
 
 
+def _skewed_u32(rng, n, share, prefix_bits):
+    """n u32 keys of which a `share` fraction have the same top `prefix_bits` bits."""
+    keys = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+    hit = rng.random(n) < share
+    low = np.uint32((1 << (32 - prefix_bits)) - 1)
+    keys[hit] = (np.uint32(0x5A5A5A5A) & ~low) | (keys[hit] & low)
+    return keys
+
+
+@pytest.mark.parametrize("case", ["normal_f32_2e24", "top_byte_shared_2e24"])
+def test_skewed_single_segment_2e24(pkg, torch_cuda, orc, case):
+    """One 2^24-item segment with skewed keys: N(0,1) f32 keys, whose top byte puts
+    about 29% of the items in one bucket of the first MSD level (that child then takes
+    the second multi-workgroup level), and u32 keys that all share their top byte; batched
+    and plain forms against the oracle."""
+    n = 1 << 24
+    if case == "normal_f32_2e24":
+        keys = pkg.gen.normal(0x5E650A7 + 14, 1, n).astype(np.float32)
+    else:
+        keys = (pkg.gen.keys_u32(0x5E650A7 + 15, 1, n, 24) | np.uint32(0x7F000000)).astype(np.uint32)
+    vals = np.arange(n, dtype=np.uint32)
+    check(pkg, torch_cuda, orc, keys, vals, np.array([0, n]), algos=("batched", "plain"))
+
+
+def test_second_msd_level_paths(pkg, torch_cuda, orc):
+    """Children of the first multi-workgroup level with more than eight partition tiles
+    go through the second level: skewed u32 and u64 keys (most items sharing their top
+    bits, some sharing 16 more, so the second level's children still need the
+    depth-first tier), keys only and pairs, ascending and descending, bit sub-ranges,
+    beside uniform segments in the same call."""
+    rng = np.random.default_rng(91)
+    segs = [_skewed_u32(rng, 600000, 0.7, 8), _skewed_u32(rng, 400000, 0.9, 16),
+            rng.integers(0, 1 << 32, 300000, dtype=np.uint64).astype(np.uint32), _skewed_u32(rng, 3000, 0.9, 8)]
+    lengths = np.array([x.size for x in segs])
+    keys = np.concatenate(segs)
+    off = pkg.gen.offsets_from_lengths(lengths)
+    n = keys.size
+    vals = np.arange(n, dtype=np.uint32)
+    for desc in (False, True):
+        check(pkg, torch_cuda, orc, keys, vals, off, descending=desc, algos=("batched",))
+    check(pkg, torch_cuda, orc, keys, None, off, algos=("batched",))
+    check(pkg, torch_cuda, orc, keys, vals, off, begin_bit=4, end_bit=28, algos=("batched",))
+    k64 = (keys.astype(np.uint64) << np.uint64(32)) | rng.integers(0, 1 << 32, n, dtype=np.uint64)
+    check(pkg, torch_cuda, orc, k64, vals, off, algos=("batched",))
+
+
 def _runs_segment(rng, n, run_len, head_uniform):
     """n u32 keys whose top 16 bits come in runs of about run_len items (low 16 bits
     random), optionally preceded by 64 keys with uniformly random top bits."""

@@ -62,6 +62,8 @@ def make_workload(pkg, name):
     if name.startswith("equal"):
         k = int(name.split(":")[1])
         return g.equal_segments(1 << 24, 1 << k, seed=0x5E650A7 + 1)
+    if name == "keys_2e20":  # keys only: 2^20 u32 in [0, 2^31), 1024 equal segments
+        return g.equal_segments(1 << 20, 1024, seed=0x5E650A7 + 0, key_bits=31, pairs=False)
     if name == "tiny_f32":
         return g.tiny_segments(0x5E650A7 + 3, key="f32")
     if name == "tiny_u64":
@@ -86,9 +88,11 @@ class Runner:
         self.pkg, self.torch = pkg, torch
         self.n, self.s = keys.size, off.size - 1
         self.kt = {np.dtype(np.uint32): 0, np.dtype(np.float32): 2, np.dtype(np.uint64): 3}[keys.dtype]
-        self.vb = vals.dtype.itemsize
-        self.kin, self.vin = to_dev(torch, keys), to_dev(torch, vals)
-        self.kout, self.vout = torch.empty_like(self.kin), torch.empty_like(self.vin)
+        self.vb = 0 if vals is None else vals.dtype.itemsize
+        self.kin = to_dev(torch, keys)
+        self.vin = to_dev(torch, vals) if vals is not None else None
+        self.kout = torch.empty_like(self.kin)
+        self.vout = torch.empty_like(self.vin) if vals is not None else None
         self.off = to_dev(torch, off.astype(np.int32))
         self.begin = self.off.data_ptr()
         self.end = self.off.data_ptr() + 4
@@ -236,8 +240,10 @@ def read_traffic(workload, kernel):
         return None
 
 
-def bench_diff(pkg, torch, steps, warmup):
-    """Forward + backward of the differentiable pair sort per step (4096 rows)."""
+def bench_diff(pkg, torch, steps, warmup, cpu=False):
+    """Forward + backward of the differentiable pair sort per step (4096 rows); with
+    cpu, a per-row stable torch.sort on the host (forward) and the gradient scatter
+    through its indices (backward) beside it."""
     x, off = pkg.gen.diff_rows(0x5E650A7 + 4)
     pkg.autotune.apply(pkg, x.size, off.size - 1, 4, 4)  # the forward's pair sort: f32 keys, int32 indices
     xd = torch.from_numpy(x).cuda().requires_grad_(True)
@@ -260,9 +266,23 @@ def bench_diff(pkg, torch, steps, warmup):
     ms = e0.elapsed_time(e1) / steps
     n = x.size
     algo_bytes = 2 * 12 * n  # forward: read x, write y + perm; backward: read g + perm, write grad
-    return {"workload": WORKLOADS["diff"], "n": n, "rows": off.size - 1, "ms_fwd_bwd": round(ms, 4),
-            "gkeys_s": round(n / (ms * 1e-3) / 1e9, 3),
-            "hbm_frac": round(algo_bytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
+    row = {"workload": WORKLOADS["diff"], "n": n, "rows": off.size - 1, "ms_fwd_bwd": round(ms, 4),
+           "gkeys_s": round(n / (ms * 1e-3) / 1e9, 3), "algorithmic_bytes": algo_bytes,
+           "hbm_frac": round(algo_bytes / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
+    if cpu:
+        xc = torch.from_numpy(x)
+        gc = g.cpu()
+        gin = torch.empty_like(xc)
+        threads = torch.get_num_threads()
+        t0 = time.perf_counter()
+        for a, b in zip(off[:-1].tolist(), off[1:].tolist()):
+            _, idx = torch.sort(xc[a:b], stable=True)
+            gin[a:b][idx] = gc[a:b]
+        t = time.perf_counter() - t0
+        row["cpu_baseline"] = {"value": round(n / t / 1e9, 4), "unit": "Gkeys/s", "cores": threads,
+                               "kind": "port", "sample": "all 4096 rows, 1 run: torch.sort(stable) per row on the "
+                                                         "host, gradient scattered through its indices"}
+    return row
 
 
 def sweep(pkg, torch, args):
@@ -291,6 +311,122 @@ def sweep(pkg, torch, args):
     return rows
 
 
+def algo_bytes_of(keys, vals, s):
+    """One read and one write of every key and value plus the begin/end offsets."""
+    return keys.size * 2 * (keys.dtype.itemsize + (0 if vals is None else vals.dtype.itemsize)) + 8 * s
+
+
+# The configurations of the reference's headline benchmark, each timed in every form
+# that applies: name -> (workload, description)
+CONFIGS = [
+    ("C1", "keys_2e20", "keys only, 2^20 u32 in [0,2^31), 1024 equal segments"),
+] + [(f"C2 k={k}", f"equal:{k}", f"2^24 u32 pairs in 2^{k} equal segments, values = index") for k in range(0, 21, 2)] + [
+    ("C3 uniform", "ragged", WORKLOADS["ragged"]),
+    ("C3 dup", "ragged_dup", WORKLOADS["ragged_dup"]),
+    ("C4a", "tiny_f32", WORKLOADS["tiny_f32"]),
+    ("C4b", "tiny_u64", WORKLOADS["tiny_u64"]),
+    ("skew", "skew_f32", WORKLOADS["skew_f32"]),
+]
+
+
+def time_config(pkg, torch, orc, workload, steps, warmup, cpu=True):
+    """ms, Gkeys/s and the end-to-end bytes-moved roofline of every form on one workload,
+    with the oracle's per-segment stable sort (all host threads, one run) beside it."""
+    keys, vals, off = make_workload(pkg, workload)
+    n, s = keys.size, off.size - 1
+    ab = algo_bytes_of(keys, vals, s)
+    algos = ["batched", "plain"] + (["fixpass"] if keys.dtype.itemsize == 4 else [])
+    row = {"n": n, "segments": s, "algorithmic_bytes": ab,
+           "tuning": pkg.lookup_tuning(n, s, keys.dtype.itemsize, 0 if vals is None else vals.dtype.itemsize),
+           "forms": {}}
+    for a in algos:
+        r = Runner(pkg, torch, keys, vals, off, a)
+        ms, _ = r.time(steps, warmup)
+        row["forms"][a] = {"ms": round(ms, 4), "gkeys_s": round(n / (ms * 1e-3) / 1e9, 3),
+                           "hbm_frac": round(ab / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+                           "sorted": sorted_within_segments(torch, r, keys.dtype)}
+        del r
+        torch.cuda.empty_cache()
+    row["batched_over_plain"] = round(row["forms"]["plain"]["ms"] / row["forms"]["batched"]["ms"], 3)
+    if "fixpass" in row["forms"]:
+        row["batched_over_fixpass"] = round(row["forms"]["fixpass"]["ms"] / row["forms"]["batched"]["ms"], 3)
+    if cpu:
+        threads = orc.max_threads()
+        t = orc.time_seg_sort(keys, vals, off, threads)
+        row["cpu_baseline"] = {"value": round(n / t / 1e9, 4), "unit": "Gkeys/s", "cores": threads, "kind": "port",
+                               "sample": "full workload, 1 run, std::stable_sort per segment, OpenMP over segments"}
+    return row
+
+
+def config_suite(pkg, torch, steps, warmup, cpu=True, done=None):
+    """Every configuration of CONFIGS (C3 uniform is reused from `done` when given)."""
+    orc = None
+    if cpu:
+        sys.path.insert(0, str(ROOT))
+        from oracle import oracle as orc  # CPU baseline only
+    out = {}
+    for name, wl, desc in CONFIGS:
+        if done is not None and wl in done:
+            out[name] = dict(done[wl], workload=desc)
+            continue
+        out[name] = dict(time_config(pkg, torch, orc, wl, steps, warmup, cpu), workload=desc)
+        f = out[name]["forms"]
+        print(f"[bench] {name}: " + ", ".join(f"{a} {v['ms']:.3f} ms" for a, v in f.items()), file=sys.stderr,
+              flush=True)
+    return out
+
+
+def sweep_ns(pkg, torch, steps, warmup):
+    """The reference's N x S sweep (scripts/genequalseg.sh: N = 2^15 .. 2^27, S = 1, 2, 4, ...
+    while S < N and S <= 2^20): batched SortPairs of N u32 keys (uniform, drawn on the
+    device) with values = index in S equal segments; one JSON line per (N, S)."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0x5E650A7)
+    rows = []
+    for ln in range(15, 28):
+        n = 1 << ln
+        kin = torch.randint(0, 1 << 32, (n,), dtype=torch.int64, device="cuda", generator=g).to(torch.int32)
+        vin = torch.arange(n, dtype=torch.int32, device="cuda")
+        kout, vout = torch.empty_like(kin), torch.empty_like(vin)
+        s = 1
+        while s < n and s <= (1 << 20):
+            off = (torch.arange(s + 1, dtype=torch.int64, device="cuda") * (n // s)).to(torch.int32)
+            tb = [0]
+            D = pkg.DeviceSegmentedRadixSort
+            assert D._call(None, tb, None, None, None, None, n, s, None, None, 0, 32, None, 0, 0, 4, "batched") == 0
+            temp = torch.empty(tb[0], dtype=torch.uint8, device="cuda")
+            st = torch.cuda.current_stream().cuda_stream
+
+            def step():
+                rc = D._call(temp, [temp.numel()], kin, kout, vin, vout, n, s, off.data_ptr(), off.data_ptr() + 4, 0,
+                             32, st, 0, 0, 4, "batched")
+                if rc:
+                    raise RuntimeError(pkg.last_error())
+
+            for _ in range(warmup):
+                step()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(steps):
+                step()
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / steps
+            k = kout.to(torch.int64) & 0xFFFFFFFF
+            seg = torch.arange(n, device="cuda") // (n // s)
+            ok = bool((((k[1:] >= k[:-1]) | (seg[1:] != seg[:-1]))).all().item())
+            row = {"n": n, "segments": s, "ms": round(ms, 4), "gkeys_s": round(n / (ms * 1e-3) / 1e9, 3),
+                   "hbm_frac": round((16 * n + 8 * s) / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4), "sorted": ok}
+            print(json.dumps(row), flush=True)
+            rows.append(row)
+            del temp
+            s *= 2
+        del kin, vin, kout, vout
+        torch.cuda.empty_cache()
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -307,6 +443,10 @@ def main():
                          "run to HBM bytes per launch, store them for --workload in profiles/traffic.json (no GPU)")
     ap.add_argument("--sweep", action="store_true",
                     help="equal-segment sweep: 2^24 u32 pairs in 2^k segments, k = 0,2,...,20, every form")
+    ap.add_argument("--sweep-ns", action="store_true",
+                    help="N x S sweep of the batched form: N = 2^15..2^27 u32 pairs in S = 1, 2, 4, ... equal segments")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="headline workload only (skip the other configurations of the benchmark)")
     args = ap.parse_args()
     if args.traffic_from:
         t = traffic_from_pmc(*args.traffic_from, args.copy_calibration or 256)
@@ -327,6 +467,9 @@ def main():
     pkg.load_library()
     if args.sweep:
         sweep(pkg, torch, args)
+        return
+    if args.sweep_ns:
+        sweep_ns(pkg, torch, args.steps, args.warmup)
         return
     if args.workload == "diff":
         print(json.dumps(bench_diff(pkg, torch, args.steps, args.warmup)), flush=True)
@@ -432,6 +575,21 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(keys, vals, off)
         out["cpu_baseline"]["value"] = round(out["cpu_baseline"]["value"], 4)
+    if not args.no_configs and args.workload == "ragged":
+        # every configuration of the benchmark, each form, steps capped for run time
+        head = {"n": n, "segments": s, "algorithmic_bytes": algo_bytes, "tuning": tuning,
+                "forms": {a: {"ms": round(v["ms"], 4), "gkeys_s": round(v["gkeys_s"], 3),
+                              "hbm_frac": round(v["hbm_frac"], 4), "sorted": v["sorted"]} for a, v in res.items()},
+                "batched_over_plain": out["batched_over_plain"]}
+        if "fixpass" in res:
+            head["batched_over_fixpass"] = round(res["fixpass"]["ms"] / ms, 3)
+        if "cpu_baseline" in out:
+            head["cpu_baseline"] = {k: out["cpu_baseline"][k] for k in ("value", "unit", "cores", "kind", "sample")}
+        del runners, rb
+        torch.cuda.empty_cache()
+        csteps, cwarm = min(args.steps, 10), min(args.warmup, 3)
+        out["configs"] = config_suite(pkg, torch, csteps, cwarm, cpu=not args.no_cpu, done={"ragged": head})
+        out["configs"]["C5"] = bench_diff(pkg, torch, csteps, cwarm, cpu=not args.no_cpu)
     print(json.dumps(out), flush=True)
 
 

@@ -129,6 +129,47 @@ class Runner:
         return e0.elapsed_time(e1) / steps, wall / steps * 1e3
 
 
+class MultiRunner(Runner):
+    """The per-segment multi-stream baseline (segsort_multistream_sort): one device-wide
+    sort per segment, dealt over 4 streams to the least-loaded one (the reference's
+    ncub.cu / newthrust.cu loops)."""
+
+    STREAMS, SCHEDULE = 4, 1
+
+    def __init__(self, pkg, torch, keys, vals, off, algo="multistream"):
+        import ctypes
+        self.pkg, self.torch = pkg, torch
+        self.n, self.s = keys.size, off.size - 1
+        self.kt = {np.dtype(np.uint32): 0, np.dtype(np.float32): 2, np.dtype(np.uint64): 3}[keys.dtype]
+        self.vb = 0 if vals is None else vals.dtype.itemsize
+        self.kin = to_dev(torch, keys)
+        self.vin = to_dev(torch, vals) if vals is not None else None
+        self.kout = torch.empty_like(self.kin)
+        self.vout = torch.empty_like(self.vin) if vals is not None else None
+        self.off = to_dev(torch, off.astype(np.int32))
+        self.h_off = np.ascontiguousarray(off, dtype=np.int32)
+        self.algo = algo
+        self.eb = keys.dtype.itemsize * 8
+        self.lib = pkg.load_library()
+        self.tb = ctypes.c_size_t(0)
+        rc = self.lib.segsort_multistream_sort(None, ctypes.byref(self.tb), None, None, None, None, *self._args())
+        assert rc == 0, pkg.last_error()
+        self.temp = torch.empty(self.tb.value, dtype=torch.uint8, device="cuda")
+
+    def _args(self):
+        st = self.torch.cuda.current_stream().cuda_stream
+        return (self.n, self.s, self.h_off.ctypes.data, 0, self.eb, self.kt, self.vb, 0, self.STREAMS, self.SCHEDULE,
+                st)
+
+    def step(self):
+        import ctypes
+        rc = self.lib.segsort_multistream_sort(self.temp.data_ptr(), ctypes.byref(self.tb), self.kin.data_ptr(),
+                                               self.kout.data_ptr(), None if self.vin is None else self.vin.data_ptr(),
+                                               None if self.vout is None else self.vout.data_ptr(), *self._args())
+        if rc != 0:
+            raise RuntimeError(f"multistream: hipError {rc}: {self.pkg.last_error()}")
+
+
 def sorted_within_segments(torch, r, key_dtype):
     """Cheap output property check on the GPU: keys non-decreasing inside every segment."""
     if key_dtype != np.uint32:
@@ -329,6 +370,9 @@ CONFIGS = [
 ]
 
 
+MULTISTREAM_MAX_SEGMENTS = 4096  # configurations with more segments skip the per-segment baseline
+
+
 def time_config(pkg, torch, orc, workload, steps, warmup, cpu=True):
     """ms, Gkeys/s and the end-to-end bytes-moved roofline of every form on one workload,
     with the oracle's per-segment stable sort (all host threads, one run) beside it."""
@@ -345,6 +389,16 @@ def time_config(pkg, torch, orc, workload, steps, warmup, cpu=True):
         row["forms"][a] = {"ms": round(ms, 4), "gkeys_s": round(n / (ms * 1e-3) / 1e9, 3),
                            "hbm_frac": round(ab / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
                            "sorted": sorted_within_segments(torch, r, keys.dtype)}
+        del r
+        torch.cuda.empty_cache()
+    if s <= MULTISTREAM_MAX_SEGMENTS:
+        # the per-segment baseline: a few steps (it launches one sort per segment)
+        r = MultiRunner(pkg, torch, keys, vals, off)
+        ms, _ = r.time(max(1, min(steps, 5)), 1)
+        row["forms"]["multistream"] = {"ms": round(ms, 4), "gkeys_s": round(n / (ms * 1e-3) / 1e9, 3),
+                                       "hbm_frac": round(ab / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+                                       "sorted": sorted_within_segments(torch, r, keys.dtype),
+                                       "streams": r.STREAMS, "schedule": "least-loaded"}
         del r
         torch.cuda.empty_cache()
     row["batched_over_plain"] = round(row["forms"]["plain"]["ms"] / row["forms"]["batched"]["ms"], 3)

@@ -141,6 +141,24 @@ int segsort_mgpu_segmented_sort_comp(void* d_keys, void* d_vals, int count, cons
 int segsort_mgpu_segmented_sort_indices(void* d_keys, int* d_indices, int count, const int* d_segments,
                                         int num_segments, int key_type, int comp, void* stream);
 
+/* Per-segment multi-stream baseline: every segment sorted by its own
+ * device-wide sort, the segments dealt over num_streams (1..32) streams round
+ * robin (schedule 0) or each to the stream with the fewest items so far
+ * (schedule 1). Replaces the loops of src/ncub.cu:113-145
+ * (cub::DeviceRadixSort::SortKeys per segment over NUM_STREAMS streams) and
+ * src/newthrust.cu:91-167 (thrust::sort per segment, round robin or SCHEDULE).
+ * h_offsets is a HOST array of num_segments + 1 ascending offsets (segment i
+ * is [h_offsets[i], h_offsets[i+1])), needed by the temp query as well; items
+ * outside every segment are not written. Temp protocol as segsort_sort. The
+ * call synchronises `stream` once (it uploads the segment lengths) and so
+ * cannot be captured; the sorts themselves run on the fan-out streams, joined
+ * back into `stream` before it returns. A comparison point, not a fast path:
+ * it launches one batched sequence per non-empty segment. */
+int segsort_multistream_sort(void* d_temp, size_t* temp_bytes, const void* d_keys_in, void* d_keys_out,
+                             const void* d_vals_in, void* d_vals_out, int num_items, int num_segments,
+                             const int* h_offsets, int begin_bit, int end_bit, int key_type, int value_bytes,
+                             int descending, int num_streams, int schedule, void* stream);
+
 /* Differentiable pair sort, forward helper: d_perm[i] = i - d_begin[r] for
  * every i of row r (the row-local index vector that a stable pair sort then
  * carries as values). One workgroup per row. */

@@ -35,7 +35,7 @@ EXPORTS = (
     "segsort_get_tuning", "segsort_tile_caps", "segsort_last_error", "segsort_version", "segsort_profile_enable",
     "segsort_profile_report", "segsort_set_graph_replay", "segsort_reset_tuning", "segsort_lookup_tuning",
     "segsort_mgpu_segmented_sort_comp", "segsort_mgpu_segmented_sort_indices", "segsort_sort_noalloc",
-    "segsort_noalloc_max_bytes",
+    "segsort_noalloc_max_bytes", "segsort_multistream_sort",
 )
 
 _lib = None
@@ -74,6 +74,9 @@ def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
     lib.segsort_mgpu_segmented_sort_comp.restype = i32
     lib.segsort_mgpu_segmented_sort_indices.argtypes = [vp, vp, i32, vp, i32, i32, i32, vp]
     lib.segsort_mgpu_segmented_sort_indices.restype = i32
+    lib.segsort_multistream_sort.argtypes = [vp, ctypes.POINTER(sz), vp, vp, vp, vp, i32, i32, vp, i32, i32, i32, i32,
+                                             i32, i32, i32, vp]
+    lib.segsort_multistream_sort.restype = i32
     lib.segsort_row_iota.argtypes = [vp, vp, i32, vp, vp]
     lib.segsort_row_iota.restype = i32
     lib.segsort_perm_scatter_f32.argtypes = [vp, vp, vp, vp, i32, vp, vp]
@@ -303,6 +306,44 @@ def segmented_sort(keys, offsets=None, values=None, *, begin=None, end=None, des
         for t in (keys, values, begin, end, keys_out, values_out, temp):
             if t is not None:
                 t.record_stream(stream)
+    return keys_out, values_out
+
+
+def multistream_sort(keys, offsets, values=None, *, num_streams=4, schedule=False, descending=False, begin_bit=0,
+                     end_bit=None):
+    """The per-segment multi-stream baseline (src/ncub.cu:113-145, src/newthrust.cu:91-167):
+    one device-wide sort per segment over ``num_streams`` streams, round robin or, with
+    ``schedule``, each segment to the stream with the fewest items so far. ``offsets`` is
+    a host sequence of S+1 ascending offsets. Runs on the current stream (synchronised
+    once, for the upload of the segment lengths); returns (keys_out, values_out), where
+    positions outside every segment are copied from the input."""
+    import numpy as np
+    import torch
+
+    _check_tensor(keys, "keys", keys.device)
+    if values is not None:
+        _check_tensor(values, "values", keys.device)
+        if values.numel() != keys.numel():
+            raise SegsortError("values must have as many elements as keys")
+    off = np.ascontiguousarray(np.asarray(offsets, dtype=np.int32))
+    n, s = keys.numel(), off.size - 1
+    if s < 0:
+        raise SegsortError("offsets needs at least one entry")
+    kt = key_type_of(keys.dtype)
+    vb = 0 if values is None else values.element_size()
+    eb = keys.element_size() * 8 if end_bit is None else end_bit
+    lib = load_library()
+    hp = off.ctypes.data if s > 0 else None
+    st = torch.cuda.current_stream(keys.device).cuda_stream
+    tb = ctypes.c_size_t(0)
+    args = (n, s, hp, begin_bit, eb, kt, vb, 1 if descending else 0, num_streams, 1 if schedule else 0, st)
+    _check(lib.segsort_multistream_sort(None, ctypes.byref(tb), None, None, None, None, *args),
+           "segsort_multistream_sort")
+    temp = torch.empty(max(tb.value, 1), dtype=torch.uint8, device=keys.device)
+    keys_out = keys.clone()
+    values_out = values.clone() if values is not None else None
+    _check(lib.segsort_multistream_sort(temp.data_ptr(), ctypes.byref(tb), keys.data_ptr(), keys_out.data_ptr(),
+                                        _ptr(values), _ptr(values_out), *args), "segsort_multistream_sort")
     return keys_out, values_out
 
 

@@ -16,7 +16,7 @@ COMMON = ["-O3", "-std=c++17", f"--offload-arch={ARCH}", "-Xclang", "-target-fea
           "-packed-fp32-ops", "-Wall", "-Wno-unused-result"]
 SOURCES = [HERE / "csrc" / "segsort_api.hip"]
 LIB = HERE / "lib" / "libsegsort.so"
-DRIVERS = ["radixseg", "fixcub", "mergeseg", "bbsegsort", "facade_example"]
+DRIVERS = ["radixseg", "fixcub", "mergeseg", "bbsegsort", "nseg", "facade_example"]
 GENERATORS = ["gen_equal", "gen_diff"]  # host-only input generators (g++)
 CXX = os.environ.get("CXX", "g++")
 

@@ -172,9 +172,18 @@ struct SideStreams {
     hipEvent_t join = nullptr;
 };
 
+// Streams of the per-segment baseline (segsort_multistream_sort), per device:
+// a fork event, and one stream and join event per fan-out slot.
+struct FanStreams {
+    hipEvent_t fork = nullptr;
+    std::vector<hipStream_t> s;
+    std::vector<hipEvent_t> join;
+};
+
 struct SidePool {
     std::map<std::pair<int, hipStream_t>, SideStreams> sets;
     std::map<int, hipStream_t> capture;  // the graph-replay mode's private capture stream, per device
+    std::map<int, FanStreams> fan;
     ~SidePool() {
         for (auto& kv : sets) {
             if (kv.second.s) (void)hipStreamDestroy(kv.second.s);
@@ -183,6 +192,11 @@ struct SidePool {
         }
         for (auto& kv : capture)
             if (kv.second) (void)hipStreamDestroy(kv.second);
+        for (auto& kv : fan) {
+            if (kv.second.fork) (void)hipEventDestroy(kv.second.fork);
+            for (hipStream_t s : kv.second.s) (void)hipStreamDestroy(s);
+            for (hipEvent_t e : kv.second.join) (void)hipEventDestroy(e);
+        }
     }
 };
 
@@ -225,6 +239,30 @@ SideStreams* side_streams(hipStream_t st, bool capturing) {
     unsigned long long id = 0;
     if (hipStreamGetCaptureInfo(st, &cs, &id) != hipSuccess || cs != hipStreamCaptureStatusActive) return nullptr;
     return id == side_id ? ss : nullptr;
+}
+
+constexpr int FAN_MAX = 32;  // streams of the per-segment baseline (the reference's loops use up to 32)
+
+// The calling thread's fan-out streams on the current device, at least n of
+// them (n <= FAN_MAX), created on first use with the default priority.
+FanStreams* fan_streams(int n) {
+    FanStreams& f = side_pool().fan[current_device()];
+    if (!f.fork && hipEventCreateWithFlags(&f.fork, hipEventDisableTiming) != hipSuccess) {
+        f.fork = nullptr;
+        return nullptr;
+    }
+    while ((int)f.s.size() < n) {
+        hipStream_t s = nullptr;
+        hipEvent_t e = nullptr;
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            (void)hipStreamDestroy(s);
+            return nullptr;
+        }
+        f.s.push_back(s);
+        f.join.push_back(e);
+    }
+    return &f;
 }
 
 // ---- hipGraph capture of the batched sequence ---------------------------------
@@ -732,6 +770,76 @@ struct Launcher {
         return ok();
     }
 
+    // The per-segment baseline: every segment sorted by its own single-segment
+    // batched sequence, the segments dealt over `ns` streams round robin
+    // (ncub.cu:113-145 over NUM_STREAMS streams, newthrust.cu:141-160) or each
+    // to the stream with the fewest items so far (newthrust.cu:91-107, SCHEDULE).
+    // Each stream owns one temp region sized for the longest segment; the
+    // segment lengths are uploaded into the temp storage (one synchronisation of
+    // st, so that the host array may be released), after which the streams fork
+    // from st and join back into it.
+    static int multistream(void* d_temp, size_t* temp_bytes, const void* ki, void* ko, const void* vi, void* vo,
+                           int N, int S, const int* h_off, KeyMode km, int ns, int schedule, hipStream_t st) {
+        int maxlen = 0;
+        for (int i = 0; i < S; ++i) {
+            const int a = h_off[i], e = h_off[i + 1];
+            if (a < 0 || e < a || e > N)
+                return fail(hipErrorInvalidValue, "segsort_multistream_sort: offsets not ascending inside [0, N]");
+            maxlen = std::max(maxlen, e - a);
+        }
+        Tuning tu = choose_tuning(sizeof(U), HAS_V ? sizeof(V) : 0, std::max(maxlen, 1), 1);
+        tu.concurrent = 0;
+        const Plan p = make_plan<C>(tu);
+        const int levels = levels_for(km, std::max(maxlen, 1), 1, p);
+        const Layout L = make_layout<C, U, V, HAS_V>(p, std::max(maxlen, 1), 1, levels, false);
+        const int used = std::max(1, std::min(ns, S));
+        const size_t lens_bytes = align256(sizeof(int) * ((size_t)S + 1));
+        const size_t per = align256(L.total);
+        if (!d_temp) {
+            *temp_bytes = 255 + lens_bytes + (size_t)used * per;
+            return ok();
+        }
+        if (*temp_bytes < 255 + lens_bytes + (size_t)used * per)
+            return fail(hipErrorInvalidValue, "segsort_multistream_sort: temp storage too small");
+        if (N == 0 || S == 0) return ok();
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (st && hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+            return fail(hipErrorStreamCaptureUnsupported, "segsort_multistream_sort: not capturable (it synchronises)");
+        FanStreams* f = fan_streams(used);
+        if (!f) return fail(hipErrorOutOfMemory, "segsort_multistream_sort: stream creation");
+        char* base = (char*)((((uintptr_t)d_temp) + 255) & ~uintptr_t(255));
+        int* d_lens = (int*)base;  // d_lens[0] = 0, d_lens[1 + i] = length of segment i
+        std::vector<int> lens((size_t)S + 1);
+        lens[0] = 0;
+        for (int i = 0; i < S; ++i) lens[(size_t)i + 1] = h_off[i + 1] - h_off[i];
+        hipError_t e = hipMemcpyAsync(d_lens, lens.data(), sizeof(int) * lens.size(), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e == hipSuccess) e = hipEventRecord(f->fork, st);
+        for (int s = 0; e == hipSuccess && s < used; ++s) e = hipStreamWaitEvent(f->s[(size_t)s], f->fork, 0);
+        if (e != hipSuccess) return fail(e, "segsort_multistream_sort: fork");
+        std::vector<long long> load((size_t)used, 0);
+        for (int i = 0; i < S; ++i) {
+            const int n = lens[(size_t)i + 1];
+            int s = i % used;
+            if (schedule) s = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            load[(size_t)s] += n;
+            if (n == 0) continue;
+            const size_t o = (size_t)h_off[i];
+            char* bs = base + lens_bytes + (size_t)s * per;
+            e = enqueue_batched(bs, L, p, tu, levels_for(km, n, 1, p), (const char*)ki + o * sizeof(U),
+                                (char*)ko + o * sizeof(U), HAS_V ? (const char*)vi + o * sizeof(V) : nullptr,
+                                HAS_V ? (char*)vo + o * sizeof(V) : nullptr, 1, d_lens, d_lens + 1 + i, km,
+                                f->s[(size_t)s]);
+            if (e != hipSuccess) return fail(e, "segsort_multistream_sort: launch");
+        }
+        for (int s = 0; e == hipSuccess && s < used; ++s) {
+            e = hipEventRecord(f->join[(size_t)s], f->s[(size_t)s]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(st, f->join[(size_t)s], 0);
+        }
+        if (e != hipSuccess) return fail(e, "segsort_multistream_sort: join");
+        return ok();
+    }
+
     static int plain(void* d_temp, size_t* temp_bytes, const void* ki, void* ko, const void* vi, void* vo, int N,
                      int S, const int* db, const int* de, KeyMode km, hipStream_t st) {
         const Tuning tu = choose_tuning(sizeof(U), HAS_V ? sizeof(V) : 0, N, S);
@@ -937,6 +1045,46 @@ int segsort_sort(void* d_temp, size_t* temp_bytes, const void* d_keys_in, void* 
     if (value_bytes == 0) return dispatch_algo<uint64_t, uint32_t, false>(SEGSORT_ARGS);
     if (value_bytes == 4) return dispatch_algo<uint64_t, uint32_t, true>(SEGSORT_ARGS);
     return dispatch_algo<uint64_t, uint64_t, true>(SEGSORT_ARGS);
+#undef SEGSORT_ARGS
+}
+
+int segsort_multistream_sort(void* d_temp, size_t* temp_bytes, const void* d_keys_in, void* d_keys_out,
+                             const void* d_vals_in, void* d_vals_out, int num_items, int num_segments,
+                             const int* h_offsets, int begin_bit, int end_bit, int key_type, int value_bytes,
+                             int descending, int num_streams, int schedule, void* stream) {
+    if (!temp_bytes) return fail(hipErrorInvalidValue, "segsort_multistream_sort: temp_bytes is NULL");
+    if (num_items < 0 || num_segments < 0) return fail(hipErrorInvalidValue, "segsort_multistream_sort: negative size");
+    if (num_segments > 0 && !h_offsets) return fail(hipErrorInvalidValue, "segsort_multistream_sort: NULL offsets");
+    if (num_streams < 1 || num_streams > FAN_MAX)
+        return fail(hipErrorInvalidValue, "segsort_multistream_sort: num_streams must be 1..32");
+    if (key_type < SEGSORT_KEY_U32 || key_type > SEGSORT_KEY_F64)
+        return fail(hipErrorInvalidValue, "segsort_multistream_sort: bad key_type");
+    if (value_bytes != 0 && value_bytes != 4 && value_bytes != 8)
+        return fail(hipErrorInvalidValue, "segsort_multistream_sort: value_bytes must be 0, 4 or 8");
+    const int kw = key_type >= SEGSORT_KEY_U64 ? 8 : 4;
+    if (begin_bit < 0 || end_bit > kw * 8 || begin_bit > end_bit)
+        return fail(hipErrorInvalidValue, "segsort_multistream_sort: bad bit range");
+    if (d_temp && num_items > 0 && num_segments > 0) {
+        if (!d_keys_in || !d_keys_out) return fail(hipErrorInvalidValue, "segsort_multistream_sort: NULL buffer");
+        if (value_bytes && (!d_vals_in || !d_vals_out))
+            return fail(hipErrorInvalidValue, "segsort_multistream_sort: NULL values");
+    }
+    KeyMode km;
+    km.kind = (key_type % 3 == 0) ? KIND_UNSIGNED : (key_type % 3 == 1) ? KIND_SIGNED : KIND_FLOAT;
+    km.desc = descending ? 1 : 0;
+    km.begin_bit = begin_bit;
+    km.end_bit = end_bit;
+    hipStream_t st = (hipStream_t)stream;
+#define SEGSORT_ARGS d_temp, temp_bytes, d_keys_in, d_keys_out, d_vals_in, d_vals_out, num_items, num_segments, \
+                     h_offsets, km, num_streams, schedule, st
+    if (kw == 4) {
+        if (value_bytes == 0) return Launcher<uint32_t, uint32_t, false>::multistream(SEGSORT_ARGS);
+        if (value_bytes == 4) return Launcher<uint32_t, uint32_t, true>::multistream(SEGSORT_ARGS);
+        return Launcher<uint32_t, uint64_t, true>::multistream(SEGSORT_ARGS);
+    }
+    if (value_bytes == 0) return Launcher<uint64_t, uint32_t, false>::multistream(SEGSORT_ARGS);
+    if (value_bytes == 4) return Launcher<uint64_t, uint32_t, true>::multistream(SEGSORT_ARGS);
+    return Launcher<uint64_t, uint64_t, true>::multistream(SEGSORT_ARGS);
 #undef SEGSORT_ARGS
 }
 

@@ -96,3 +96,27 @@ def test_tuning_table(pkg):
         pkg.reset_tuning(5, 4)
     caps = pkg.tile_caps(4, 4)
     assert caps == sorted(caps) and caps[-1] >= 8192
+
+
+def _ms_query(pkg, off, n, num_streams=4, kt=0, vb=4):
+    lib = pkg.load_library()
+    tb = ctypes.c_size_t(0)
+    h = (ctypes.c_int * len(off))(*off)
+    rc = lib.segsort_multistream_sort(None, ctypes.byref(tb), None, None, None, None, n, len(off) - 1, h, 0,
+                                      64 if kt >= 3 else 32, kt, vb, 0, num_streams, 0, None)
+    return rc, tb.value
+
+
+def test_multistream_query_and_errors(pkg):
+    off = [0, 100, 100, 5000, 70000]
+    rc, one = _ms_query(pkg, off, 70000, num_streams=1)
+    assert rc == 0 and one >= 256
+    rc, four = _ms_query(pkg, off, 70000, num_streams=4)
+    assert rc == 0 and four > one  # one temp region per stream (4 segments: 4 streams used)
+    assert _ms_query(pkg, off, 70000, num_streams=32)[1] == four  # never more streams than segments
+    assert _ms_query(pkg, off, 70000, num_streams=0)[0] == HIP_INVALID_VALUE
+    assert _ms_query(pkg, off, 70000, num_streams=33)[0] == HIP_INVALID_VALUE
+    assert _ms_query(pkg, [0, 10, 5], 10)[0] == HIP_INVALID_VALUE  # descending offsets
+    assert "ascending" in pkg.last_error()
+    assert _ms_query(pkg, [0, 10, 20], 15)[0] == HIP_INVALID_VALUE  # past num_items
+    assert _ms_query(pkg, off, 70000, kt=7)[0] == HIP_INVALID_VALUE

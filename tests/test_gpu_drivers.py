@@ -1,4 +1,4 @@
-"""The reference-compatible drivers (radixseg, fixcub, mergeseg, bbsegsort) read the
+"""The reference-compatible drivers (radixseg, fixcub, mergeseg, bbsegsort, nseg) read the
 reference's stdin format and print its stdout format; their output equals the
 oracle's, formatted the way src/cub.out is."""
 import json
@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 ROOT = Path(__file__).resolve().parent.parent
 BIN = ROOT / "segmented-sorting_amd" / "bin"
-DRIVERS = ("radixseg", "fixcub", "mergeseg", "bbsegsort")
+DRIVERS = ("radixseg", "fixcub", "mergeseg", "bbsegsort", "nseg")
 
 
 def stdin_text(off, keys):

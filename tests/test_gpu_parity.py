@@ -330,6 +330,35 @@ def test_mgpu_comparator_and_indices(pkg, torch_cuda, orc):
                                                 None) != 0
 
 
+@pytest.mark.parametrize("streams,schedule", [(1, 0), (4, 0), (4, 1), (32, 1)])
+def test_multistream_baseline(pkg, torch_cuda, orc, streams, schedule):
+    """The per-segment multi-stream baseline (ncub.cu:113-145, newthrust.cu:91-167):
+    one device-wide sort per segment over 1..32 streams, round robin or greedy; empty,
+    one-item, wave-, tile- and MSD-sized segments, pairs and keys only, ascending and
+    descending, against the oracle."""
+    torch = torch_cuda
+    rng = np.random.default_rng(83 + streams + schedule)
+    lengths = np.concatenate([[0, 1, 2, 17], rng.integers(0, 3000, 40), [0, 20000, 70000, 5, 300000]])
+    rng.shuffle(lengths)
+    off = pkg.gen.offsets_from_lengths(lengths)
+    n = int(off[-1])
+    cases = [(pkg.gen.u64(83, 1, n).astype(np.uint32), np.arange(n, dtype=np.uint32), False),
+             (rng.standard_normal(n).astype(np.float32), None, True),
+             (pkg.gen.u64(84, 1, n) >> np.uint64(20), np.arange(n, dtype=np.uint64), False)]
+    for keys, vals, desc in cases:
+        print(f"[case] multistream streams={streams} schedule={schedule} keys={keys.dtype} desc={desc}", flush=True)
+        ek, ev = orc.seg_sort(keys, vals, offsets=off, descending=desc)
+        as_torch = {np.dtype(np.uint32): torch.uint32, np.dtype(np.float32): torch.float32,
+                    np.dtype(np.uint64): torch.uint64}
+        kd = dev(torch, keys).view(as_torch[keys.dtype])
+        vd = None if vals is None else dev(torch, vals).view(as_torch[vals.dtype])
+        ko, vo = pkg.multistream_sort(kd, off, vd, num_streams=streams, schedule=bool(schedule), descending=desc)
+        torch.cuda.synchronize()
+        assert np.array_equal(host(ko.view(torch.uint8), keys.dtype), ek)
+        if vals is not None:
+            assert np.array_equal(host(vo.view(torch.uint8), vals.dtype), ev)
+
+
 def _skewed_u32(rng, n, share, prefix_bits):
     """n u32 keys of which a `share` fraction have the same top `prefix_bits` bits."""
     keys = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)

@@ -518,8 +518,12 @@ __device__ __forceinline__ int npos_of(int hi_bit, int begin_bit) {
 // highest varying digit position. The level's digit is the range's highest
 // varying position, which no chunk exceeds; a chunk whose own top is lower
 // holds a single digit value there, known from its AND, so one histogram per
-// chunk is all the partition needs. A chunk of one tile is read once (its
-// keys stay in registers); a longer chunk is read again for the histogram.
+// chunk is all the partition needs. The first read of a chunk counts the
+// digits of the range's top position as it goes, which is the chunk's own
+// top in the common case (keys that vary in their high bits), so the chunk is
+// read once; a chunk whose top digit turns out constant is counted again at
+// its own top (re-read when it spans more than one tile; a one-tile chunk's
+// keys stay in registers).
 template <int NT, int IPT, typename U, typename V, bool HAS_V>
 __global__ __launch_bounds__(NT) void k_large_hist(Bufs b, Sched sc, int level, KeyMode km) {
     constexpr int NW = NT / WAVE, TILE = NT * IPT;
@@ -536,22 +540,54 @@ __global__ __launch_bounds__(NT) void k_large_hist(Bufs b, Sched sc, int level, 
         const int len = chunk_items(it, c);
         const U rmask = range_mask<U>(km.begin_bit, it.hi_bit);
         const U* ks = (const U*)b.k[it.src] + ((int64_t)it.begin + (int64_t)c * it.chunk_len);
-        U key[IPT];
-        auto load_tile = [&](int t0) {
+        U key[IPT], raw[IPT];
+        auto fetch = [&](int t0) {  // raw keys of the tile at t0 (slots past the end read 0)
             const int tn = (len - t0) < TILE ? (len - t0) : TILE;
 #pragma unroll
             for (int j = 0; j < IPT; ++j) {
                 const int i = j * NT + t;
-                key[j] = i < tn ? at32(ks + t0, i) : U(0);
+                raw[j] = i < tn ? at32(ks + t0, i) : U(0);
             }
-#pragma unroll
-            for (int j = 0; j < IPT; ++j) key[j] = U(twiddle_in<U>(key[j], km.kind, km.desc) & rmask);
-            return tn;
         };
+        auto convert = [&]() {
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) key[j] = U(twiddle_in<U>(raw[j], km.kind, km.desc) & rmask);
+        };
+        // digit counts of the tile in registers at position shift sh (digit mask dm)
+        auto count_digits = [&](int tn, int sh, U dm) {
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                if (j * NT + (t & ~63) >= tn) break;  // this and later rows: whole wave past the end
+                const bool v = j * NT + t < tn;
+                const uint32_t d = (uint32_t)((key[j] >> sh) & dm);
+                const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);
+                const uint64_t act = __ballot(v);
+                if (__ballot(v && d != d0) == 0) {
+                    if ((t & 63) == 0) atomicAdd(&hist[d0], (uint32_t)__popcll(act));
+                } else if (v) {
+                    atomicAdd(&hist[d], 1u);
+                }
+            }
+        };
+        auto digit_mask = [&](int sh) {
+            return U((U(1) << ((it.hi_bit - sh) < RADIX_BITS ? (it.hi_bit - sh) : RADIX_BITS)) - 1);
+        };
+        const int top = npos_of(it.hi_bit, km.begin_bit) - 1;
+        const int tsh = km.begin_bit + top * RADIX_BITS;
+        const U tdm = digit_mask(tsh);
+        for (int i = t; i < RADIX; i += NT) hist[i] = 0;
+        __syncthreads();
         U o = 0, a = Bits<U>::ALL;
         int tn = 0;
+        // the next tile's loads are issued before this one is counted, where the
+        // registers allow (16 keys of 64 bits per thread do not)
+        constexpr bool PREFETCH = sizeof(U) * IPT <= 64;
+        if (PREFETCH) fetch(0);
         for (int t0 = 0; t0 < len; t0 += TILE) {
-            tn = load_tile(t0);
+            tn = (len - t0) < TILE ? (len - t0) : TILE;
+            if (!PREFETCH) fetch(t0);
+            convert();
+            if (PREFETCH && t0 + TILE < len) fetch(t0 + TILE);
 #pragma unroll
             for (int j = 0; j < IPT; ++j) {
                 if (j * NT + t < tn) {
@@ -559,6 +595,7 @@ __global__ __launch_bounds__(NT) void k_large_hist(Bufs b, Sched sc, int level, 
                     a &= key[j];
                 }
             }
+            count_digits(tn, tsh, tdm);
         }
         o = wave_or(o);
         a = wave_and(a);
@@ -566,7 +603,6 @@ __global__ __launch_bounds__(NT) void k_large_hist(Bufs b, Sched sc, int level, 
             red_or[t >> 6] = o;
             red_and[t >> 6] = a;
         }
-        for (int i = t; i < RADIX; i += NT) hist[i] = 0;
         __syncthreads();
         o = 0;
         a = Bits<U>::ALL;
@@ -576,24 +612,19 @@ __global__ __launch_bounds__(NT) void k_large_hist(Bufs b, Sched sc, int level, 
         }
         const U var = U((o ^ a) & rmask);
         const int lp = var ? (bit_length<U>(var) - 1 - km.begin_bit) / RADIX_BITS : -1;
-        if (lp >= 0) {
+        if (lp >= 0 && lp != top) {
+            // the top digit is constant in this chunk: count its own top instead
+            for (int i = t; i < RADIX; i += NT) hist[i] = 0;
+            __syncthreads();
             const int sh = km.begin_bit + lp * RADIX_BITS;
-            const U dm = U((U(1) << ((it.hi_bit - sh) < RADIX_BITS ? (it.hi_bit - sh) : RADIX_BITS)) - 1);
+            const U dm = digit_mask(sh);
             for (int t0 = 0; t0 < len; t0 += TILE) {
-                if (len > TILE) tn = load_tile(t0);  // a one-tile chunk still holds its keys
-#pragma unroll
-                for (int j = 0; j < IPT; ++j) {
-                    if (j * NT + (t & ~63) >= tn) break;  // this and later rows: whole wave past the end
-                    const bool v = j * NT + t < tn;
-                    const uint32_t d = (uint32_t)((key[j] >> sh) & dm);
-                    const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);
-                    const uint64_t act = __ballot(v);
-                    if (__ballot(v && d != d0) == 0) {
-                        if ((t & 63) == 0) atomicAdd(&hist[d0], (uint32_t)__popcll(act));
-                    } else if (v) {
-                        atomicAdd(&hist[d], 1u);
-                    }
+                if (len > TILE) {  // a one-tile chunk still holds its keys
+                    tn = (len - t0) < TILE ? (len - t0) : TILE;
+                    fetch(t0);
+                    convert();
                 }
+                count_digits(tn, sh, dm);
             }
         }
         __syncthreads();
@@ -740,11 +771,7 @@ __global__ __launch_bounds__(NT) void k_large_scatter(Bufs b, Sched sc, int leve
             }
         }
         const int dbuf = final_ ? BUF_OUT : child_buf;
-        for (int t0 = 0; t0 < len; t0 += T::TILE) {
-            const int tn = (len - t0) < T::TILE ? (len - t0) : T::TILE;
-            T::partition_run(sm, ks, vs, buf_ptr<U>(b.k[dbuf]), buf_ptr<V>(b.v[dbuf]), start + t0, tn, km, shift,
-                             bits, run);
-        }
+        T::partition_range(sm, ks, vs, buf_ptr<U>(b.k[dbuf]), buf_ptr<V>(b.v[dbuf]), start, len, km, shift, bits, run);
     }
 }
 
@@ -856,11 +883,8 @@ __global__ __launch_bounds__(NT) void k_large_local(Bufs b, Sched sc, int level,
                     fnext[depth] = 0;
                 }
                 __syncthreads();  // the histogram (aliased with the tile) is dead from here
-                for (int c0 = 0; c0 < n; c0 += T::TILE) {
-                    const int cn = (n - c0) < T::TILE ? (n - c0) : T::TILE;
-                    T::partition_run(u.sm, ks, vs, buf_ptr<U>(b.k[dst]), buf_ptr<V>(b.v[dst]), (int64_t)beg + c0,
-                                     cn, km, shift, bits, run);
-                }
+                T::template partition_range<false>(u.sm, ks, vs, buf_ptr<U>(b.k[dst]), buf_ptr<V>(b.v[dst]),
+                                                   (int64_t)beg, n, km, shift, bits, run);
                 if (!final_) {
                     const uint32_t cb = t < RADIX ? cstart[depth][t] : 0u;
                     const uint32_t cn = t < RADIX ? cstart[depth][t + 1] - cb : 0u;
@@ -943,11 +967,8 @@ __global__ __launch_bounds__(NT) void k_seg_lsd(Bufs b, Sched sc, KeyMode km) {
             run[threadIdx.x] = (uint32_t)it.begin + base;
         }
         __syncthreads();
-        for (int t0 = 0; t0 < it.n; t0 += T::TILE) {
-            const int n = (it.n - t0) < T::TILE ? (it.n - t0) : T::TILE;
-            T::partition_run(sm, ks, vs, buf_ptr<U>(b.k[dbuf]), buf_ptr<V>(b.v[dbuf]), (int64_t)it.begin + t0, n,
-                             km, shift, bits, run);
-        }
+        T::partition_range(sm, ks, vs, buf_ptr<U>(b.k[dbuf]), buf_ptr<V>(b.v[dbuf]), (int64_t)it.begin, it.n, km,
+                           shift, bits, run);
     }
 }
 

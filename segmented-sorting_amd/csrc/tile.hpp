@@ -988,33 +988,12 @@ struct RadixTile {
         __syncthreads();
     }
 
-    // Stable partition of the n <= TILE items at ks[begin..) by digit
-    // (twiddled >> shift) & (2^bits - 1). Item with digit d and rank r among
-    // the chunk's digit-d items goes to global index run[d] + r. The chunk is
-    // ordered by digit in LDS first, so each digit run is written contiguously.
-    // The destinations come from an LDS table run[] that the workgroup keeps
-    // across the tiles of one range: afterwards run[d] has advanced by the
-    // tile's count of digit d.
-    __device__ static void partition_run(Smem& sm, const U* ks, const V* vs, U* kd, V* vd, int64_t begin, int n,
-                                         const KeyMode& km, int shift, int bits, uint32_t* run) {
-        partition_body(sm, ks, vs, kd, vd, begin, n, km, shift, bits, run);
-        for (int d = threadIdx.x; d < RADIX; d += NT) {
-            const uint32_t end = d + 1 < RADIX ? sm.dstart[d + 1] : (uint32_t)n;
-            run[d] += end - sm.dstart[d];
-        }
-        __syncthreads();
-    }
-
-    __device__ static void partition_body(Smem& sm, const U* ks, const V* vs, U* kd, V* vd, int64_t begin, int n,
-                                          const KeyMode& km, int shift, int bits, const uint32_t* goff) {
-        partition_with(sm, ks, vs, begin, n, km, shift, bits, goff, [&](uint32_t g, U key, V val) {
-            kd[g] = twiddle_out<U>(key, km.kind, km.desc);
-            if constexpr (HAS_V) vd[g] = val;
-        });
-    }
-
-    // partition_body with the stores left to `out(g, key, value)`: g is the
-    // item's global index, key the (twiddled) key as it sits in LDS.
+    // Stable partition by digit (twiddled >> shift) & (2^bits - 1) of n <= TILE
+    // items: the item with digit d and rank r among the tile's digit-d items
+    // goes to global index goff[d] + r. The tile is ordered by digit in LDS
+    // first, so each digit run is written contiguously. `out(g, key, value)`
+    // stores one item: g is its global index, key the (twiddled) key as it
+    // sits in LDS.
     template <typename Out>
     __device__ static void partition_with(Smem& sm, const U* ks, const V* vs, int64_t begin, int n,
                                           const KeyMode& km, int shift, int bits, const uint32_t* goff, Out out) {
@@ -1032,6 +1011,16 @@ struct RadixTile {
                 }
             }
         }
+        partition_loaded(sm, k, v, n, shift, bits, goff, out);
+    }
+
+    // The partition of n <= TILE items whose twiddled keys (all-ones past n)
+    // and values the threads already hold, in the slot layout of load_keys.
+    template <typename Out>
+    __device__ static void partition_loaded(Smem& sm, const U (&k)[IPT], const V (&v)[IPT], int n, int shift,
+                                            int bits, const uint32_t* goff, Out out) {
+        const int lb = lane_base(n);
+        const bool act = wave() < active_waves(n);
         uint32_t dst[IPT];
         rank_digits(sm, k, n, shift, bits, dst);  // contains barriers
         if (act) {
@@ -1058,6 +1047,57 @@ struct RadixTile {
             }
         }
         __syncthreads();
+    }
+
+    // Stable partition of the len items at ks[begin..) (values alongside) into
+    // kd/vd, tile by tile, with running destinations run[d] in LDS that advance
+    // by each tile's count of digit d (the workgroup keeps them across the
+    // tiles of one range). Software pipelined: the next tile's keys are loaded
+    // into registers before this tile is ranked, and this tile's values right
+    // before that, so that both are in flight during the ranking (barriers wait
+    // for LDS only). Every thread loads every slot of a tile (slots past its
+    // end read its last item again), so each thread's loads keep one shape.
+    template <bool PIPELINE = true>
+    __device__ static void partition_range(Smem& sm, const U* ks, const V* vs, U* kd, V* vd, int64_t begin, int len,
+                                           const KeyMode& km, int shift, int bits, uint32_t* run) {
+        if (len <= 0) return;
+        U rk[IPT];
+        auto fetch = [&](int64_t b0, int n) {
+            const int lb = lane_base(n);
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int p = lb + j * WAVE;
+                rk[j] = at32(ks + b0, (p < n ? p : n - 1));
+            }
+        };
+        // (no prefetch where a second set of keys does not fit the registers:
+        // 16 keys of 64 bits per thread)
+        constexpr bool PREFETCH = PIPELINE && sizeof(U) * IPT <= 64;
+        if (PREFETCH) fetch(begin, len < TILE ? len : TILE);
+        for (int t0 = 0; t0 < len; t0 += TILE) {
+            const int n = (len - t0) < TILE ? (len - t0) : TILE;
+            if (!PREFETCH) fetch(begin + t0, n);
+            U k[IPT];
+            V v[IPT];
+            const int lb = lane_base(n);
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int p = lb + j * WAVE;
+                if constexpr (HAS_V) v[j] = at32(vs + begin + t0, (p < n ? p : n - 1));
+                k[j] = p < n ? twiddle_in<U>(rk[j], km.kind, km.desc) : Bits<U>::ALL;
+            }
+            const int nn = len - t0 - TILE;
+            if (PREFETCH && nn > 0) fetch(begin + t0 + TILE, nn < TILE ? nn : TILE);
+            partition_loaded(sm, k, v, n, shift, bits, run, [&](uint32_t g, U key, V val) {
+                kd[g] = twiddle_out<U>(key, km.kind, km.desc);
+                if constexpr (HAS_V) vd[g] = val;
+            });
+            for (int d = threadIdx.x; d < RADIX; d += NT) {
+                const uint32_t end = d + 1 < RADIX ? sm.dstart[d + 1] : (uint32_t)n;
+                run[d] += end - sm.dstart[d];
+            }
+            __syncthreads();
+        }
     }
 
     // Digit histograms of the n <= TILE items at ks[begin..) for `npos` digit

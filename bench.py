@@ -84,8 +84,15 @@ def to_dev(torch, a):
 
 
 class Runner:
+    """One sort form on device-resident copies of a workload. "batched_bounded" is the
+    batched form called with the workload's longest segment as its length bound
+    (segsort_sort_hinted), as a caller that holds the offsets on the host can."""
+
     def __init__(self, pkg, torch, keys, vals, off, algo):
         self.pkg, self.torch = pkg, torch
+        self.bound = int(np.diff(off).max()) if algo == "batched_bounded" and off.size > 1 else 0
+        if algo == "batched_bounded":
+            algo = "batched"
         self.n, self.s = keys.size, off.size - 1
         self.kt = {np.dtype(np.uint32): 0, np.dtype(np.float32): 2, np.dtype(np.uint64): 3}[keys.dtype]
         self.vb = 0 if vals is None else vals.dtype.itemsize
@@ -100,7 +107,7 @@ class Runner:
         self.eb = keys.dtype.itemsize * 8
         tb = [0]
         rc = pkg.DeviceSegmentedRadixSort._call(None, tb, None, None, None, None, self.n, self.s, None, None, 0,
-                                                self.eb, None, 0, self.kt, self.vb, algo)
+                                                self.eb, None, 0, self.kt, self.vb, algo, self.bound)
         assert rc == 0, pkg.last_error()
         self.temp = torch.empty(tb[0], dtype=torch.uint8, device="cuda")
         self.stream = torch.cuda.current_stream().cuda_stream
@@ -109,7 +116,7 @@ class Runner:
         tb = [self.temp.numel()]
         rc = self.pkg.DeviceSegmentedRadixSort._call(self.temp, tb, self.kin, self.kout, self.vin, self.vout, self.n,
                                                      self.s, self.begin, self.end, 0, self.eb, self.stream, 0,
-                                                     self.kt, self.vb, self.algo)
+                                                     self.kt, self.vb, self.algo, self.bound)
         if rc != 0:
             raise RuntimeError(f"{self.algo}: hipError {rc}: {self.pkg.last_error()}")
 
@@ -379,9 +386,11 @@ def time_config(pkg, torch, orc, workload, steps, warmup, cpu=True):
     keys, vals, off = make_workload(pkg, workload)
     n, s = keys.size, off.size - 1
     ab = algo_bytes_of(keys, vals, s)
-    algos = ["batched", "plain"] + (["fixpass"] if keys.dtype.itemsize == 4 else [])
+    algos = ["batched", "batched_bounded", "plain"] + (["fixpass"] if keys.dtype.itemsize == 4 else [])
+    vbytes = 0 if vals is None else vals.dtype.itemsize
     row = {"n": n, "segments": s, "algorithmic_bytes": ab,
-           "tuning": pkg.lookup_tuning(n, s, keys.dtype.itemsize, 0 if vals is None else vals.dtype.itemsize),
+           "tuning": pkg.lookup_tuning(n, s, keys.dtype.itemsize, vbytes),
+           "tuning_bounded": pkg.lookup_tuning(n, s, keys.dtype.itemsize, vbytes, int(np.diff(off).max())),
            "forms": {}}
     for a in algos:
         r = Runner(pkg, torch, keys, vals, off, a)
@@ -487,7 +496,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="ragged", help="ragged | ragged_dup | equal:<k> | tiny_f32 | tiny_u64")
-    ap.add_argument("--algos", default="batched,plain,fixpass")
+    ap.add_argument("--algos", default="batched,batched_bounded,plain,fixpass")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only", default=None, help="time one algorithm and print its ms (profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

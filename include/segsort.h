@@ -72,6 +72,30 @@ int segsort_sort(void* d_temp, size_t* temp_bytes,
                  int key_type, int value_bytes, int descending, int algo,
                  void* stream);
 
+/* segsort_sort with the caller's bound on the segment lengths: every segment
+ * has end - begin <= max_segment_length (0: no bound, which is segsort_sort).
+ * No counterpart in cub, whose calls cannot see the offsets on the host
+ * either; callers that hold their offsets on the host (the reference's drivers
+ * read them from stdin, src/radixseg.cu:77-90) know the bound for free.
+ * The bound does two things. It selects the tuning measured for that length
+ * distribution (the table's entries carry ceil(log2 max length) as a third
+ * feature beside round(log2 num_items) and round(log2 mean length)). And when
+ * it is at most the largest LDS tile of that tuning, the batched form launches
+ * no multi-workgroup MSD kernels (about 8 us of launches that would find no
+ * work). A wrong bound never gives a wrong result: a segment longer than the
+ * largest tile is then sorted by one workgroup (all digit passes over global
+ * memory, as the plain form's per-segment LSD), only slower. The temp query
+ * must pass the same bound as the call (the tuning it selects sizes temp).
+ * Other arguments as segsort_sort. */
+int segsort_sort_hinted(void* d_temp, size_t* temp_bytes,
+                        const void* d_keys_in, void* d_keys_out,
+                        const void* d_vals_in, void* d_vals_out,
+                        int num_items, int num_segments,
+                        const int* d_begin_offsets, const int* d_end_offsets,
+                        int begin_bit, int end_bit,
+                        int key_type, int value_bytes, int descending, int algo,
+                        int max_segment_length, void* stream);
+
 /* segsort_sort with the batched algorithm and no temp argument, for callers
  * that cannot allocate device memory (code under graph capture, tight serving
  * loops, harnesses that forbid hipMalloc). No counterpart in the reference
@@ -209,6 +233,10 @@ int segsort_get_tuning(int key_width, int value_bytes, int* wave_cap, int* tile_
  * is 1 when it comes from the built-in table, 0 when from an override. */
 int segsort_lookup_tuning(int num_items, int num_segments, int key_width, int value_bytes, int* wave_cap,
                           int* tile_enable, int* grid_mult, int* concurrent, int* from_table);
+/* The same for a call of segsort_sort_hinted with this length bound (0: none). */
+int segsort_lookup_tuning_hinted(int num_items, int num_segments, int max_segment_length, int key_width,
+                                 int value_bytes, int* wave_cap, int* tile_enable, int* grid_mult, int* concurrent,
+                                 int* from_table);
 /* Capacities (items) of the LDS tile classes for a key width / value size;
  * writes up to 4 values and returns how many. */
 int segsort_tile_caps(int key_width, int value_bytes, int* caps);

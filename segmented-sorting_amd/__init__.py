@@ -35,7 +35,7 @@ EXPORTS = (
     "segsort_get_tuning", "segsort_tile_caps", "segsort_last_error", "segsort_version", "segsort_profile_enable",
     "segsort_profile_report", "segsort_set_graph_replay", "segsort_reset_tuning", "segsort_lookup_tuning",
     "segsort_mgpu_segmented_sort_comp", "segsort_mgpu_segmented_sort_indices", "segsort_sort_noalloc",
-    "segsort_noalloc_max_bytes", "segsort_multistream_sort",
+    "segsort_noalloc_max_bytes", "segsort_multistream_sort", "segsort_sort_hinted", "segsort_lookup_tuning_hinted",
 )
 
 _lib = None
@@ -58,6 +58,9 @@ def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
     lib.segsort_sort.argtypes = [vp, ctypes.POINTER(sz), vp, vp, vp, vp, i32, i32, vp, vp, i32, i32, i32, i32, i32,
                                  i32, vp]
     lib.segsort_sort.restype = i32
+    lib.segsort_sort_hinted.argtypes = [vp, ctypes.POINTER(sz), vp, vp, vp, vp, i32, i32, vp, vp, i32, i32, i32, i32,
+                                        i32, i32, i32, vp]
+    lib.segsort_sort_hinted.restype = i32
     lib.segsort_sort_noalloc.argtypes = [vp, vp, vp, vp, i32, i32, vp, vp, i32, i32, i32, i32, i32, vp]
     lib.segsort_sort_noalloc.restype = i32
     lib.segsort_noalloc_max_bytes.argtypes = []
@@ -91,6 +94,8 @@ def load_library(path: str | os.PathLike | None = None) -> ctypes.CDLL:
     ip = ctypes.POINTER(i32)
     lib.segsort_lookup_tuning.argtypes = [i32, i32, i32, i32, ip, ip, ip, ip, ip]
     lib.segsort_lookup_tuning.restype = i32
+    lib.segsort_lookup_tuning_hinted.argtypes = [i32, i32, i32, i32, i32, ip, ip, ip, ip, ip]
+    lib.segsort_lookup_tuning_hinted.restype = i32
     lib.segsort_tile_caps.argtypes = [i32, i32, ctypes.POINTER(i32)]
     lib.segsort_tile_caps.restype = i32
     lib.segsort_last_error.argtypes = []
@@ -166,7 +171,7 @@ class DeviceSegmentedRadixSort:
     @staticmethod
     def _call(d_temp_storage, temp_storage_bytes, d_keys_in, d_keys_out, d_values_in, d_values_out, num_items,
               num_segments, d_begin_offsets, d_end_offsets, begin_bit, end_bit, stream, descending, key_type,
-              value_bytes, algo):
+              value_bytes, algo, max_segment_length=0):
         lib = load_library()
         if key_type is None:
             key_type = key_type_of(d_keys_in.dtype)
@@ -176,11 +181,15 @@ class DeviceSegmentedRadixSort:
         if end_bit is None:
             end_bit = kbits
         tb = ctypes.c_size_t(int(temp_storage_bytes[0]) if temp_storage_bytes else 0)
-        rc = lib.segsort_sort(_ptr(d_temp_storage), ctypes.byref(tb), _ptr(d_keys_in), _ptr(d_keys_out),
-                              _ptr(d_values_in), _ptr(d_values_out), int(num_items), int(num_segments),
-                              _ptr(d_begin_offsets), _ptr(d_end_offsets), int(begin_bit), int(end_bit), int(key_type),
-                              int(value_bytes), int(descending), int(ALGOS.get(algo, algo)),
-                              _stream(stream) if d_temp_storage is not None else None)
+        args = (_ptr(d_temp_storage), ctypes.byref(tb), _ptr(d_keys_in), _ptr(d_keys_out), _ptr(d_values_in),
+                _ptr(d_values_out), int(num_items), int(num_segments), _ptr(d_begin_offsets), _ptr(d_end_offsets),
+                int(begin_bit), int(end_bit), int(key_type), int(value_bytes), int(descending),
+                int(ALGOS.get(algo, algo)))
+        st = _stream(stream) if d_temp_storage is not None else None
+        if max_segment_length:
+            rc = lib.segsort_sort_hinted(*args, int(max_segment_length), st)
+        else:
+            rc = lib.segsort_sort(*args, st)
         if temp_storage_bytes is not None and d_temp_storage is None:
             temp_storage_bytes[0] = tb.value
         return rc
@@ -188,42 +197,45 @@ class DeviceSegmentedRadixSort:
     @staticmethod
     def SortPairs(d_temp_storage, temp_storage_bytes, d_keys_in, d_keys_out, d_values_in, d_values_out, num_items,
                   num_segments, d_begin_offsets, d_end_offsets, begin_bit=0, end_bit=None, stream=None,
-                  key_type=None, value_bytes=None, algo="batched"):
+                  key_type=None, value_bytes=None, algo="batched", max_segment_length=0):
         return DeviceSegmentedRadixSort._call(d_temp_storage, temp_storage_bytes, d_keys_in, d_keys_out, d_values_in,
                                               d_values_out, num_items, num_segments, d_begin_offsets, d_end_offsets,
-                                              begin_bit, end_bit, stream, 0, key_type, value_bytes, algo)
+                                              begin_bit, end_bit, stream, 0, key_type, value_bytes, algo,
+                                              max_segment_length)
 
     @staticmethod
     def SortPairsDescending(d_temp_storage, temp_storage_bytes, d_keys_in, d_keys_out, d_values_in, d_values_out,
                             num_items, num_segments, d_begin_offsets, d_end_offsets, begin_bit=0, end_bit=None,
-                            stream=None, key_type=None, value_bytes=None, algo="batched"):
+                            stream=None, key_type=None, value_bytes=None, algo="batched", max_segment_length=0):
         return DeviceSegmentedRadixSort._call(d_temp_storage, temp_storage_bytes, d_keys_in, d_keys_out, d_values_in,
                                               d_values_out, num_items, num_segments, d_begin_offsets, d_end_offsets,
-                                              begin_bit, end_bit, stream, 1, key_type, value_bytes, algo)
+                                              begin_bit, end_bit, stream, 1, key_type, value_bytes, algo,
+                                              max_segment_length)
 
     @staticmethod
     def SortKeys(d_temp_storage, temp_storage_bytes, d_keys_in, d_keys_out, num_items, num_segments,
                  d_begin_offsets, d_end_offsets, begin_bit=0, end_bit=None, stream=None, key_type=None,
-                 algo="batched"):
+                 algo="batched", max_segment_length=0):
         return DeviceSegmentedRadixSort._call(d_temp_storage, temp_storage_bytes, d_keys_in, d_keys_out, None, None,
                                               num_items, num_segments, d_begin_offsets, d_end_offsets, begin_bit,
-                                              end_bit, stream, 0, key_type, 0, algo)
+                                              end_bit, stream, 0, key_type, 0, algo, max_segment_length)
 
     @staticmethod
     def SortKeysDescending(d_temp_storage, temp_storage_bytes, d_keys_in, d_keys_out, num_items, num_segments,
                            d_begin_offsets, d_end_offsets, begin_bit=0, end_bit=None, stream=None, key_type=None,
-                           algo="batched"):
+                           algo="batched", max_segment_length=0):
         return DeviceSegmentedRadixSort._call(d_temp_storage, temp_storage_bytes, d_keys_in, d_keys_out, None, None,
                                               num_items, num_segments, d_begin_offsets, d_end_offsets, begin_bit,
-                                              end_bit, stream, 1, key_type, 0, algo)
+                                              end_bit, stream, 1, key_type, 0, algo, max_segment_length)
 
 
-def temp_bytes(num_items, num_segments, key_dtype, value_bytes=0, algo="batched", begin_bit=0, end_bit=None):
+def temp_bytes(num_items, num_segments, key_dtype, value_bytes=0, algo="batched", begin_bit=0, end_bit=None,
+               max_segment_length=0):
     """Temp storage the call would need (host-only query, no GPU work)."""
     tb = [0]
     kt = key_type_of(key_dtype) if not isinstance(key_dtype, int) else key_dtype
     rc = DeviceSegmentedRadixSort._call(None, tb, None, None, None, None, num_items, num_segments, None, None,
-                                        begin_bit, end_bit, None, 0, kt, value_bytes, algo)
+                                        begin_bit, end_bit, None, 0, kt, value_bytes, algo, max_segment_length)
     _check(rc, "temp size query")
     return tb[0]
 
@@ -243,7 +255,7 @@ def _check_tensor(t, what, device, dtype=None):
 
 def segmented_sort(keys, offsets=None, values=None, *, begin=None, end=None, descending=False, begin_bit=0,
                    end_bit=None, algo="batched", keys_out=None, values_out=None, temp=None, stream=None,
-                   full=False):
+                   full=False, max_segment_length=0):
     """Stable per-segment sort of cuda tensors; returns (keys_out, values_out).
 
     Segments are [offsets[i], offsets[i+1]) or [begin[i], end[i]) (int32, contiguous, on
@@ -256,6 +268,11 @@ def segmented_sort(keys, offsets=None, values=None, *, begin=None, end=None, des
     the current stream, so the inputs are ready and a later reuse of the temp storage is
     ordered after the sort; the inputs are marked as used on it. The caller synchronises
     with ``stream`` before reading the outputs elsewhere, as with any torch side stream.
+
+    ``max_segment_length`` (0: none) is the caller's bound on the segment lengths
+    (segsort_sort_hinted): it selects the tuning measured for such lengths and, when no
+    segment can exceed the largest LDS tile, skips the multi-workgroup MSD launches. A
+    wrong bound is still sorted correctly, more slowly.
     """
     import torch
 
@@ -280,7 +297,7 @@ def segmented_sort(keys, offsets=None, values=None, *, begin=None, end=None, des
     s = begin.numel()
     kt = key_type_of(keys.dtype)
     vb = 0 if values is None else values.element_size()
-    need = temp_bytes(n, s, kt, vb, algo, begin_bit, end_bit)
+    need = temp_bytes(n, s, kt, vb, algo, begin_bit, end_bit, max_segment_length)
     caller = torch.cuda.current_stream(keys.device)
     if stream is None:
         stream = caller
@@ -298,7 +315,8 @@ def segmented_sort(keys, offsets=None, values=None, *, begin=None, end=None, des
             temp = torch.empty(need, dtype=torch.uint8, device=keys.device)
         tb = [temp.numel()]
         rc = DeviceSegmentedRadixSort._call(temp, tb, keys, keys_out, values, values_out, n, s, begin, end,
-                                            begin_bit, end_bit, stream, 1 if descending else 0, kt, vb, algo)
+                                            begin_bit, end_bit, stream, 1 if descending else 0, kt, vb, algo,
+                                            max_segment_length)
     _check(rc, "segsort_sort")
     if side:
         # tensors made on another stream and used by this sort: their memory is not
@@ -367,12 +385,13 @@ def get_tuning(key_width, value_bytes):
     return {"wave_cap": a.value, "tile_enable": b.value, "grid_mult": c.value, "concurrent": d.value}
 
 
-def lookup_tuning(num_items, num_segments, key_width, value_bytes):
-    """The tuning a call of this shape runs with (the override, else the built-in table's
-    entry), and whether it comes from the table."""
+def lookup_tuning(num_items, num_segments, key_width, value_bytes, max_segment_length=0):
+    """The tuning a call of this shape (and length bound, 0: none) runs with (the
+    override, else the built-in table's entry), and whether it comes from the table."""
     v = [ctypes.c_int() for _ in range(5)]
-    _check(load_library().segsort_lookup_tuning(int(num_items), int(num_segments), key_width, value_bytes,
-                                                *[ctypes.byref(x) for x in v]), "lookup_tuning")
+    _check(load_library().segsort_lookup_tuning_hinted(int(num_items), int(num_segments), int(max_segment_length),
+                                                       key_width, value_bytes, *[ctypes.byref(x) for x in v]),
+           "lookup_tuning")
     return {"wave_cap": v[0].value, "tile_enable": v[1].value, "grid_mult": v[2].value,
             "concurrent": v[3].value, "from_table": bool(v[4].value)}
 

@@ -13,12 +13,17 @@ screening stage.
 `tune()` times every candidate on a resident workload and keeps the fastest one
 whose output is bit-identical to the default tuning's. Results are stored in
 tuning.json keyed by a signature the host knows without reading device memory:
-key width, value width, round(log2 N) and round(log2 mean segment length).
+key width, value width, round(log2 N) and round(log2 mean segment length), plus,
+for calls that state a bound on their segment lengths (segsort_sort_hinted),
+ceil(log2 of that bound): k4v4-n24-m8 for unbounded calls, k4v4-n24-m8-x17 for
+bounded ones. `--bounded` tunes the named workloads as bounded calls (the bound
+is each workload's longest segment, as a caller holding the offsets knows it).
 build.py compiles the table into the library (csrc/tuning_table.inc), which picks
 the entry of each call's signature (or the nearest one) itself, inside the call;
 `apply()` reports that choice.
 
-    python autotune.py --workload ragged   # tunes on the named bench workload (GPU)
+    python autotune.py --workload ragged              # tunes on the named bench workload (GPU)
+    python autotune.py --workload ragged --bounded    # the same, as a call with a length bound
 """
 from __future__ import annotations
 
@@ -32,15 +37,21 @@ TABLE = Path(__file__).resolve().parent / "tuning.json"
 DEFAULT = {"wave_cap": 256, "tile_enable": 0xEF, "grid_mult": 1, "concurrent": 0}
 
 
-def signature(num_items: int, num_segments: int, key_width: int, value_bytes: int) -> str:
+def ceil_log2(v: int) -> int:
+    """The smallest x with v <= 2^x."""
+    return max(int(v) - 1, 0).bit_length()
+
+
+def signature(num_items: int, num_segments: int, key_width: int, value_bytes: int, max_len: int = 0) -> str:
     n = max(num_items, 1)
     mean = max(n / max(num_segments, 1), 1.0)
-    return f"k{key_width}v{value_bytes}-n{round(math.log2(n))}-m{round(math.log2(mean))}"
+    sig = f"k{key_width}v{value_bytes}-n{round(math.log2(n))}-m{round(math.log2(mean))}"
+    return sig + (f"-x{ceil_log2(max_len)}" if max_len > 0 else "")
 
 
 def _parse(sig: str):
-    head, n, m = sig.split("-")
-    return head, int(n[1:]), int(m[1:])
+    parts = sig.split("-")
+    return parts[0], int(parts[1][1:]), int(parts[2][1:]), (int(parts[3][1:]) if len(parts) > 3 else -1)
 
 
 # LDS tile classes (bits 0..3) and wave-group classes (bits 4..7: groups of 4, 8, 16, 32 lanes)
@@ -69,39 +80,45 @@ def load_table(path: Path = TABLE) -> dict:
 
 
 def lookup(sig: str, table: dict | None = None) -> dict:
-    """The stored tuning of the nearest signature with the same key/value widths."""
+    """The stored tuning of the nearest signature with the same key/value widths (the
+    library's rule: a bounded signature looks at the bounded entries first)."""
     table = load_table() if table is None else table
     if sig in table:
         return {k: table[sig][k] for k in DEFAULT}
-    head, n, m = _parse(sig)
-    best, dist = None, None
-    for s, t in table.items():
-        h2, n2, m2 = _parse(s)
-        if h2 != head:
-            continue
-        d = abs(n2 - n) + 2 * abs(m2 - m)
-        if dist is None or d < dist:
-            best, dist = t, d
-    return {k: best[k] for k in DEFAULT} if best else dict(DEFAULT)
+    head, n, m, x = _parse(sig)
+    for bounded in ((True, False) if x >= 0 else (False,)):
+        best, dist = None, None
+        for s, t in table.items():
+            h2, n2, m2, x2 = _parse(s)
+            if h2 != head or (x2 >= 0) != bounded:
+                continue
+            d = abs(n2 - n) + 2 * abs(m2 - m) + (2 * abs(x2 - x) if bounded else 0)
+            if dist is None or d < dist:
+                best, dist = t, d
+        if best:
+            return {k: best[k] for k in DEFAULT}
+    return dict(DEFAULT)
 
 
-def apply(pkg, num_items: int, num_segments: int, key_width: int, value_bytes: int, table: dict | None = None) -> dict:
-    """The tuning the library picks for a call of this shape. With `table` (a table
-    other than the built-in one), install its nearest entry as an override."""
+def apply(pkg, num_items: int, num_segments: int, key_width: int, value_bytes: int, table: dict | None = None,
+          max_len: int = 0) -> dict:
+    """The tuning the library picks for a call of this shape (and length bound, 0:
+    none). With `table` (a table other than the built-in one), install its nearest
+    entry as an override."""
     if table is not None:
-        t = lookup(signature(num_items, num_segments, key_width, value_bytes), table)
+        t = lookup(signature(num_items, num_segments, key_width, value_bytes, max_len), table)
         pkg.set_tuning(key_width, value_bytes, t["wave_cap"], t["tile_enable"], t["grid_mult"], t["concurrent"])
         return t
     pkg.reset_tuning(key_width, value_bytes)
-    t = pkg.lookup_tuning(num_items, num_segments, key_width, value_bytes)
+    t = pkg.lookup_tuning(num_items, num_segments, key_width, value_bytes, max_len)
     return {k: t[k] for k in DEFAULT}
 
 
 def _time(pkg, torch, args, steps, warmup):
-    (kin, kout, vin, vout, n, s, begin, end, kt, vb) = args
+    (kin, kout, vin, vout, n, s, begin, end, kt, vb, mx) = args
     tb = [0]
     rc = pkg.DeviceSegmentedRadixSort._call(None, tb, None, None, None, None, n, s, None, None, 0, None, None, 0, kt,
-                                            vb, "batched")
+                                            vb, "batched", mx)
     if rc:
         raise pkg.SegsortError(pkg.last_error())
     temp = torch.empty(tb[0], dtype=torch.uint8, device="cuda")
@@ -109,7 +126,7 @@ def _time(pkg, torch, args, steps, warmup):
 
     def step():
         rc = pkg.DeviceSegmentedRadixSort._call(temp, [temp.numel()], kin, kout, vin, vout, n, s, begin, end, 0, None,
-                                                st, 0, kt, vb, "batched")
+                                                st, 0, kt, vb, "batched", mx)
         if rc:
             raise pkg.SegsortError(pkg.last_error())
 
@@ -129,17 +146,18 @@ def _same(torch, a, b) -> bool:
     return torch.equal(a.view(torch.uint8), b.view(torch.uint8))
 
 
-def tune(pkg, torch, keys, vals, offsets, steps=5, warmup=2, verbose=True, quick=False, cands=None):
+def tune(pkg, torch, keys, vals, offsets, steps=5, warmup=2, verbose=True, quick=False, cands=None, max_len=0):
     """Time every candidate (default: all of `candidates(quick)`) on device-resident
-    (keys, vals, offsets) tensors; return (best tuning, {candidate: ms}). Candidates
-    whose output differs from the default tuning's are rejected (they would be a bug)."""
+    (keys, vals, offsets) tensors, as calls with the length bound max_len (0: none);
+    return (best tuning, {candidate: ms}). Candidates whose output differs from the
+    default tuning's are rejected (they would be a bug)."""
     kt = pkg.key_type_of(keys.dtype)
     kw = keys.element_size()
     vb = 0 if vals is None else vals.element_size()
     n, s = keys.numel(), offsets.numel() - 1
     kout = torch.empty_like(keys)
     vout = torch.empty_like(vals) if vals is not None else None
-    args = (keys, kout, vals, vout, n, s, offsets[:-1], offsets[1:], kt, vb)
+    args = (keys, kout, vals, vout, n, s, offsets[:-1], offsets[1:], kt, vb, int(max_len))
     pkg.set_tuning(kw, vb, **DEFAULT)
     _time(pkg, torch, args, 1, 0)
     ref_k, ref_v = kout.clone(), (vout.clone() if vout is not None else None)
@@ -193,6 +211,8 @@ def main(argv=None):
     ap.add_argument("--workload", action="append", default=None)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--finalists", type=int, default=6)
+    ap.add_argument("--bounded", action="store_true",
+                    help="tune as calls that pass their longest segment as the length bound")
     args = ap.parse_args(argv)
     root = Path(__file__).resolve().parent.parent
     sys.path.insert(0, str(root))
@@ -207,13 +227,14 @@ def main(argv=None):
                "i": {4: torch.int32, 8: torch.int64}}[keys.dtype.kind][keys.dtype.itemsize]
         dev = (bench.to_dev(torch, keys).view(kdt), bench.to_dev(torch, vals).view(torch.uint32),
                torch.from_numpy(off).cuda())
-        best, results = tune(pkg, torch, *dev, steps=args.steps, verbose=False)
-        sig = signature(keys.size, off.size - 1, keys.dtype.itemsize, vals.dtype.itemsize)
-        by_sig.setdefault(sig, []).append((wl, dev, results))
+        mx = int((off[1:] - off[:-1]).max()) if args.bounded and off.size > 1 else 0
+        best, results = tune(pkg, torch, *dev, steps=args.steps, verbose=False, max_len=mx)
+        sig = signature(keys.size, off.size - 1, keys.dtype.itemsize, vals.dtype.itemsize, mx)
+        by_sig.setdefault(sig, []).append((wl, dev, results, mx))
         print(json.dumps({"workload": wl, "signature": sig, "screen_best": best[0], "ms": round(best[1], 4)}),
               flush=True)
     for sig, runs in by_sig.items():
-        score = _score([r[2] for r in runs])
+        score = _score([r[2] for r in runs])  # r = (workload, tensors, results, length bound)
         ranked = sorted((c for c, v in score.items() if v is not None), key=lambda c: score[c])
         shapes = []  # (wave_cap, tile_enable) of the best screened candidates, in rank order
         for c in ranked:
@@ -225,8 +246,8 @@ def main(argv=None):
         finalists = [{"wave_cap": wc, "tile_enable": te, "grid_mult": gm, "concurrent": k}
                      for wc, te in shapes for gm in FINAL_GRID_MULTS for k in (0, 1)]
         refined = []
-        for wl, dev, _ in runs:
-            _, res = tune(pkg, torch, *dev, steps=args.steps * 6, verbose=False, cands=finalists)
+        for wl, dev, _, mx in runs:
+            _, res = tune(pkg, torch, *dev, steps=args.steps * 6, verbose=False, cands=finalists, max_len=mx)
             refined.append(res)
             print(json.dumps({"workload": wl, "finalists": {c: round(v, 4) for c, v in res.items() if v}}),
                   flush=True)

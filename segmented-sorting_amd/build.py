@@ -34,16 +34,20 @@ TABLE_INC = HERE / "csrc" / "tuning_table.inc"
 
 def table_source(table: dict) -> str:
     """C++ text of the library's built-in tuning table (segsort_api.hip includes it)
-    from the autotuner's tuning.json: one entry per signature k<K>v<V>-n<N>-m<M>."""
+    from the autotuner's tuning.json: one entry per signature k<K>v<V>-n<N>-m<M>, or
+    k<K>v<V>-n<N>-m<M>-x<X> for calls that bound their segment lengths by 2^X."""
     rows = []
     for sig in sorted(table):
-        head, n, m = sig.split("-")
+        parts = sig.split("-")
+        head, n, m = parts[:3]
+        x = int(parts[3][1:]) if len(parts) > 3 else -1  # ceil(log2 max length) of bounded calls, -1: no bound
         kw, vb = head[1:].split("v")
         t = table[sig]
-        rows.append(f"    {{{int(kw)}, {int(vb)}, {int(n[1:])}, {int(m[1:])}, "
+        rows.append(f"    {{{int(kw)}, {int(vb)}, {int(n[1:])}, {int(m[1:])}, {x}, "
                     f"{{{t['wave_cap']}, 0x{t['tile_enable']:02X}, {t['grid_mult']}, {t['concurrent']}}}}},  // {sig}")
     return ("// Generated by build.py from tuning.json (the autotuner's table); do not edit.\n"
             "// {key width, value bytes, round(log2 N), round(log2 mean length),\n"
+            "//  ceil(log2 max length) of calls that bound their lengths (-1: no bound),\n"
             "//  {wave_cap, tile_enable, grid_mult, concurrent}}\n"
             f"constexpr int TUNING_TABLE_SIZE = {len(rows)};\n"
             "constexpr TableEntry TUNING_TABLE[TUNING_TABLE_SIZE > 0 ? TUNING_TABLE_SIZE : 1] = {\n"

@@ -176,6 +176,23 @@ __device__ __forceinline__ void push_large(const Sched& sc, int level, int begin
     }
 }
 
+// A segment longer than the largest tile in a call whose caller promised that
+// no segment is (a wrong max_segment_length): queued for the one-workgroup-per-
+// segment LSD, which such calls launch in place of the MSD tier.
+__device__ __forceinline__ void push_oversize(const Sched& sc, int begin, int n) {
+    const uint32_t slot = (uint32_t)atomicAdd(&sc.ctr->level[MAX_LEVELS], 1ull);
+    atomicAdd(&sc.ctr->level_items[MAX_LEVELS], (unsigned long long)n);
+    if (slot < sc.lq_cap) {
+        LargeItem li;
+        li.begin = begin;
+        li.n = n;
+        li.src = BUF_IN;
+        li.hi_bit = 0;
+        li.chunk_base = li.chunk_len = li.nch = li.pad = 0;
+        sc.lq[MAX_LEVELS][slot] = li;
+    }
+}
+
 // Bin every segment by length: length 1 is copied here, the rest go to the
 // queue of their class (batched and plain forms) or to MSD level 0 (large).
 // Each block bins spt * BIN_NT segments (spt <= BIN_SPT, chosen by the host so
@@ -220,7 +237,8 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
             if constexpr (HAS_V) buf_ptr<V>(b.v[BUF_OUT])[beg[j]] = ((const V*)b.v[BUF_IN])[beg[j]];
         }
         cls[j] = n[j] >= 2 ? class_of(cc, n[j]) : -2;
-        if (cls[j] == -1 && large_to_level0) push_large(sc, MSD_LEVEL0, beg[j], n[j], BUF_IN, end_bit, true);
+        if (cls[j] == -1 && large_to_level0 == 1) push_large(sc, MSD_LEVEL0, beg[j], n[j], BUF_IN, end_bit, true);
+        if (cls[j] == -1 && large_to_level0 == 2) push_oversize(sc, beg[j], n[j]);
     }
     // per class: the wave's slots counted with ballots in registers, then one LDS
     // atomic per class and wave for the wave's base inside the block
@@ -929,7 +947,8 @@ __global__ __launch_bounds__(NT) void k_large_local(Bufs b, Sched sc, int level,
 
 // All digit passes of one segment by one workgroup (the reference's one-CTA-
 // per-segment radix, dispatch_radix_sort.cuh:313-426, with 8-bit digits):
-// histogram the whole segment, scan, then rank and scatter tile by tile.
+// histogram the whole segment, scan, then rank and scatter tile by tile. A
+// workgroup takes queue entries blockIdx.x, blockIdx.x + gridDim.x, ...
 template <int NT, int IPT, typename U, typename V, bool HAS_V>
 __global__ __launch_bounds__(NT) void k_seg_lsd(Bufs b, Sched sc, KeyMode km) {
     using T = RadixTile<NT, IPT, U, V, HAS_V>;
@@ -939,36 +958,38 @@ __global__ __launch_bounds__(NT) void k_seg_lsd(Bufs b, Sched sc, KeyMode km) {
     __shared__ uint32_t wsum[RADIX / WAVE > NT / WAVE ? RADIX / WAVE : NT / WAVE];
     const uint32_t nl = (uint32_t)sc.ctr->level[MAX_LEVELS];
     const int count = (int)(nl < sc.lq_cap ? nl : sc.lq_cap);
-    if ((int)blockIdx.x >= count) return;
-    const LargeItem it = sc.lq[MAX_LEVELS][blockIdx.x];
     const int nbits = km.end_bit - km.begin_bit;
     const int passes = (nbits + RADIX_BITS - 1) / RADIX_BITS;
-    for (int ps = 0; ps < passes; ++ps) {
-        // pass ps reads IN (ps==0) or the ping buffer, writes OUT on the last pass
-        const int sbuf = ps == 0 ? BUF_IN : ((ps & 1) ? BUF_TMP_A : BUF_TMP_B);
-        const int dbuf = ps == passes - 1 ? BUF_OUT : ((ps & 1) ? BUF_TMP_B : BUF_TMP_A);
-        const int shift = km.begin_bit + ps * RADIX_BITS;
-        const int bits = (km.end_bit - shift) < RADIX_BITS ? (km.end_bit - shift) : RADIX_BITS;
-        const U* ks = (const U*)b.k[sbuf];
-        const V* vs = (const V*)b.v[sbuf];
-        for (int i = threadIdx.x; i < RADIX; i += NT) hist[i] = 0;
-        __syncthreads();
-        T::histogram(hist, ks, it.begin, it.n, km, shift, 1, shift + bits);
-        __syncthreads();
-        // exclusive scan of hist -> run (begin-relative global offsets)
-        uint32_t mine = 0;
-        if ((int)threadIdx.x < RADIX) mine = hist[threadIdx.x];
-        const uint32_t incl = wave_incl_scan(mine);
-        if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = incl;
-        __syncthreads();
-        if ((int)threadIdx.x < RADIX) {
-            uint32_t base = incl - mine;
-            for (int ww = 0; ww < (int)(threadIdx.x >> 6); ++ww) base += wsum[ww];
-            run[threadIdx.x] = (uint32_t)it.begin + base;
+    for (int q = blockIdx.x; q < count; q += gridDim.x) {
+        if (q != (int)blockIdx.x) __syncthreads();  // the previous segment's last partition is done with LDS
+        const LargeItem it = sc.lq[MAX_LEVELS][q];
+        for (int ps = 0; ps < passes; ++ps) {
+            // pass ps reads IN (ps==0) or the ping buffer, writes OUT on the last pass
+            const int sbuf = ps == 0 ? BUF_IN : ((ps & 1) ? BUF_TMP_A : BUF_TMP_B);
+            const int dbuf = ps == passes - 1 ? BUF_OUT : ((ps & 1) ? BUF_TMP_B : BUF_TMP_A);
+            const int shift = km.begin_bit + ps * RADIX_BITS;
+            const int bits = (km.end_bit - shift) < RADIX_BITS ? (km.end_bit - shift) : RADIX_BITS;
+            const U* ks = (const U*)b.k[sbuf];
+            const V* vs = (const V*)b.v[sbuf];
+            for (int i = threadIdx.x; i < RADIX; i += NT) hist[i] = 0;
+            __syncthreads();
+            T::histogram(hist, ks, it.begin, it.n, km, shift, 1, shift + bits);
+            __syncthreads();
+            // exclusive scan of hist -> run (begin-relative global offsets)
+            uint32_t mine = 0;
+            if ((int)threadIdx.x < RADIX) mine = hist[threadIdx.x];
+            const uint32_t incl = wave_incl_scan(mine);
+            if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = incl;
+            __syncthreads();
+            if ((int)threadIdx.x < RADIX) {
+                uint32_t base = incl - mine;
+                for (int ww = 0; ww < (int)(threadIdx.x >> 6); ++ww) base += wsum[ww];
+                run[threadIdx.x] = (uint32_t)it.begin + base;
+            }
+            __syncthreads();
+            T::partition_range(sm, ks, vs, buf_ptr<U>(b.k[dbuf]), buf_ptr<V>(b.v[dbuf]), (int64_t)it.begin, it.n, km,
+                               shift, bits, run);
         }
-        __syncthreads();
-        T::partition_range(sm, ks, vs, buf_ptr<U>(b.k[dbuf]), buf_ptr<V>(b.v[dbuf]), (int64_t)it.begin, it.n, km,
-                           shift, bits, run);
     }
 }
 

@@ -27,6 +27,10 @@ thread_local std::string g_last_error = "no error";
 // and allocates nothing.
 thread_local bool t_no_alloc = false;
 
+// The caller's bound on the segment lengths of the call in progress
+// (segsort_sort_hinted), 0 when none was given.
+thread_local int t_max_len = 0;
+
 int fail(hipError_t e, const char* what) {
     g_last_error = std::string(what) + ": " + hipGetErrorString(e);
     return (int)e;
@@ -43,6 +47,10 @@ size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 // child of a long segment that holds more than MSD_BIG_TILES tiles (skewed
 // keys), and a small grid keeps its launches short when it has none.
 constexpr int MSD_LEVEL1_GRID = 128;
+
+// Workgroups of the oversize fallback (k_seg_lsd over the segments that break
+// a caller's length bound): empty in every correct call, so small.
+constexpr int FALLBACK_GRID = 32;
 
 // Segments per binning thread: one while that keeps >= 64 blocks, at most BIN_SPT.
 int bin_spt(int S) { return std::min(BIN_SPT, std::max(1, (S + BIN_NT * 64 - 1) / (BIN_NT * 64))); }
@@ -63,13 +71,15 @@ struct Tuning {
 
 // The built-in tuning table (generated from tuning.json by build.py): one entry
 // per signature (key width, value width, round(log2 N), round(log2 mean
-// segment length)), as the autotuner measured them. A call takes the entry of
-// its own signature, else the nearest one with the same key and value widths
-// (distance |d log2 N| + 2 |d log2 mean|), else DEFAULT_TUNING, the way cub
-// picks its tuning policy inside the call (dispatch_radix_sort.cuh:441-661):
-// host arithmetic only, no device read.
+// segment length), and ceil(log2 max segment length) for the calls that state a
+// bound on their segment lengths, -1 for those that do not), as the autotuner
+// measured them. A call takes the entry of its own signature, else the nearest
+// one with the same key and value widths (distance |d log2 N| + 2 |d log2 mean|
+// + 2 |d log2 max|; a call with a bound looks at the bounded entries first),
+// else DEFAULT_TUNING, the way cub picks its tuning policy inside the call
+// (dispatch_radix_sort.cuh:441-661): host arithmetic only, no device read.
 struct TableEntry {
-    int key_width, value_bytes, log2n, log2m;
+    int key_width, value_bytes, log2n, log2m, log2x;
     Tuning tu;
 };
 #include "tuning_table.inc"
@@ -90,27 +100,41 @@ int tuning_slot(int key_width, int value_bytes) {
 
 int rounded_log2(double x) { return (int)std::lround(std::log2(x < 1.0 ? 1.0 : x)); }
 
-const TableEntry* table_lookup(int key_width, int value_bytes, long long num_items, long long num_segments) {
+// Bits needed for a length: the smallest x with max_len <= 2^x.
+int ceil_log2(long long v) {
+    int x = 0;
+    while (x < 62 && (1ll << x) < v) ++x;
+    return x;
+}
+
+const TableEntry* table_lookup(int key_width, int value_bytes, long long num_items, long long num_segments,
+                               long long max_len) {
     const int ln = rounded_log2((double)(num_items > 1 ? num_items : 1));
     const double mean = (double)(num_items > 1 ? num_items : 1) / (double)(num_segments > 1 ? num_segments : 1);
     const int lm = rounded_log2(mean);
-    const TableEntry* best = nullptr;
-    int dist = 0;
-    for (int i = 0; i < TUNING_TABLE_SIZE; ++i) {
-        const TableEntry& e = TUNING_TABLE[i];
-        if (e.key_width != key_width || e.value_bytes != value_bytes) continue;
-        const int d = std::abs(e.log2n - ln) + 2 * std::abs(e.log2m - lm);
-        if (!best || d < dist) {
-            best = &e;
-            dist = d;
+    const int lx = max_len > 0 ? ceil_log2(max_len) : -1;
+    for (int pass = lx >= 0 ? 0 : 1; pass < 2; ++pass) {
+        // pass 0: the bounded entries (calls with a bound); pass 1: the others
+        const TableEntry* best = nullptr;
+        int dist = 0;
+        for (int i = 0; i < TUNING_TABLE_SIZE; ++i) {
+            const TableEntry& e = TUNING_TABLE[i];
+            if (e.key_width != key_width || e.value_bytes != value_bytes) continue;
+            if ((pass == 0) != (e.log2x >= 0)) continue;
+            const int d = std::abs(e.log2n - ln) + 2 * std::abs(e.log2m - lm) + (pass == 0 ? 2 * std::abs(e.log2x - lx) : 0);
+            if (!best || d < dist) {
+                best = &e;
+                dist = d;
+            }
         }
+        if (best) return best;
     }
-    return best;
+    return nullptr;
 }
 
 // The tuning a call of this shape runs with: the override of its slot if one is
 // installed, else the table's.
-Tuning choose_tuning(int key_width, int value_bytes, long long num_items, long long num_segments,
+Tuning choose_tuning(int key_width, int value_bytes, long long num_items, long long num_segments, long long max_len,
                      bool* from_override = nullptr) {
     const int slot = tuning_slot(key_width, value_bytes);
     {
@@ -118,7 +142,7 @@ Tuning choose_tuning(int key_width, int value_bytes, long long num_items, long l
         if (from_override) *from_override = g_override_on[slot];
         if (g_override_on[slot]) return g_override[slot];
     }
-    const TableEntry* e = table_lookup(key_width, value_bytes, num_items, num_segments);
+    const TableEntry* e = table_lookup(key_width, value_bytes, num_items, num_segments, max_len);
     return e ? e->tu : DEFAULT_TUNING;
 }
 
@@ -271,6 +295,7 @@ FanStreams* fan_streams(int n) {
 struct GraphKey {
     const void* p[8];  // temp, keys in/out, values in/out, begin, end, stream
     int n, s, kw, vb, dev;
+    int bounded;  // the caller's length bound replaced the MSD tier by the oversize fallback
     ssort::KeyMode km;
     Tuning tu;
 };
@@ -472,6 +497,9 @@ Layout make_layout(const Plan& p, int num_items, int num_segments, int levels, b
         L.lq[MSD_LEVEL0] = take(L.lq_cap * sizeof(LargeItem));
         L.lq[MSD_LEVEL1] = take(L.lq_cap * sizeof(LargeItem));
         L.lq[MSD_LOCAL] = take(L.lq_cap * sizeof(LargeItem));
+        // segments above the largest tile in a call that bounded its lengths below
+        // it (a wrong bound), for the one-workgroup-per-segment fallback
+        L.lq[MAX_LEVELS] = take(L.lq_cap * sizeof(LargeItem));
         for (int l = 0; l < 2; ++l) {
             L.hist[l] = take(L.chunks * RADIX * sizeof(uint32_t));
             L.cinfo[l] = take(L.chunks * sizeof(ChunkInfo));
@@ -596,7 +624,7 @@ struct Launcher {
         if (plain) {
             sc.lq[MAX_LEVELS] = (LargeItem*)(base + L.lq[MAX_LEVELS]);
         } else {
-            for (int l = 0; l < MAX_LEVELS; ++l) sc.lq[l] = L.lq[l] ? (LargeItem*)(base + L.lq[l]) : nullptr;
+            for (int l = 0; l <= MAX_LEVELS; ++l) sc.lq[l] = L.lq[l] ? (LargeItem*)(base + L.lq[l]) : nullptr;
             for (int l = 0; l < 2; ++l) {
                 sc.hist[l] = L.hist[l] ? (uint32_t*)(base + L.hist[l]) : nullptr;
                 sc.cinfo[l] = L.cinfo[l] ? (ChunkInfo*)(base + L.cinfo[l]) : nullptr;
@@ -649,9 +677,12 @@ struct Launcher {
     }
 
     // Launch the batched sequence (bin, MSD levels, tile tiers) on st.
+    // With `fallback` (levels is then 0) the call's caller bounded its segment
+    // lengths by the largest tile: the MSD tier is not launched, and a segment
+    // that breaks the bound is sorted by one workgroup (k_seg_lsd) at the end.
     static hipError_t enqueue_batched(char* base, const Layout& L, const Plan& p, const Tuning& tu, int levels,
                                       const void* ki, void* ko, const void* vi, void* vo, int S, const int* db,
-                                      const int* de, KeyMode km, hipStream_t st) {
+                                      const int* de, KeyMode km, hipStream_t st, bool fallback = false) {
         // With `concurrent` and long segments possible, the
         // binned ranges are sorted on a second stream while the MSD tier (a few
         // hundred workgroups at most, latency-bound) partitions the long ones on
@@ -678,14 +709,17 @@ struct Launcher {
         const Bufs b = make_bufs(base, L, ki, ko, vi, vo);
         const size_t mark = Profiler::get().mark();
         SEGSORT_LAUNCH("init", st, -1, 0.0, 0.0, (k_init<<<1, 64, 0, st>>>(sc.ctr)));
+        const int large_mode = fallback ? 2 : 1;  // k_bin: segments above the largest tile to MSD level 0 or the fallback
         if (split && !capturing) {
             // the fork event completes with the binning kernel itself (no separate marker on the caller's stream)
             SEGSORT_LAUNCH("bin", st, -1, 0.0, 8.0 * S,
                            hipExtLaunchKernelGGL(k_bin<U, V, HAS_V>, dim3(bin_grid(S)), dim3(BIN_NT), 0, st, nullptr,
-                                                 ss->fork, 0, db, de, S, b, sc, p.cc, km.end_bit, 1, bin_spt(S)));
+                                                 ss->fork, 0, db, de, S, b, sc, p.cc, km.end_bit, large_mode,
+                                                 bin_spt(S)));
         } else {
             SEGSORT_LAUNCH("bin", st, -1, 0.0, 8.0 * S,
-                           (k_bin<U, V, HAS_V><<<bin_grid(S), BIN_NT, 0, st>>>(db, de, S, b, sc, p.cc, km.end_bit, 1, bin_spt(S))));
+                           (k_bin<U, V, HAS_V><<<bin_grid(S), BIN_NT, 0, st>>>(db, de, S, b, sc, p.cc, km.end_bit,
+                                                                             large_mode, bin_spt(S))));
         }
         hipError_t e = hipSuccess;
         if (split) {
@@ -726,6 +760,10 @@ struct Launcher {
                            (kl<<<gl, C::NT3, 0, st>>>(b, sc, MSD_LOCAL, km, p.cc, p.tile_max)));
         }
         launch_classes(p, b, sc, sc.child_q, km, st, tu.grid_mult);
+        if (fallback) {
+            SEGSORT_LAUNCH("oversize_lsd", st, 100 + MAX_LEVELS, PAIR_BYTES, 0.0,
+                           (k_seg_lsd<C::NT3, C::IPT3, U, V, HAS_V><<<FALLBACK_GRID, C::NT3, 0, st>>>(b, sc, km)));
+        }
         if (split && (e = hipStreamWaitEvent(st, ss->join, 0)) != hipSuccess) return e;
         profile_attribute(mark, sc, p.nclass, st);
         return hipGetLastError();
@@ -733,9 +771,12 @@ struct Launcher {
 
     static int batched(void* d_temp, size_t* temp_bytes, const void* ki, void* ko, const void* vi, void* vo, int N,
                        int S, const int* db, const int* de, KeyMode km, hipStream_t st) {
-        const Tuning tu = choose_tuning(sizeof(U), HAS_V ? sizeof(V) : 0, N, S);
+        const Tuning tu = choose_tuning(sizeof(U), HAS_V ? sizeof(V) : 0, N, S, t_max_len);
         const Plan p = make_plan<C>(tu);
         const int levels = levels_for(km, N, S, p);
+        // the caller's bound keeps every segment inside the largest tile: no MSD
+        // launches (the layout keeps its buffers, for the fallback)
+        const bool fallback = levels > 0 && t_max_len > 0 && t_max_len <= p.tile_max;
         const Layout L = make_layout<C, U, V, HAS_V>(p, N, S, levels, false);
         if (!d_temp) {
             *temp_bytes = L.total;
@@ -755,15 +796,17 @@ struct Launcher {
         key.vb = HAS_V ? (int)sizeof(V) : 0;
         key.tu = tu;
         key.dev = current_device();
+        key.bounded = fallback ? 1 : 0;
+        const int run_levels = fallback ? 0 : levels;
         hipError_t e;
         if (t_no_alloc) {
             Tuning one = tu;
             one.concurrent = 0;
-            e = enqueue_batched(base, L, p, one, levels, ki, ko, vi, vo, S, db, de, km, st);
+            e = enqueue_batched(base, L, p, one, run_levels, ki, ko, vi, vo, S, db, de, km, st, fallback);
         } else {
             e = run_graph(key, st, [&](hipStream_t cs, bool captured) {
                 (void)captured;
-                return enqueue_batched(base, L, p, tu, levels, ki, ko, vi, vo, S, db, de, km, cs);
+                return enqueue_batched(base, L, p, tu, run_levels, ki, ko, vi, vo, S, db, de, km, cs, fallback);
             });
         }
         if (e != hipSuccess) return fail(e, "segsort_sort(batched): launch");
@@ -787,7 +830,7 @@ struct Launcher {
                 return fail(hipErrorInvalidValue, "segsort_multistream_sort: offsets not ascending inside [0, N]");
             maxlen = std::max(maxlen, e - a);
         }
-        Tuning tu = choose_tuning(sizeof(U), HAS_V ? sizeof(V) : 0, std::max(maxlen, 1), 1);
+        Tuning tu = choose_tuning(sizeof(U), HAS_V ? sizeof(V) : 0, std::max(maxlen, 1), 1, 0);
         tu.concurrent = 0;
         const Plan p = make_plan<C>(tu);
         const int levels = levels_for(km, std::max(maxlen, 1), 1, p);
@@ -842,7 +885,7 @@ struct Launcher {
 
     static int plain(void* d_temp, size_t* temp_bytes, const void* ki, void* ko, const void* vi, void* vo, int N,
                      int S, const int* db, const int* de, KeyMode km, hipStream_t st) {
-        const Tuning tu = choose_tuning(sizeof(U), HAS_V ? sizeof(V) : 0, N, S);
+        const Tuning tu = choose_tuning(sizeof(U), HAS_V ? sizeof(V) : 0, N, S, t_max_len);
         const Plan p = make_plan<C>(tu);
         const Layout L = make_layout<C, U, V, HAS_V>(p, N, S, 0, true);
         if (!d_temp) {
@@ -1128,6 +1171,20 @@ extern "C" {
 
 size_t segsort_noalloc_max_bytes(void) { return SCRATCH_BYTES; }
 
+int segsort_sort_hinted(void* d_temp, size_t* temp_bytes, const void* d_keys_in, void* d_keys_out,
+                        const void* d_vals_in, void* d_vals_out, int num_items, int num_segments,
+                        const int* d_begin_offsets, const int* d_end_offsets, int begin_bit, int end_bit, int key_type,
+                        int value_bytes, int descending, int algo, int max_segment_length, void* stream) {
+    if (max_segment_length < 0) return fail(hipErrorInvalidValue, "segsort_sort_hinted: negative max_segment_length");
+    struct Scope {
+        int prev;
+        explicit Scope(int m) : prev(t_max_len) { t_max_len = m; }
+        ~Scope() { t_max_len = prev; }
+    } scope(max_segment_length);
+    return segsort_sort(d_temp, temp_bytes, d_keys_in, d_keys_out, d_vals_in, d_vals_out, num_items, num_segments,
+                        d_begin_offsets, d_end_offsets, begin_bit, end_bit, key_type, value_bytes, descending, algo,
+                        stream);
+}
 int segsort_sort_noalloc(const void* d_keys_in, void* d_keys_out, const void* d_vals_in, void* d_vals_out,
                          int num_items, int num_segments, const int* d_begin_offsets, const int* d_end_offsets,
                          int begin_bit, int end_bit, int key_type, int value_bytes, int descending, void* stream) {
@@ -1224,10 +1281,16 @@ int segsort_get_tuning(int key_width, int value_bytes, int* wave_cap, int* tile_
 
 int segsort_lookup_tuning(int num_items, int num_segments, int key_width, int value_bytes, int* wave_cap,
                           int* tile_enable, int* grid_mult, int* concurrent, int* from_table) {
-    if (tuning_slot(key_width, value_bytes) < 0 || num_items < 0 || num_segments < 0)
+    return segsort_lookup_tuning_hinted(num_items, num_segments, 0, key_width, value_bytes, wave_cap, tile_enable,
+                                        grid_mult, concurrent, from_table);
+}
+int segsort_lookup_tuning_hinted(int num_items, int num_segments, int max_segment_length, int key_width,
+                                 int value_bytes, int* wave_cap, int* tile_enable, int* grid_mult, int* concurrent,
+                                 int* from_table) {
+    if (tuning_slot(key_width, value_bytes) < 0 || num_items < 0 || num_segments < 0 || max_segment_length < 0)
         return fail(hipErrorInvalidValue, "segsort_lookup_tuning: bad arguments");
     bool ovr = false;
-    const Tuning t = choose_tuning(key_width, value_bytes, num_items, num_segments, &ovr);
+    const Tuning t = choose_tuning(key_width, value_bytes, num_items, num_segments, max_segment_length, &ovr);
     if (wave_cap) *wave_cap = t.wave_cap;
     if (tile_enable) *tile_enable = t.tile_enable;
     if (grid_mult) *grid_mult = t.grid_mult;

@@ -19,6 +19,7 @@ struct Options {
     bool values = false;   // also print the sorted values, as a second line in the keys' format
     int executions = 1;    // timed repetitions (-DEXECUTIONS)
     int algo = 0;          // segsort_algo
+    bool bound = false;    // pass the longest segment (known from the host offsets) as the length bound
 };
 
 inline Options parse(int argc, char** argv, int default_algo) {
@@ -27,6 +28,7 @@ inline Options parse(int argc, char** argv, int default_algo) {
     for (int i = 1; i < argc; ++i) {
         if (!strcmp(argv[i], "--time")) o.time = true;
         else if (!strcmp(argv[i], "--values")) o.values = true;
+        else if (!strcmp(argv[i], "--bound")) o.bound = true;
         else if (!strcmp(argv[i], "--executions") && i + 1 < argc) o.executions = atoi(argv[++i]);
         else if (!strcmp(argv[i], "--algo") && i + 1 < argc) {
             const char* a = argv[++i];

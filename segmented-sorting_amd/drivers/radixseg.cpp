@@ -5,6 +5,8 @@
 #include "driver_io.hpp"
 #include "segsort.hpp"
 
+#include <algorithm>
+
 int main(int argc, char** argv) {
     drv::Options o = drv::parse(argc, argv, SEGSORT_ALGO_BATCHED);
     std::vector<int> seg;
@@ -14,6 +16,10 @@ int main(int argc, char** argv) {
         return EXIT_FAILURE;
     }
     const int S = (int)seg.size() - 1, N = (int)keys.size();
+    // --bound: the longest segment, which the host offsets give for free, as the
+    // call's length bound (segsort_sort_hinted); 0 = none, cub's call
+    int bound = 0;
+    for (int i = 0; o.bound && i < S; ++i) bound = std::max(bound, seg[(size_t)i + 1] - seg[(size_t)i]);
     std::vector<unsigned> vals((size_t)N);
     for (int i = 0; i < N; ++i) vals[(size_t)i] = (unsigned)i;
     unsigned *d_vec, *d_vec_out, *d_val, *d_val_out;
@@ -37,14 +43,14 @@ int main(int argc, char** argv) {
         if (!d_temp) {
             drv::check(segsort::DeviceSegmentedRadixSort::SortPairs(nullptr, temp_bytes, d_vec, d_vec_out, d_val,
                                                                     d_val_out, N, S, d_seg, d_seg + 1, 0, 32, 0,
-                                                                    false, o.algo),
+                                                                    false, o.algo, bound),
                        "temp query");
             drv::check(hipMalloc(&d_temp, temp_bytes), "hipMalloc temp");
         }
         drv::check(hipEventRecord(start, 0), "hipEventRecord");
         drv::check(segsort::DeviceSegmentedRadixSort::SortPairs(d_temp, temp_bytes, d_vec, d_vec_out, d_val,
                                                                 d_val_out, N, S, d_seg, d_seg + 1, 0, 32, 0, false,
-                                                                o.algo),
+                                                                o.algo, bound),
                    "SortPairs");
         drv::check(hipEventRecord(stop, 0), "hipEventRecord");
         drv::check(hipEventSynchronize(stop), "hipEventSynchronize");

@@ -16,6 +16,16 @@ def test_signature_and_lookup(pkg, tmp_path):
     assert at.lookup("k4v4-n23-m7", table)["tile_enable"] == 12        # nearest of the same widths
     assert at.lookup("k4v4-n20-m3", table)["wave_cap"] == 512
     assert at.lookup("k8v8-n24-m8", table) == at.DEFAULT                 # no entry for these widths
+    # calls with a length bound: signature feature ceil(log2 bound), bounded entries first
+    assert at.signature(1 << 24, 1 << 16, 4, 4, 16384) == "k4v4-n24-m8-x14"
+    assert at.signature(1 << 24, 1 << 16, 4, 4, 16385) == "k4v4-n24-m8-x15"
+    assert at.signature(1 << 24, 1 << 16, 4, 4, 1) == "k4v4-n24-m8-x0"
+    bt = dict(table, **{"k4v4-n24-m8-x8": {"wave_cap": 256, "tile_enable": 3, "grid_mult": 6, "concurrent": 0},
+                        "k4v4-n24-m8-x17": {"wave_cap": 512, "tile_enable": 11, "grid_mult": 4, "concurrent": 1}})
+    assert at.lookup("k4v4-n24-m8-x8", bt)["grid_mult"] == 6            # exact bounded entry
+    assert at.lookup("k4v4-n24-m8-x16", bt)["grid_mult"] == 4           # nearest bounded entry
+    assert at.lookup("k4v4-n24-m8", bt)["wave_cap"] == 128              # unbounded calls: unbounded entries only
+    assert at.lookup("k4v4-n20-m3-x5", table)["wave_cap"] == 512        # no bounded entry: unbounded ones
     p = tmp_path / "t.json"
     at.store("k4v0-n20-m10", at.DEFAULT, 1.5, path=p, note="x")
     assert json.loads(p.read_text())["k4v0-n20-m10"]["ms"] == 1.5
@@ -48,9 +58,10 @@ def test_library_table_matches_tuning_json(pkg):
                                                              (1 << 26, 1 << 15), (16809984, 4096)]
     for kw, vb in ((4, 4), (4, 0), (8, 4), (4, 8), (8, 8), (8, 0)):
         for n, s in shapes:
-            want = pkg.autotune.lookup(pkg.autotune.signature(n, s, kw, vb), table)
-            got = pkg.lookup_tuning(n, s, kw, vb)
-            assert got["from_table"] and {k: got[k] for k in keys} == want, (kw, vb, n, s)
+            for mx in (0, 1, 32, 256, 4096, 16384, 16385, 1 << 17, n):
+                want = pkg.autotune.lookup(pkg.autotune.signature(n, s, kw, vb, mx), table)
+                got = pkg.lookup_tuning(n, s, kw, vb, mx)
+                assert got["from_table"] and {k: got[k] for k in keys} == want, (kw, vb, n, s, mx)
 
 
 @pytest.mark.gpu

@@ -24,9 +24,9 @@ def host(t, dtype):
 
 
 def gpu_sort(pkg, torch, keys, vals=None, offsets=None, begin=None, end=None, algo="batched", begin_bit=0,
-             end_bit=None, descending=False):
-    """Run segsort_sort through the C-ABI; outputs start filled with FILL bytes so that
-    any write outside the segments shows."""
+             end_bit=None, descending=False, max_segment_length=0):
+    """Run segsort_sort (segsort_sort_hinted with a length bound) through the C-ABI;
+    outputs start filled with FILL bytes so that any write outside the segments shows."""
     if begin is None:
         offsets = np.asarray(offsets, dtype=np.int32)
         begin, end = offsets[:-1], offsets[1:]
@@ -43,11 +43,12 @@ def gpu_sort(pkg, torch, keys, vals=None, offsets=None, begin=None, end=None, al
     D = pkg.DeviceSegmentedRadixSort
     eb = end_bit if end_bit is not None else keys.dtype.itemsize * 8
     rc = D._call(None, tb, None, None, None, None, n, s, None, None, begin_bit, eb, None, int(descending), kt, vb,
-                 algo)
+                 algo, max_segment_length)
     assert rc == 0, pkg.last_error()
     temp = torch.empty(max(tb[0], 1), dtype=torch.uint8, device="cuda")
     tb = [temp.numel()]
-    rc = D._call(temp, tb, kin, kout, vin, vout, n, s, bd, ed, begin_bit, eb, None, int(descending), kt, vb, algo)
+    rc = D._call(temp, tb, kin, kout, vin, vout, n, s, bd, ed, begin_bit, eb, None, int(descending), kt, vb, algo,
+                 max_segment_length)
     assert rc == 0, pkg.last_error()
     torch.cuda.synchronize()
     ko = host(kout, keys.dtype)
@@ -63,14 +64,15 @@ def oracle_sort(orc, keys, vals=None, offsets=None, begin=None, end=None, begin_
                         descending=descending, out_keys=fill_k, out_vals=fill_v)
 
 
-def check(pkg, torch, orc, keys, vals, offsets=None, begin=None, end=None, algos=ALGOS, **kw):
+def check(pkg, torch, orc, keys, vals, offsets=None, begin=None, end=None, algos=ALGOS, max_segment_length=0, **kw):
     ek, ev = oracle_sort(orc, keys, vals, offsets, begin, end, **kw)
     for algo in algos:
         if algo == "fixpass" and (keys.dtype.itemsize == 8 or begin is not None):
             continue
-        print(f"[case] n={keys.size} algo={algo} keys={keys.dtype} vals={None if vals is None else vals.dtype} {kw}",
-              flush=True)
-        gk, gv = gpu_sort(pkg, torch, keys, vals, offsets, begin, end, algo=algo, **kw)
+        print(f"[case] n={keys.size} algo={algo} keys={keys.dtype} vals={None if vals is None else vals.dtype} {kw} "
+              f"bound={max_segment_length}", flush=True)
+        gk, gv = gpu_sort(pkg, torch, keys, vals, offsets, begin, end, algo=algo,
+                          max_segment_length=max_segment_length, **kw)
         bad = np.flatnonzero(gk.view(np.uint8) != ek.view(np.uint8))
         assert np.array_equal(gk.view(np.uint8), ek.view(np.uint8)), (algo, bad[:10])
         if vals is not None:
@@ -560,3 +562,53 @@ def test_tuning_variants(pkg, torch_cuda, orc, tuning, concurrent):
     finally:
         for kw, vb in befores:
             pkg.reset_tuning(kw, vb)
+
+
+def test_length_bound_equal_segments(pkg, torch_cuda, orc):
+    """segsort_sort_hinted with a true bound at most the largest tile: the batched form
+    launches no MSD kernels (2^24 pairs in 2^16 segments of 256, and 2^20 keys in 1024
+    segments of 1024, keys only)."""
+    keys, vals, off = pkg.gen.equal_segments(1 << 24, 1 << 16, seed=0x5E650A7 + 1)
+    check(pkg, torch_cuda, orc, keys, vals, off, algos=("batched", "plain"), max_segment_length=256)
+    keys, _, off = pkg.gen.equal_segments(1 << 20, 1024, seed=0x5E650A7 + 0, key_bits=31, pairs=False)
+    check(pkg, torch_cuda, orc, keys, None, off, algos=("batched",), max_segment_length=1024)
+
+
+def test_length_bound_ragged_true_bound(pkg, torch_cuda, orc):
+    """The ragged workload at 2^24 with its longest segment as the bound (above the
+    largest tile: the MSD tier runs, the tuning is the bounded entry's)."""
+    keys, vals, off = pkg.gen.ragged_lognormal(0x5E650A7 + 2, dup_heavy=False)
+    mx = int(np.diff(off).max())
+    check(pkg, torch_cuda, orc, keys, vals, off, algos=("batched",), max_segment_length=mx)
+
+
+@pytest.mark.parametrize("case", ["u32-pairs", "f32-desc-pairs", "u64-keys", "gaps"])
+def test_length_bound_broken(pkg, torch_cuda, orc, case):
+    """A wrong bound (segments of up to ~40K items under a bound of 1000) is still
+    sorted bit-exactly: the segments above the largest tile go to the one-workgroup
+    fallback LSD instead of the MSD tier."""
+    rng = np.random.default_rng(77)
+    lengths = np.concatenate([rng.integers(0, 900, 3000), [40000, 17000, 16385, 70000, 0, 1],
+                              rng.integers(0, 300, 500)]).astype(np.int64)
+    rng.shuffle(lengths)
+    off = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int32)
+    n = int(off[-1])
+    vals = np.arange(n, dtype=np.uint32)
+    if case == "u32-pairs":
+        keys = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+        check(pkg, torch_cuda, orc, keys, vals, off, algos=("batched",), max_segment_length=1000)
+    elif case == "f32-desc-pairs":
+        keys = rng.standard_normal(n).astype(np.float32)
+        check(pkg, torch_cuda, orc, keys, vals, off, algos=("batched",), max_segment_length=1000, descending=True)
+    elif case == "u64-keys":
+        keys = rng.integers(0, 1 << 63, n, dtype=np.uint64)
+        check(pkg, torch_cuda, orc, keys, None, off, algos=("batched",), max_segment_length=1000)
+    else:
+        # gaps between segments and unordered segments: begin/end arrays
+        begin = off[:-1].copy()
+        end = off[1:].copy()
+        end = np.maximum(begin, end - 3).astype(np.int32)
+        perm = rng.permutation(begin.size)
+        keys = rng.integers(0, 256, n, dtype=np.uint32)
+        check(pkg, torch_cuda, orc, keys, vals, begin=begin[perm], end=end[perm], algos=("batched",),
+              max_segment_length=1000)

@@ -178,6 +178,22 @@ def tune(pkg, torch, keys, vals, offsets, steps=5, warmup=2, verbose=True, quick
     return best, results
 
 
+def largest_tile(caps, tuning: dict) -> int:
+    """Longest range the tuning's classes hold (above it: the MSD tier); caps as
+    segsort_tile_caps gives them for the key and value widths."""
+    t = [c for i, c in enumerate(caps) if tuning["tile_enable"] & (1 << i)]
+    return max(t + [min(512, max(2, tuning["wave_cap"]))])
+
+
+def canonical(tuning: dict, caps, max_len: int) -> dict:
+    """The tuning as stored: `concurrent` only matters when the MSD tier runs, which a
+    call bounded by the largest tile never launches; there it is stored as 0."""
+    t = dict(tuning)
+    if 0 < max_len <= largest_tile(caps, t):
+        t["concurrent"] = 0
+    return t
+
+
 def store(sig: str, tuning: dict, ms: float, path: Path = TABLE, note: str = "") -> None:
     table = load_table(path)
     table[sig] = dict(tuning, ms=round(ms, 4), note=note)
@@ -225,11 +241,11 @@ def main(argv=None):
         keys, vals, off = bench.make_workload(pkg, wl)
         kdt = {"u": {4: torch.uint32, 8: torch.uint64}, "f": {4: torch.float32, 8: torch.float64},
                "i": {4: torch.int32, 8: torch.int64}}[keys.dtype.kind][keys.dtype.itemsize]
-        dev = (bench.to_dev(torch, keys).view(kdt), bench.to_dev(torch, vals).view(torch.uint32),
-               torch.from_numpy(off).cuda())
+        dev = (bench.to_dev(torch, keys).view(kdt),
+               None if vals is None else bench.to_dev(torch, vals).view(torch.uint32), torch.from_numpy(off).cuda())
         mx = int((off[1:] - off[:-1]).max()) if args.bounded and off.size > 1 else 0
         best, results = tune(pkg, torch, *dev, steps=args.steps, verbose=False, max_len=mx)
-        sig = signature(keys.size, off.size - 1, keys.dtype.itemsize, vals.dtype.itemsize, mx)
+        sig = signature(keys.size, off.size - 1, keys.dtype.itemsize, 0 if vals is None else vals.dtype.itemsize, mx)
         by_sig.setdefault(sig, []).append((wl, dev, results, mx))
         print(json.dumps({"workload": wl, "signature": sig, "screen_best": best[0], "ms": round(best[1], 4)}),
               flush=True)
@@ -254,7 +270,9 @@ def main(argv=None):
         score = _score(refined)
         cand = min((c for c, v in score.items() if v is not None), key=lambda c: score[c])
         ms = sum(r[cand] for r in refined) / len(refined)
-        store(sig, json.loads(cand), ms, note="tuned on " + ", ".join(r[0] for r in runs))
+        head = sig.split("-")[0]
+        caps = pkg.tile_caps(int(head[1:].split("v")[0]), int(head.split("v")[1]))
+        store(sig, canonical(json.loads(cand), caps, runs[0][3]), ms, note="tuned on " + ", ".join(r[0] for r in runs))
         print(json.dumps({"signature": sig, "chosen": json.loads(cand), "relative": round(score[cand], 4)}),
               flush=True)
 

@@ -43,10 +43,6 @@ int ok() {
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-// Workgroups of the second multi-workgroup MSD level: it has work only for a
-// child of a long segment that holds more than MSD_BIG_TILES tiles (skewed
-// keys), and a small grid keeps its launches short when it has none.
-constexpr int MSD_LEVEL1_GRID = 128;
 
 // Workgroups of the oversize fallback (k_seg_lsd over the segments that break
 // a caller's length bound): empty in every correct call, so small.
@@ -731,8 +727,8 @@ struct Launcher {
         }
         if (levels > 0) {
             // level 0 over the long segments, level 1 over their children of more
-            // than MSD_BIG_TILES tiles (launched only when such a child can exist;
-            // a smaller grid, most calls find it empty), then the depth-first tier
+            // than MSD_BIG_TILES tiles (launched only when such a child can exist),
+            // then the depth-first tier
             auto kh = k_large_hist<C::NTC, C::IPTC, U, V, HAS_V>;
             auto ks = k_large_scatter<C::NTC, C::IPTC, U, V, HAS_V>;
             auto kl = k_large_local<C::NT3, C::IPT3, U, V, HAS_V>;
@@ -750,11 +746,14 @@ struct Launcher {
             SEGSORT_LAUNCH("large_scatter", st, 100 + MSD_LEVEL0, PAIR_BYTES, 0.0,
                            (ks<<<gs, C::NTC, 0, st>>>(b, sc, MSD_LEVEL0, km, p.cc, p.tile_max, big)));
             if (levels > 1) {
-                const int g1 = std::min(gs, MSD_LEVEL1_GRID);
+                // the full grids: keys skewed in their top digit leave most of a
+                // segment to this level (one 2^24 segment of N(0,1) floats:
+                // 0.625 -> 0.533 ms against half the scatter grid), and an empty
+                // launch costs the same either way
                 SEGSORT_LAUNCH("large_hist", st, 100 + MSD_LEVEL1, (double)sizeof(U), 0.0,
-                               (kh<<<g1, C::NTC, 0, st>>>(b, sc, MSD_LEVEL1, km)));
+                               (kh<<<gh, C::NTC, 0, st>>>(b, sc, MSD_LEVEL1, km)));
                 SEGSORT_LAUNCH("large_scatter", st, 100 + MSD_LEVEL1, PAIR_BYTES, 0.0,
-                               (ks<<<g1, C::NTC, 0, st>>>(b, sc, MSD_LEVEL1, km, p.cc, p.tile_max, big)));
+                               (ks<<<gs, C::NTC, 0, st>>>(b, sc, MSD_LEVEL1, km, p.cc, p.tile_max, big)));
             }
             SEGSORT_LAUNCH("large_local", st, 100 + MSD_LOCAL, PAIR_BYTES + sizeof(U), 0.0,
                            (kl<<<gl, C::NT3, 0, st>>>(b, sc, MSD_LOCAL, km, p.cc, p.tile_max)));

@@ -653,14 +653,13 @@ __global__ __launch_bounds__(NT) void k_large_hist(Bufs b, Sched sc, int level, 
     }
 }
 
-// Items of digit d in chunk gc at the level's position: the chunk's histogram
-// when that is its own top varying position, else all of its n items or none.
-// The histogram word is read whether or not it is used (it may be stale when
-// the chunk's top is lower), so that both loads of a chunk are in flight at once.
-__device__ __forceinline__ uint32_t chunk_digit_count(const Sched& sc, int level, size_t gc, int pstar, int shift,
-                                                      unsigned long long dmask, uint32_t d, int n) {
-    const uint32_t h = sc.hist[level][gc * RADIX + d];
-    const ChunkInfo ci = sc.cinfo[level][gc];
+// Items of digit d in a chunk (record ci, histogram word h of digit d) at the
+// level's position: the chunk's histogram when that is its own top varying
+// position, else all of its n items or none. The caller reads the histogram
+// word whether or not it is used (it may be stale when the chunk's top is
+// lower), so that both loads of a chunk are in flight at once.
+__device__ __forceinline__ uint32_t chunk_count_of(const ChunkInfo& ci, uint32_t h, int pstar, int shift,
+                                                   unsigned long long dmask, uint32_t d, int n) {
     if (ci.lp == pstar) return h;
     return (uint32_t)((ci.kand >> shift) & dmask) == d ? (uint32_t)n : 0u;
 }
@@ -741,16 +740,35 @@ __global__ __launch_bounds__(NT) void k_large_scatter(Bufs b, Sched sc, int leve
         const int bits = (it.hi_bit - shift) < RADIX_BITS ? (it.hi_bit - shift) : RADIX_BITS;
         const unsigned long long dmask = (1ull << bits) - 1;
         const bool final_ = (var & ((1ull << shift) - 1)) == 0;
-        // digit d's total over the range and over the chunks before this one
+        // digit d's total over the range and over the chunks before this one:
+        // the group's share of the chunk records is read XB chunks at a time
+        // (all loads of a batch in flight together; a range of 256 chunks costs
+        // its groups 8 round trips instead of 64), the records through the
+        // scalar unit (the share is wave-uniform)
         uint32_t tot = 0, pre = 0;
         {
-            const int x0 = (it.nch * grp) / G, x1 = (it.nch * (grp + 1)) / G;
-#pragma unroll 4
-            for (int x = x0; x < x1; ++x) {
-                const uint32_t k = chunk_digit_count(sc, level, (size_t)(it.chunk_base + x), pstar, shift, dmask,
-                                                     (uint32_t)d, chunk_items(it, x));
-                tot += k;
-                pre += x < c ? k : 0u;
+            constexpr int XB = 8;
+            const int x0 = __builtin_amdgcn_readfirstlane((it.nch * grp) / G);
+            const int x1 = __builtin_amdgcn_readfirstlane((it.nch * (grp + 1)) / G);
+            for (int xb = x0; xb < x1; xb += XB) {
+                uint32_t h[XB];
+                ChunkInfo ci[XB];
+#pragma unroll
+                for (int q = 0; q < XB; ++q) {
+                    const size_t gc = (size_t)it.chunk_base + (size_t)(xb + q < x1 ? xb + q : x1 - 1);
+                    h[q] = sc.hist[level][gc * RADIX + d];
+                    ci[q] = sc.cinfo[level][gc];
+                }
+#pragma unroll
+                for (int q = 0; q < XB; ++q) {
+                    const int x = xb + q;
+                    if (x < x1) {
+                        const uint32_t k = chunk_count_of(ci[q], h[q], pstar, shift, dmask, (uint32_t)d,
+                                                          chunk_items(it, x));
+                        tot += k;
+                        pre += x < c ? k : 0u;
+                    }
+                }
             }
         }
         uint32_t* part = sm.wcnt;  // free until the partition's first ranking pass

@@ -499,12 +499,18 @@ __global__ __launch_bounds__(256, 8) void k_wave_tile_tier(Bufs b, Sched sc, int
 
 // ---- large tier (MSD partition over many workgroups) --------------------------
 
+// The last record whose first chunk is at or before g (chunk_base grows with the
+// record index). Wave-parallel: each round the 64 lanes test 64 evenly spaced
+// records of the remaining span at once, so a level of up to 4096 ranges takes
+// two dependent loads (a binary search took log2(count), each a round trip).
 __device__ __forceinline__ int find_item(const LargeItem* items, int count, int g) {
-    int lo = 0, hi = count - 1;
+    int lo = 0, hi = count - 1;  // items[lo].chunk_base <= g throughout
     while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (items[mid].chunk_base <= g) lo = mid;
-        else hi = mid - 1;
+        const int step = (hi - lo + WAVE) / WAVE;  // ceil((hi - lo + 1) / WAVE)
+        const int idx = lo + (int)lane_id() * step;
+        const uint64_t ok = __ballot(idx <= hi && items[idx].chunk_base <= g);
+        lo += (63 - __builtin_clzll(ok)) * step;  // lane 0 (idx = lo) always passes
+        hi = min(lo + step - 1, hi);
     }
     return lo;
 }

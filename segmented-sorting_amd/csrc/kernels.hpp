@@ -13,6 +13,14 @@ constexpr int MAX_TILE_CLASSES = 9;  // wave-group classes, the wave tier, the L
 // (so the binned ranges can be sorted while the MSD tier still runs).
 constexpr int MAX_QUEUES = 2 * MAX_TILE_CLASSES;
 constexpr int MAX_LEVELS = 8;        // MSD levels of the large tier (64-bit keys)
+// Length buckets of the binned ranges of the large LDS tile classes: bucket b
+// of class c holds the ranges whose length falls in the b-th of LPT_BUCKETS
+// equal parts of the class's length span, and the class kernel takes the
+// buckets longest first. A persistent grid over a class of a few hundred or
+// thousand ranges is a list schedule, and taking the longest ranges first
+// keeps the last ranges short: the kernel no longer ends on one long range
+// that started late while most CUs idle.
+constexpr int LPT_BUCKETS = 8;
 
 // Device-resident counters of one call; cleared by k_init at the start.
 struct Counters {
@@ -22,6 +30,7 @@ struct Counters {
     unsigned long long tile_items[MAX_QUEUES];  // elements queued per queue
     unsigned long long level_items[MAX_LEVELS + 1];   // elements queued per MSD level
     uint32_t claim;                                   // next range of the depth-first tier
+    uint32_t bcount[MAX_TILE_CLASSES][LPT_BUCKETS];   // binned ranges per length bucket (bucketed classes)
 };
 
 // MSD levels of the batched form: two multi-workgroup levels (the long
@@ -63,7 +72,53 @@ struct Sched {
     uint32_t lq_cap;
     uint32_t chunk_cap;
     int child_q;  // queue of class c's MSD children: child_q + c (0: the binned ranges' queue)
+    // Bucketed classes (nullptr: none): class c's binned ranges in length bucket
+    // b are at bq[c][b * bcap[c] + i]; its queue tq[c] then holds the MSD
+    // children only.
+    TileItem* bq[MAX_TILE_CLASSES];
+    uint32_t bcap[MAX_TILE_CLASSES];
 };
+
+// Length bucket of a range of n items in class c (cap[c-1] < n <= cap[c]).
+__device__ __forceinline__ int lpt_bucket(const ClassCaps& cc, int c, int n) {
+    const int lo = c > 0 ? cc.cap[c - 1] + 1 : 2;
+    const int b = (int)(((long long)(n - lo) * LPT_BUCKETS) / (cc.cap[c] - lo + 1));
+    return b < 0 ? 0 : b >= LPT_BUCKETS ? LPT_BUCKETS - 1 : b;
+}
+
+// The ranges a bucketed class's kernel takes before its queue tq[cls]: the
+// bucket counts (clamped to the capacity), wave-uniform, and their total.
+struct BucketView {
+    uint32_t cnt[LPT_BUCKETS];
+    int total;
+};
+
+__device__ __forceinline__ BucketView bucket_view(const Sched& sc, int cls) {
+    BucketView v;
+    v.total = 0;
+#pragma unroll
+    for (int b = 0; b < LPT_BUCKETS; ++b) v.cnt[b] = 0;
+    if (cls >= MAX_TILE_CLASSES || sc.bq[cls] == nullptr) return v;  // queue cls alone (MSD children, plain form)
+    const uint32_t cap = sc.bcap[cls];
+#pragma unroll
+    for (int b = 0; b < LPT_BUCKETS; ++b) {
+        const uint32_t c = sc.ctr->bcount[cls][b];
+        v.cnt[b] = (uint32_t)__builtin_amdgcn_readfirstlane((int)(c < cap ? c : cap));
+        v.total += (int)v.cnt[b];
+    }
+    return v;
+}
+
+// Entry i < v.total of a bucketed class, the longest bucket first.
+__device__ __forceinline__ const TileItem* bucket_entry(const Sched& sc, const BucketView& v, int cls, int i) {
+    int b = LPT_BUCKETS - 1;
+    uint32_t r = (uint32_t)i;
+    while (b > 0 && r >= v.cnt[b]) {
+        r -= v.cnt[b];
+        --b;
+    }
+    return sc.bq[cls] + (size_t)b * sc.bcap[cls] + r;
+}
 
 // Every lane of a wave reads the same count and the same queue entry: taken
 // into scalar registers, the range's bounds and its buffer pointers are
@@ -210,13 +265,17 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
     __shared__ uint32_t lcount[MAX_TILE_CLASSES];
     __shared__ uint32_t lbase[MAX_TILE_CLASSES];
     __shared__ unsigned long long litems[MAX_TILE_CLASSES];
+    __shared__ uint32_t lbcount[MAX_TILE_CLASSES][LPT_BUCKETS];  // bucketed classes: per length bucket
+    __shared__ uint32_t lbbase[MAX_TILE_CLASSES][LPT_BUCKETS];
     if (threadIdx.x < MAX_TILE_CLASSES) {
         lcount[threadIdx.x] = 0;
         litems[threadIdx.x] = 0;
     }
+    if (threadIdx.x < MAX_TILE_CLASSES * LPT_BUCKETS) lbcount[threadIdx.x / LPT_BUCKETS][threadIdx.x % LPT_BUCKETS] = 0;
     __syncthreads();
     int beg[BIN_SPT], n[BIN_SPT], cls[BIN_SPT];
     uint32_t slot[BIN_SPT];
+    int bkt[BIN_SPT];  // length bucket of a bucketed class's range, else -1
     const int s0 = blockIdx.x * (BIN_NT * spt) + threadIdx.x;
     // every offset load issued before any is used
 #pragma unroll
@@ -239,6 +298,14 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
         cls[j] = n[j] >= 2 ? class_of(cc, n[j]) : -2;
         if (cls[j] == -1 && large_to_level0 == 1) push_large(sc, MSD_LEVEL0, beg[j], n[j], BUF_IN, end_bit, true);
         if (cls[j] == -1 && large_to_level0 == 2) push_oversize(sc, beg[j], n[j]);
+        // ranges of a bucketed class take a slot of their length bucket (an LDS
+        // atomic each: they are the few long segments) and skip the ballots below
+        bkt[j] = -1;
+        if (cls[j] >= 0 && sc.bq[cls[j]] != nullptr) {
+            bkt[j] = lpt_bucket(cc, cls[j], n[j]);
+            slot[j] = atomicAdd(&lbcount[cls[j]][bkt[j]], 1u);
+            atomicAdd(&litems[cls[j]], (unsigned long long)n[j]);
+        }
     }
     // per class: the wave's slots counted with ballots in registers, then one LDS
     // atomic per class and wave for the wave's base inside the block
@@ -250,8 +317,9 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
         uint32_t run = 0, items = 0;
 #pragma unroll
         for (int j = 0; j < BIN_SPT; ++j) {
-            const uint64_t m = __ballot(cls[j] == c);
-            if (cls[j] == c) {
+            const bool mine = cls[j] == c && bkt[j] < 0;
+            const uint64_t m = __ballot(mine);
+            if (mine) {
                 slot[j] = run + lanes_below(m);
                 items += (uint32_t)n[j];
             }
@@ -269,7 +337,7 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
         const uint32_t wb = (uint32_t)__builtin_amdgcn_readfirstlane((int)wbase[c]);
 #pragma unroll
         for (int j = 0; j < BIN_SPT; ++j)
-            if (cls[j] == c) slot[j] += wb;
+            if (cls[j] == c && bkt[j] < 0) slot[j] += wb;
     }
     __syncthreads();
     if (threadIdx.x < (unsigned)cc.nclass) {
@@ -277,12 +345,20 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
         lbase[c] = lcount[c] ? atomicAdd(&sc.ctr->tile_count[c], lcount[c]) : 0u;
         if (litems[c]) atomicAdd(&sc.ctr->tile_items[c], litems[c]);
     }
+    if (threadIdx.x < (unsigned)(cc.nclass * LPT_BUCKETS)) {
+        const int c = threadIdx.x / LPT_BUCKETS, bb = threadIdx.x % LPT_BUCKETS;
+        const uint32_t k = lbcount[c][bb];
+        lbbase[c][bb] = k ? atomicAdd(&sc.ctr->bcount[c][bb], k) : 0u;
+    }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < BIN_SPT; ++j) {
-        if (cls[j] >= 0) {
+        if (cls[j] >= 0 && bkt[j] < 0) {
             const uint32_t g = lbase[cls[j]] + slot[j];
             if (g < sc.tq_cap[cls[j]]) sc.tq[cls[j]][g] = make_ti(beg[j], n[j], BUF_IN);
+        } else if (cls[j] >= 0) {
+            const uint32_t g = lbbase[cls[j]][bkt[j]] + slot[j];
+            if (g < sc.bcap[cls[j]]) sc.bq[cls[j]][(size_t)bkt[j] * sc.bcap[cls[j]] + g] = make_ti(beg[j], n[j], BUF_IN);
         }
     }
 }
@@ -433,8 +509,11 @@ __global__ __launch_bounds__(NT, 4) void k_tile_tier(Bufs b, Sched sc, int cls, 
             typename T2::Smem half;
         } sm;
         const int count = tile_count(sc, cls);
-        for (int i = blockIdx.x; i < count; i += persistent ? (int)gridDim.x : count) {
-            const TileItem cur = queue_item(sc.tq[cls], i);
+        const BucketView bv = bucket_view(sc, cls);  // the binned ranges, longest first, then the queue
+        const int total = bv.total + count;
+        for (int i = blockIdx.x; i < total; i += persistent ? (int)gridDim.x : total) {
+            const TileItem cur = i < bv.total ? queue_item(bucket_entry(sc, bv, cls, i), 0)
+                                              : queue_item(sc.tq[cls], i - bv.total);
             const int src = ti_src(cur);
             const int n = ti_n(cur);
             const U* ks = (const U*)b.k[src];

@@ -453,6 +453,7 @@ Plan make_plan(const Tuning& tu) {
 
 struct Layout {
     size_t ctr, tq[MAX_TILE_CLASSES], lq[MAX_LEVELS + 1], hist[2], cinfo[2], tk[2], tv[2];
+    size_t bq[MAX_TILE_CLASSES], bcap[MAX_TILE_CLASSES];  // length buckets of the large tile classes
     size_t total;
     size_t tq_cap[MAX_TILE_CLASSES];
     size_t lq_cap;
@@ -481,6 +482,18 @@ Layout make_layout(const Plan& p, int num_items, int num_segments, int levels, b
     for (int c = 0; c < p.nclass; ++c) {
         L.tq_cap[c] = S + bucket_bound;
         L.tq[c] = take(L.tq_cap[c] * sizeof(TileItem));
+    }
+    // the batched form's large tile classes (16 items per thread) take their
+    // binned ranges longest first: LPT_BUCKETS length buckets, each able to hold
+    // every binned range of the class (at most min(S, N / shortest length)).
+    // Measured (2^24 items in 2^16 lognormal segments): 0.217 -> 0.200 ms, the
+    // 1024x16 and 256x16 kernels no longer ending on long ranges started last;
+    // equal segments (one bucket) within 1-3%.
+    for (int c = 0; c < p.nclass && !plain; ++c) {
+        if (p.kind[c] < KIND_TILE + 1 || p.kind[c] > KIND_TILE + 3) continue;
+        const size_t lo = c > 0 ? (size_t)p.cc.cap[c - 1] + 1 : 2;
+        L.bcap[c] = std::min(S, N / lo + 1);
+        L.bq[c] = take(L.bcap[c] * LPT_BUCKETS * sizeof(TileItem));
     }
     if (plain) {
         L.lq_cap = std::max<size_t>(std::min(S, L.lq_cap), 1);
@@ -628,6 +641,10 @@ struct Launcher {
         }
         sc.chunk = C::CHUNK;
         for (int c = 0; c < p.nclass; ++c) sc.tq_cap[c] = (uint32_t)std::min<size_t>(L.tq_cap[c], 0xFFFFFFFFu);
+        for (int c = 0; c < p.nclass; ++c) {
+            sc.bq[c] = L.bcap[c] ? (TileItem*)(base + L.bq[c]) : nullptr;
+            sc.bcap[c] = (uint32_t)L.bcap[c];
+        }
         if (split) {
             sc.child_q = MAX_TILE_CLASSES;
             for (int c = 0; c < p.nclass; ++c) {

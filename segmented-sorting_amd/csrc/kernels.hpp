@@ -397,11 +397,14 @@ __global__ void k_bin_large_ids(const int* __restrict__ d_begin, const int* __re
 
 // Wave tier: each wave sorts one queued range of n <= MAXN items (compare-count
 // up to rank_max items, a register bitonic network above). For MAXN = 512 the
-// per-wave LDS area allows 6 waves per SIMD and the register budget is held to
-// the same; ranges of at most 256 items need half the area and fewer
-// registers, and aim at 8.
+// per-wave LDS area allows 7 waves per SIMD with 32-bit keys (6 with 64-bit
+// keys, whose bucket path also keeps a position array) and the register budget
+// is held to the same (72 VGPRs spill about 23 per lane with 32-bit keys, and
+// still measured faster than 6 waves: one 2^24 segment 0.373 -> 0.366 ms);
+// ranges of at most 256 items need half the area and fewer registers, and aim
+// at 8.
 template <typename U, typename V, bool HAS_V, int MAXN>
-__global__ __launch_bounds__(256, MAXN <= 256 ? 8 : 6) void k_wave_tier(Bufs b, Sched sc, int cls, KeyMode km, int persistent,
+__global__ __launch_bounds__(256, MAXN <= 256 ? 8 : sizeof(U) == 4 ? 7 : 6) void k_wave_tier(Bufs b, Sched sc, int cls, KeyMode km, int persistent,
                                                    int rank_max) {
     const int waves_per_block = blockDim.x / WAVE;
     const int wid = __builtin_amdgcn_readfirstlane(blockIdx.x * waves_per_block + (threadIdx.x >> 6));

@@ -1395,7 +1395,7 @@ template <typename U, typename V, int MAXN = WAVE_SORT_MAX>
 struct alignas(16) WaveSmem {
     U keys[MAXN];        // raw keys in bucket order; 64-bit keys: order keys of the bucket path
     V vals[MAXN];        // values by original position
-    uint16_t pos[MAXN];  // original position of each bucket-ordered key
+    uint16_t pos[sizeof(U) == 8 ? MAXN : 1];  // 64-bit keys: original position of each bucket-ordered key
     uint32_t cnt[RADIX + 1];      // running digit counts, then bucket starts (cnt[buckets] = n)
     uint32_t dummy[WAVE];         // store target of the lanes that do not advance a count
     static_assert(sizeof(U) == 8 || sizeof(U) * MAXN + sizeof(V) * MAXN >= 8 * MAXN, "sort words overlay");

@@ -490,6 +490,43 @@ def sweep_ns(pkg, torch, steps, warmup):
     return rows
 
 
+def driver_times(pkg, torch, args):
+    """The workload through segmented-sorting_amd/bin/radixseg (the reference's radixseg.cu
+    driver over this library): its input written in the reference's stdin format, then
+    `--time` (hipEvents around the SortPairs call only, like -DELAPSED_TIME) in each mode,
+    beside the library call timed back to back by this script."""
+    import subprocess
+    import tempfile
+
+    keys, vals, off = make_workload(pkg, args.workload)
+    if keys.dtype != np.uint32 or vals is None:
+        raise SystemExit("--driver: radixseg sorts u32 pairs")
+    exe = PKG_DIR / "bin" / "radixseg"
+    r = Runner(pkg, torch, keys, vals, off, "batched")
+    lib_ms, _ = r.time(args.steps, args.warmup)
+    del r
+    torch.cuda.empty_cache()
+    out = {"workload": WORKLOADS.get(args.workload, args.workload), "library_batched_ms": round(lib_ms, 4), "driver": {}}
+    with tempfile.NamedTemporaryFile("w", suffix=".in", delete=True) as f:
+        f.write(f"{off.size - 1}\n" + " ".join(map(str, off.tolist())) + f"\n{keys.size}\n")
+        f.write(" ".join(map(str, keys.tolist())) + "\n")
+        f.flush()
+        modes = {"reference (upload before each call, first call timed)": [],
+                 "upload before each call, 3 untimed warm-up calls": ["--warmup", "3"],
+                 "inputs resident, 3 warm-up calls": ["--warmup", "3", "--resident"],
+                 "inputs resident, 3 warm-up calls, length bound": ["--warmup", "3", "--resident", "--bound"],
+                 "plain, inputs resident, 3 warm-up calls": ["--warmup", "3", "--resident", "--algo", "plain"]}
+        for name, extra in modes.items():
+            with open(f.name) as inp:
+                p = subprocess.run([str(exe), "--time", "--executions", str(args.steps), *extra], stdin=inp,
+                                   capture_output=True, text=True, timeout=600)
+            if p.returncode != 0:
+                raise SystemExit(f"radixseg {extra} failed: {p.stderr}")
+            out["driver"][name] = round(float(p.stdout.strip()), 4)
+            print(f"[bench] radixseg {' '.join(extra)}: {p.stdout.strip()} ms", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -508,6 +545,9 @@ def main():
                     help="equal-segment sweep: 2^24 u32 pairs in 2^k segments, k = 0,2,...,20, every form")
     ap.add_argument("--sweep-ns", action="store_true",
                     help="N x S sweep of the batched form: N = 2^15..2^27 u32 pairs in S = 1, 2, 4, ... equal segments")
+    ap.add_argument("--driver", action="store_true",
+                    help="time the workload through the rebuilt radixseg driver (stdin format, --time) beside the "
+                         "library call: uploads before every call as the reference does, and inputs resident")
     ap.add_argument("--no-configs", action="store_true",
                     help="headline workload only (skip the other configurations of the benchmark)")
     args = ap.parse_args()
@@ -533,6 +573,9 @@ def main():
         return
     if args.sweep_ns:
         sweep_ns(pkg, torch, args.steps, args.warmup)
+        return
+    if args.driver:
+        print(json.dumps(driver_times(pkg, torch, args)), flush=True)
         return
     if args.workload == "diff":
         print(json.dumps(bench_diff(pkg, torch, args.steps, args.warmup)), flush=True)

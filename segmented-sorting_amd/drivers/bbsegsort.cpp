@@ -32,7 +32,7 @@ int main(int argc, char** argv) {
     drv::check(hipEventCreate(&stop), "hipEventCreate");
     float total_ms = 0;
     std::vector<int> out = h_vec;
-    for (int ex = 0; ex < o.executions; ++ex) {
+    for (int ex = -o.warmup; ex < o.executions; ++ex) {  // ex < 0: untimed warm-up calls
         drv::check(hipMemcpy(key_d, h_vec.data(), sizeof(int) * (size_t)N, hipMemcpyHostToDevice), "H2D");
         drv::check(hipMemcpy(val_d, h_val.data(), sizeof(double) * (size_t)N, hipMemcpyHostToDevice), "H2D");
         drv::check(hipEventRecord(start, 0), "hipEventRecord");
@@ -44,7 +44,7 @@ int main(int argc, char** argv) {
         drv::check(hipEventSynchronize(stop), "hipEventSynchronize");
         float ms = 0;
         drv::check(hipEventElapsedTime(&ms, start, stop), "hipEventElapsedTime");
-        total_ms += ms;
+        if (ex >= 0) total_ms += ms;
     }
     drv::check(hipMemcpy(out.data(), key_d, sizeof(int) * (size_t)N, hipMemcpyDeviceToHost), "D2H");
     std::vector<double> vout((size_t)N);

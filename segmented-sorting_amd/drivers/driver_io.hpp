@@ -18,8 +18,10 @@ struct Options {
     bool time = false;     // print mean ms instead of the keys (-DELAPSED_TIME=1 in the reference)
     bool values = false;   // also print the sorted values, as a second line in the keys' format
     int executions = 1;    // timed repetitions (-DEXECUTIONS)
+    int warmup = 0;        // untimed calls before them (0, as the reference: its first call is timed too)
     int algo = 0;          // segsort_algo
     bool bound = false;    // pass the longest segment (known from the host offsets) as the length bound
+    bool resident = false; // upload the inputs once instead of before every call (radixseg)
 };
 
 inline Options parse(int argc, char** argv, int default_algo) {
@@ -29,13 +31,16 @@ inline Options parse(int argc, char** argv, int default_algo) {
         if (!strcmp(argv[i], "--time")) o.time = true;
         else if (!strcmp(argv[i], "--values")) o.values = true;
         else if (!strcmp(argv[i], "--bound")) o.bound = true;
+        else if (!strcmp(argv[i], "--resident")) o.resident = true;
         else if (!strcmp(argv[i], "--executions") && i + 1 < argc) o.executions = atoi(argv[++i]);
+        else if (!strcmp(argv[i], "--warmup") && i + 1 < argc) o.warmup = atoi(argv[++i]);
         else if (!strcmp(argv[i], "--algo") && i + 1 < argc) {
             const char* a = argv[++i];
             o.algo = !strcmp(a, "plain") ? 1 : !strcmp(a, "fixpass") ? 2 : 0;
         }
     }
     if (o.executions < 1) o.executions = 1;
+    if (o.warmup < 0) o.warmup = 0;
     return o;
 }
 

@@ -27,7 +27,7 @@ int main(int argc, char** argv) {
     drv::check(hipEventCreate(&start), "hipEventCreate");
     drv::check(hipEventCreate(&stop), "hipEventCreate");
     float total_ms = 0;
-    for (int ex = 0; ex < o.executions; ++ex) {
+    for (int ex = -o.warmup; ex < o.executions; ++ex) {  // ex < 0: untimed warm-up calls
         if (S > 0) drv::check(hipMemcpy(d_seg, seg.data(), sizeof(int) * (size_t)S, hipMemcpyHostToDevice), "H2D");
         drv::check(hipMemcpy(d_vec, keys.data(), sizeof(unsigned) * (size_t)N, hipMemcpyHostToDevice), "H2D");
         drv::check(hipMemcpy(d_idx, idx.data(), sizeof(unsigned) * (size_t)N, hipMemcpyHostToDevice), "H2D");
@@ -41,7 +41,7 @@ int main(int argc, char** argv) {
         drv::check(hipEventSynchronize(stop), "hipEventSynchronize");
         float ms = 0;
         drv::check(hipEventElapsedTime(&ms, start, stop), "hipEventElapsedTime");
-        total_ms += ms;
+        if (ex >= 0) total_ms += ms;
     }
     std::vector<unsigned> out((size_t)N);
     drv::check(hipMemcpy(out.data(), d_vec, sizeof(unsigned) * (size_t)N, hipMemcpyDeviceToHost), "D2H");

@@ -45,7 +45,7 @@ int main(int argc, char** argv) {
     void* d_temp = nullptr;
     drv::check(hipMalloc(&d_temp, temp_bytes), "hipMalloc temp");
     float total_ms = 0;
-    for (int ex = 0; ex < o.executions; ++ex) {
+    for (int ex = -o.warmup; ex < o.executions; ++ex) {  // ex < 0: untimed warm-up calls
         drv::check(hipMemcpy(d_vec, keys.data(), sizeof(unsigned) * (size_t)N, hipMemcpyHostToDevice), "H2D");
         drv::check(hipMemcpy(d_vec_out, keys.data(), sizeof(unsigned) * (size_t)N, hipMemcpyHostToDevice), "H2D");
         if (o.values) {
@@ -62,7 +62,7 @@ int main(int argc, char** argv) {
         drv::check(hipEventSynchronize(stop), "hipEventSynchronize");
         float ms = 0;
         drv::check(hipEventElapsedTime(&ms, start, stop), "hipEventElapsedTime");
-        total_ms += ms;
+        if (ex >= 0) total_ms += ms;
     }
     drv::check(hipMemcpy(keys.data(), d_vec_out, sizeof(unsigned) * (size_t)N, hipMemcpyDeviceToHost), "D2H");
     if (o.values)

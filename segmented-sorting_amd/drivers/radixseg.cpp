@@ -36,10 +36,15 @@ int main(int argc, char** argv) {
     void* d_temp = nullptr;
     size_t temp_bytes = 0;
     float total_ms = 0;
-    for (int ex = 0; ex < o.executions; ++ex) {
-        drv::check(hipMemcpy(d_seg, seg.data(), sizeof(int) * seg.size(), hipMemcpyHostToDevice), "H2D");
-        drv::check(hipMemcpy(d_vec, keys.data(), sizeof(unsigned) * (size_t)N, hipMemcpyHostToDevice), "H2D");
-        drv::check(hipMemcpy(d_val, vals.data(), sizeof(unsigned) * (size_t)N, hipMemcpyHostToDevice), "H2D");
+    for (int ex = -o.warmup; ex < o.executions; ++ex) {  // ex < 0: untimed warm-up calls
+        // the reference uploads the inputs before every timed call (radixseg.cu:111-114); with
+        // --resident they are uploaded once (the sort never modifies them), so that the call is
+        // timed on inputs already resident in HBM, as bench.py times it
+        if (!o.resident || ex == -o.warmup) {
+            drv::check(hipMemcpy(d_seg, seg.data(), sizeof(int) * seg.size(), hipMemcpyHostToDevice), "H2D");
+            drv::check(hipMemcpy(d_vec, keys.data(), sizeof(unsigned) * (size_t)N, hipMemcpyHostToDevice), "H2D");
+            drv::check(hipMemcpy(d_val, vals.data(), sizeof(unsigned) * (size_t)N, hipMemcpyHostToDevice), "H2D");
+        }
         if (!d_temp) {
             drv::check(segsort::DeviceSegmentedRadixSort::SortPairs(nullptr, temp_bytes, d_vec, d_vec_out, d_val,
                                                                     d_val_out, N, S, d_seg, d_seg + 1, 0, 32, 0,
@@ -56,7 +61,7 @@ int main(int argc, char** argv) {
         drv::check(hipEventSynchronize(stop), "hipEventSynchronize");
         float ms = 0;
         drv::check(hipEventElapsedTime(&ms, start, stop), "hipEventElapsedTime");
-        total_ms += ms;
+        if (ex >= 0) total_ms += ms;
     }
     drv::check(hipMemcpy(keys.data(), d_vec_out, sizeof(unsigned) * (size_t)N, hipMemcpyDeviceToHost), "D2H");
     drv::check(hipMemcpy(vals.data(), d_val_out, sizeof(unsigned) * (size_t)N, hipMemcpyDeviceToHost), "D2H");

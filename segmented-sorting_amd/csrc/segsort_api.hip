@@ -533,7 +533,10 @@ struct Launcher {
     static void launch_tile(Bufs b, Sched sc, int cls, KeyMode km, hipStream_t st, int grid, bool persistent,
                             int grid_mult) {
         auto k = k_tile_tier<NT, IPT, U, V, HAS_V>;
-        const int g = persistent ? resident_blocks(k, NT) * grid_mult : grid;
+        // persistent: every resident slot, but no more workgroups than the class
+        // can have ranges (`grid` > 0: that bound)
+        int g = persistent ? resident_blocks(k, NT) * grid_mult : grid;
+        if (persistent && grid > 0) g = std::min(g, grid);
         char name[32];
         snprintf(name, sizeof(name), "tile_%dx%d", NT, IPT);
         if (g > 0) SEGSORT_LAUNCH(name, st, cls, PAIR_BYTES, 0.0, (k<<<g, NT, 0, st>>>(b, sc, cls, km, persistent)));
@@ -545,20 +548,24 @@ struct Launcher {
                              int grid_mult) {
         auto k = k_group_tier<G, U, V, HAS_V>;
         constexpr int PER_BLOCK = 4 * (WAVE / G);
-        const int g = persistent ? resident_blocks(k, 256) * grid_mult : (grid + PER_BLOCK - 1) / PER_BLOCK;
+        int g = persistent ? resident_blocks(k, 256) * grid_mult : (grid + PER_BLOCK - 1) / PER_BLOCK;
+        if (persistent && grid > 0) g = std::min(g, (grid + PER_BLOCK - 1) / PER_BLOCK);
         char name[32];
         snprintf(name, sizeof(name), "group_%d", G);
         if (g > 0) SEGSORT_LAUNCH(name, st, cls, PAIR_BYTES, 0.0, (k<<<g, 256, 0, st>>>(b, sc, cls, km)));
     }
 
-    // wave_max: the class's longest range (for the wave tier's build)
+    // wave_max: the class's longest range (for the wave tier's build). grid: the
+    // ranges to sort (non-persistent, the plain form), or (persistent) a bound on
+    // them that caps the grid (0: none).
     static void launch_class(int kind, Bufs b, Sched sc, int cls, KeyMode km, hipStream_t st, int grid, bool persistent,
                              int grid_mult, int wave_max) {
         switch (kind) {
         case 0: {
             // ranges of at most 256 items: the smaller-LDS, higher-occupancy build
             auto k = wave_max <= 256 ? k_wave_tier<U, V, HAS_V, 256> : k_wave_tier<U, V, HAS_V, 512>;
-            const int g = persistent ? resident_blocks(k, 256) * grid_mult : (grid + 3) / 4;
+            int g = persistent ? resident_blocks(k, 256) * grid_mult : (grid + 3) / 4;
+            if (persistent && grid > 0) g = std::min(g, (grid + 3) / 4);
             if (g > 0)
                 SEGSORT_LAUNCH("wave_tier", st, cls, PAIR_BYTES, 0.0,
                                (k<<<g, 256, 0, st>>>(b, sc, cls, km, persistent ? 1 : 0, WAVE_RANK_MAX)));
@@ -582,7 +589,17 @@ struct Launcher {
     // 2^20 segments of U[1,32] f32 keys about 6% faster with the groups inside;
     // with 64-bit keys the shared kernel's register budget made the groups
     // slower (9.6% at 5 waves per SIMD), so those keep their own launches.
-    static void launch_classes(const Plan& p, Bufs b, Sched sc, int q0, KeyMode km, hipStream_t st, int grid_mult) {
+    //
+    // A class whose ranges hold at least lo items has at most N / lo of them, so
+    // its persistent grid is capped there: a small call launches few workgroups
+    // (an empty 16K-workgroup launch takes about 4.5 us here, a 2K one 2.5 us).
+    static int max_ranges(const Plan& p, int c, long long n_items) {
+        const long long lo = c > 0 ? p.cc.cap[c - 1] + 1 : 2;
+        return (int)std::min<long long>(n_items / lo + 1, 0x7FFFFFFF);
+    }
+
+    static void launch_classes(const Plan& p, Bufs b, Sched sc, int q0, KeyMode km, hipStream_t st, int grid_mult,
+                               long long n_items) {
         int w = 0;  // the wave class, after the group classes it takes in
         if (sizeof(U) == 4)
             while (w < p.nclass && p.kind[w] >= KIND_GROUP) ++w;
@@ -601,13 +618,21 @@ struct Launcher {
                     more |= 1u << (q0 + cg);
                 }
                 auto k = k_wave_tile_tier<U, V, HAS_V, 256, C::IPT0>;
-                const int g = resident_blocks(k, 256) * grid_mult;
+                // workgroups the shared launch can use: four waves per block for the
+                // wave tier, 4 * 64 / G ranges per block for a group class, one range
+                // per block for the tiles
+                int cap = std::max((max_ranges(p, w, n_items) + 3) / 4, max_ranges(p, w + 1, n_items));
+                for (int cg = g0; cg < w; ++cg) {
+                    const int per = 4 * (WAVE / (4 << (p.kind[cg] - KIND_GROUP)));
+                    cap = std::max(cap, (max_ranges(p, cg, n_items) + per - 1) / per);
+                }
+                const int g = std::min(resident_blocks(k, 256) * grid_mult, cap);
                 SEGSORT_LAUNCHM("wave_tile", st, q0 + c, more, PAIR_BYTES,
                                 (k<<<g, 256, 0, st>>>(b, sc, q0 + c, q0 + c + 1, km, WAVE_RANK_MAX, gq)));
                 ++c;
                 continue;
             }
-            launch_class(p.kind[c], b, sc, q0 + c, km, st, 0, true, grid_mult, p.cc.cap[c]);
+            launch_class(p.kind[c], b, sc, q0 + c, km, st, max_ranges(p, c, n_items), true, grid_mult, p.cc.cap[c]);
         }
     }
 
@@ -694,7 +719,7 @@ struct Launcher {
     // lengths by the largest tile: the MSD tier is not launched, and a segment
     // that breaks the bound is sorted by one workgroup (k_seg_lsd) at the end.
     static hipError_t enqueue_batched(char* base, const Layout& L, const Plan& p, const Tuning& tu, int levels,
-                                      const void* ki, void* ko, const void* vi, void* vo, int S, const int* db,
+                                      const void* ki, void* ko, const void* vi, void* vo, int N, int S, const int* db,
                                       const int* de, KeyMode km, hipStream_t st, bool fallback = false) {
         // With `concurrent` and long segments possible, the
         // binned ranges are sorted on a second stream while the MSD tier (a few
@@ -739,7 +764,7 @@ struct Launcher {
             if ((e = hipGetLastError()) != hipSuccess) return e;
             if (capturing && (e = hipEventRecord(ss->fork, st)) != hipSuccess) return e;
             if ((e = hipStreamWaitEvent(ss->s, ss->fork, 0)) != hipSuccess) return e;
-            launch_classes(p, b, sc, 0, km, ss->s, tu.grid_mult);
+            launch_classes(p, b, sc, 0, km, ss->s, tu.grid_mult, N);
             if ((e = hipEventRecord(ss->join, ss->s)) != hipSuccess) return e;
         }
         if (levels > 0) {
@@ -755,9 +780,11 @@ struct Launcher {
             const int big = levels > 1 ? MSD_BIG_TILES * C::CHUNK : 0x7FFFFFFF;
             // one workgroup per resident slot (measured: twice or three times as
             // many, mostly finding no chunk, only wait for CUs under the overlap)
-            const int gh = resident_blocks(kh, C::NTC);
-            const int gs = resident_blocks(ks, C::NTC);
-            const int gl = resident_blocks(kl, C::NT3);
+            // (no more workgroups than chunks, or than depth-first ranges)
+            const int nch = (int)std::min<size_t>(L.chunks, 0x7FFFFFFF);
+            const int gh = std::min(resident_blocks(kh, C::NTC), nch);
+            const int gs = std::min(resident_blocks(ks, C::NTC), nch);
+            const int gl = std::min(resident_blocks(kl, C::NT3), (int)std::min<size_t>(L.lq_cap, 0x7FFFFFFF));
             SEGSORT_LAUNCH("large_hist", st, 100 + MSD_LEVEL0, (double)sizeof(U), 0.0,
                            (kh<<<gh, C::NTC, 0, st>>>(b, sc, MSD_LEVEL0, km)));
             SEGSORT_LAUNCH("large_scatter", st, 100 + MSD_LEVEL0, PAIR_BYTES, 0.0,
@@ -775,7 +802,7 @@ struct Launcher {
             SEGSORT_LAUNCH("large_local", st, 100 + MSD_LOCAL, PAIR_BYTES + sizeof(U), 0.0,
                            (kl<<<gl, C::NT3, 0, st>>>(b, sc, MSD_LOCAL, km, p.cc, p.tile_max)));
         }
-        launch_classes(p, b, sc, sc.child_q, km, st, tu.grid_mult);
+        launch_classes(p, b, sc, sc.child_q, km, st, tu.grid_mult, N);
         if (fallback) {
             SEGSORT_LAUNCH("oversize_lsd", st, 100 + MAX_LEVELS, PAIR_BYTES, 0.0,
                            (k_seg_lsd<C::NT3, C::IPT3, U, V, HAS_V><<<FALLBACK_GRID, C::NT3, 0, st>>>(b, sc, km)));
@@ -818,11 +845,11 @@ struct Launcher {
         if (t_no_alloc) {
             Tuning one = tu;
             one.concurrent = 0;
-            e = enqueue_batched(base, L, p, one, run_levels, ki, ko, vi, vo, S, db, de, km, st, fallback);
+            e = enqueue_batched(base, L, p, one, run_levels, ki, ko, vi, vo, N, S, db, de, km, st, fallback);
         } else {
             e = run_graph(key, st, [&](hipStream_t cs, bool captured) {
                 (void)captured;
-                return enqueue_batched(base, L, p, tu, run_levels, ki, ko, vi, vo, S, db, de, km, cs, fallback);
+                return enqueue_batched(base, L, p, tu, run_levels, ki, ko, vi, vo, N, S, db, de, km, cs, fallback);
             });
         }
         if (e != hipSuccess) return fail(e, "segsort_sort(batched): launch");
@@ -887,7 +914,7 @@ struct Launcher {
             char* bs = base + lens_bytes + (size_t)s * per;
             e = enqueue_batched(bs, L, p, tu, levels_for(km, n, 1, p), (const char*)ki + o * sizeof(U),
                                 (char*)ko + o * sizeof(U), HAS_V ? (const char*)vi + o * sizeof(V) : nullptr,
-                                HAS_V ? (char*)vo + o * sizeof(V) : nullptr, 1, d_lens, d_lens + 1 + i, km,
+                                HAS_V ? (char*)vo + o * sizeof(V) : nullptr, n, 1, d_lens, d_lens + 1 + i, km,
                                 f->s[(size_t)s]);
             if (e != hipSuccess) return fail(e, "segsort_multistream_sort: launch");
         }

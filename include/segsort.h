@@ -213,7 +213,10 @@ int segsort_perm_scatter_f32(const float* d_grad_out, const int* d_perm,
  * to the smallest enabled class that holds them (groups, then the wave tier
  * if wave_cap is larger, then the tiles); ranges above the largest enabled
  * class go to the multi-workgroup MSD tier (batched) or the
- * one-workgroup-per-segment LSD (plain). Default 0xEF.
+ * one-workgroup-per-segment LSD (plain). Bit 8 (0x100): the batched form's
+ * 16-item tile classes take their ranges longest first (length buckets
+ * filled by the binning pass); it pays off for mixed lengths and costs a few
+ * percent for equal ones. Default 0xEF.
  * grid_mult: persistent workgroups per CU per resident slot (1..8).
  * concurrent: 1 sorts the ranges that fit a class on a second stream while
  * the multi-workgroup tier partitions the longer segments on the caller's

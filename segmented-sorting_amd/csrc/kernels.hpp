@@ -79,11 +79,11 @@ struct Sched {
     uint32_t bcap[MAX_TILE_CLASSES];
 };
 
-// Length bucket of a range of n items in class c (cap[c-1] < n <= cap[c]).
-__device__ __forceinline__ int lpt_bucket(const ClassCaps& cc, int c, int n) {
-    const int lo = c > 0 ? cc.cap[c - 1] + 1 : 2;
-    const int b = (int)(((long long)(n - lo) * LPT_BUCKETS) / (cc.cap[c] - lo + 1));
-    return b < 0 ? 0 : b >= LPT_BUCKETS ? LPT_BUCKETS - 1 : b;
+// Length bucket of a range of n items in a class of the lengths lo..hi (a tile
+// class: hi <= 16384, so (n - lo) * LPT_BUCKETS fits 32 bits).
+__device__ __forceinline__ int lpt_bucket(int lo, int hi, int n) {
+    const uint32_t b = ((uint32_t)(n - lo) * LPT_BUCKETS) / (uint32_t)(hi - lo + 1);
+    return b >= LPT_BUCKETS ? LPT_BUCKETS - 1 : (int)b;
 }
 
 // The ranges a bucketed class's kernel takes before its queue tq[cls]: the
@@ -255,27 +255,30 @@ __device__ __forceinline__ void push_oversize(const Sched& sc, int begin, int n)
 // atomic per block and class); slots are taken from LDS counters
 // (one LDS atomic per wave and class) and then from the global counters with
 // one atomic per block and class, so the hot global counters see few atomics.
+// LPT: the classes with length buckets (sc.bq[c] set) file their ranges by
+// bucket instead, with an LDS atomic per range.
 constexpr int BIN_NT = 1024;
 constexpr int BIN_SPT = 8;
 
-template <typename U, typename V, bool HAS_V>
+template <typename U, typename V, bool HAS_V, bool LPT>
 __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin, const int* __restrict__ d_end,
                                                 int num_segments, Bufs b, Sched sc, ClassCaps cc, int end_bit,
                                                 int large_to_level0, int spt) {
+    constexpr int NB = LPT ? MAX_TILE_CLASSES * LPT_BUCKETS : 1;
     __shared__ uint32_t lcount[MAX_TILE_CLASSES];
     __shared__ uint32_t lbase[MAX_TILE_CLASSES];
     __shared__ unsigned long long litems[MAX_TILE_CLASSES];
-    __shared__ uint32_t lbcount[MAX_TILE_CLASSES][LPT_BUCKETS];  // bucketed classes: per length bucket
-    __shared__ uint32_t lbbase[MAX_TILE_CLASSES][LPT_BUCKETS];
+    __shared__ uint32_t lbcount[NB];  // LPT: per class and length bucket
+    __shared__ uint32_t lbbase[NB];
     if (threadIdx.x < MAX_TILE_CLASSES) {
         lcount[threadIdx.x] = 0;
         litems[threadIdx.x] = 0;
     }
-    if (threadIdx.x < MAX_TILE_CLASSES * LPT_BUCKETS) lbcount[threadIdx.x / LPT_BUCKETS][threadIdx.x % LPT_BUCKETS] = 0;
+    if (LPT && threadIdx.x < NB) lbcount[threadIdx.x] = 0;
     __syncthreads();
     int beg[BIN_SPT], n[BIN_SPT], cls[BIN_SPT];
     uint32_t slot[BIN_SPT];
-    int bkt[BIN_SPT];  // length bucket of a bucketed class's range, else -1
+    int bkt[BIN_SPT];  // LPT: length bucket of a bucketed class's range, else -1
     const int s0 = blockIdx.x * (BIN_NT * spt) + threadIdx.x;
     // every offset load issued before any is used
 #pragma unroll
@@ -298,14 +301,7 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
         cls[j] = n[j] >= 2 ? class_of(cc, n[j]) : -2;
         if (cls[j] == -1 && large_to_level0 == 1) push_large(sc, MSD_LEVEL0, beg[j], n[j], BUF_IN, end_bit, true);
         if (cls[j] == -1 && large_to_level0 == 2) push_oversize(sc, beg[j], n[j]);
-        // ranges of a bucketed class take a slot of their length bucket (an LDS
-        // atomic each: they are the few long segments) and skip the ballots below
         bkt[j] = -1;
-        if (cls[j] >= 0 && sc.bq[cls[j]] != nullptr) {
-            bkt[j] = lpt_bucket(cc, cls[j], n[j]);
-            slot[j] = atomicAdd(&lbcount[cls[j]][bkt[j]], 1u);
-            atomicAdd(&litems[cls[j]], (unsigned long long)n[j]);
-        }
     }
     // per class: the wave's slots counted with ballots in registers, then one LDS
     // atomic per class and wave for the wave's base inside the block
@@ -315,20 +311,27 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
         wbase[c] = 0;
         if (c >= cc.nclass) continue;
         uint32_t run = 0, items = 0;
+        const bool bucketed = LPT && sc.bq[c] != nullptr;  // wave-uniform
 #pragma unroll
         for (int j = 0; j < BIN_SPT; ++j) {
-            const bool mine = cls[j] == c && bkt[j] < 0;
+            const bool mine = cls[j] == c;
             const uint64_t m = __ballot(mine);
-            if (mine) {
-                slot[j] = run + lanes_below(m);
-                items += (uint32_t)n[j];
+            if (mine) items += (uint32_t)n[j];
+            if (!bucketed) {
+                if (mine) slot[j] = run + lanes_below(m);
+                run += (uint32_t)__popcll(m);
+            } else if (mine) {
+                // a slot of the range's length bucket: an LDS atomic per range (the
+                // long ranges of a mixed call spread over the buckets)
+                const int lo = c > 0 ? cc.cap[c - 1] + 1 : 2;
+                bkt[j] = lpt_bucket(lo, cc.cap[c], n[j]);
+                slot[j] = atomicAdd(&lbcount[c * LPT_BUCKETS + bkt[j]], 1u);
             }
-            run += (uint32_t)__popcll(m);
         }
-        if (run == 0) continue;
+        if (__ballot(items != 0) == 0) continue;
         items = wave_sum(items);
         if (lane_id() == 0) {
-            wbase[c] = atomicAdd(&lcount[c], run);
+            if (run) wbase[c] = atomicAdd(&lcount[c], run);
             atomicAdd(&litems[c], (unsigned long long)items);
         }
     }
@@ -340,24 +343,24 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
             if (cls[j] == c && bkt[j] < 0) slot[j] += wb;
     }
     __syncthreads();
-    if (threadIdx.x < (unsigned)cc.nclass) {
-        const int c = threadIdx.x;
-        lbase[c] = lcount[c] ? atomicAdd(&sc.ctr->tile_count[c], lcount[c]) : 0u;
-        if (litems[c]) atomicAdd(&sc.ctr->tile_items[c], litems[c]);
-    }
-    if (threadIdx.x < (unsigned)(cc.nclass * LPT_BUCKETS)) {
-        const int c = threadIdx.x / LPT_BUCKETS, bb = threadIdx.x % LPT_BUCKETS;
-        const uint32_t k = lbcount[c][bb];
-        lbbase[c][bb] = k ? atomicAdd(&sc.ctr->bcount[c][bb], k) : 0u;
-    }
+    // the block's global slots: every atomic issued before any result is used
+    uint32_t g0 = 0, g1 = 0;
+    if (threadIdx.x < (unsigned)cc.nclass && lcount[threadIdx.x])
+        g0 = atomicAdd(&sc.ctr->tile_count[threadIdx.x], lcount[threadIdx.x]);
+    if (LPT && threadIdx.x < (unsigned)(cc.nclass * LPT_BUCKETS) && lbcount[threadIdx.x])
+        g1 = atomicAdd(&sc.ctr->bcount[threadIdx.x / LPT_BUCKETS][threadIdx.x % LPT_BUCKETS], lbcount[threadIdx.x]);
+    if (threadIdx.x < (unsigned)cc.nclass && litems[threadIdx.x])
+        atomicAdd(&sc.ctr->tile_items[threadIdx.x], litems[threadIdx.x]);
+    if (threadIdx.x < (unsigned)cc.nclass) lbase[threadIdx.x] = g0;
+    if (LPT && threadIdx.x < (unsigned)(cc.nclass * LPT_BUCKETS)) lbbase[threadIdx.x] = g1;
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < BIN_SPT; ++j) {
         if (cls[j] >= 0 && bkt[j] < 0) {
             const uint32_t g = lbase[cls[j]] + slot[j];
             if (g < sc.tq_cap[cls[j]]) sc.tq[cls[j]][g] = make_ti(beg[j], n[j], BUF_IN);
-        } else if (cls[j] >= 0) {
-            const uint32_t g = lbbase[cls[j]][bkt[j]] + slot[j];
+        } else if (LPT && cls[j] >= 0) {
+            const uint32_t g = lbbase[cls[j] * LPT_BUCKETS + bkt[j]] + slot[j];
             if (g < sc.bcap[cls[j]]) sc.bq[cls[j]][(size_t)bkt[j] * sc.bcap[cls[j]] + g] = make_ti(beg[j], n[j], BUF_IN);
         }
     }
@@ -496,7 +499,10 @@ __device__ __forceinline__ void tile_loop_pipelined(typename RadixTile<NT, IPT, 
 // LDS tier: each workgroup sorts one queued range of n <= NT*IPT items.
 // At least 4 waves per SIMD (16 per CU) so that LDS and memory latency can be
 // hidden; this caps the tile kernels at 128 VGPRs.
-template <int NT, int IPT, typename U, typename V, bool HAS_V>
+// LPT (16-item tiles): the class's length buckets are taken longest first,
+// then its queue. A separate build: the bucket walk's state costs the kernel
+// scalar registers (spills) that equal-length segments gain nothing for.
+template <int NT, int IPT, typename U, typename V, bool HAS_V, bool LPT>
 __global__ __launch_bounds__(NT, 4) void k_tile_tier(Bufs b, Sched sc, int cls, KeyMode km, int persistent) {
     using T = RadixTile<NT, IPT, U, V, HAS_V>;
     if constexpr (IPT > 4) {
@@ -512,11 +518,17 @@ __global__ __launch_bounds__(NT, 4) void k_tile_tier(Bufs b, Sched sc, int cls, 
             typename T2::Smem half;
         } sm;
         const int count = tile_count(sc, cls);
-        const BucketView bv = bucket_view(sc, cls);  // the binned ranges, longest first, then the queue
+        BucketView bv;
+        bv.total = 0;
+        if constexpr (LPT) bv = bucket_view(sc, cls);  // the binned ranges, longest first, then the queue
         const int total = bv.total + count;
         for (int i = blockIdx.x; i < total; i += persistent ? (int)gridDim.x : total) {
-            const TileItem cur = i < bv.total ? queue_item(bucket_entry(sc, bv, cls, i), 0)
-                                              : queue_item(sc.tq[cls], i - bv.total);
+            TileItem cur;
+            if constexpr (LPT) {
+                cur = i < bv.total ? queue_item(bucket_entry(sc, bv, cls, i), 0) : queue_item(sc.tq[cls], i - bv.total);
+            } else {
+                cur = queue_item(sc.tq[cls], i);
+            }
             const int src = ti_src(cur);
             const int n = ti_n(cur);
             const U* ks = (const U*)b.k[src];

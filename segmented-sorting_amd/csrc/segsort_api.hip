@@ -60,7 +60,8 @@ constexpr int WAVE_RANK_MAX = 24;
 
 struct Tuning {
     int wave_cap;     // ranges of 2..wave_cap items go to the wave tier
-    int tile_enable;  // bit i < 4: LDS tile class i on; bit 4 + g: wave-group class of 4 << g items on
+    int tile_enable;  // bit i < 4: LDS tile class i on; bit 4 + g: wave-group class of 4 << g items on;
+                      // TUNE_LPT: the 16-item tile classes take their ranges longest first
     int grid_mult;    // persistent workgroups per resident slot
     int concurrent;   // binned ranges sorted on a second stream while the MSD tier runs (direct launches only)
 };
@@ -79,6 +80,15 @@ struct TableEntry {
     Tuning tu;
 };
 #include "tuning_table.inc"
+
+// tile_enable bit 8: the batched form's 16-item tile classes (256x16, 512x16,
+// 1024x16) take their binned ranges longest first, from length buckets. Measured
+// (2^24 items in 2^16 lognormal segments): 0.209 -> 0.202 ms, the 1024x16 and
+// 256x16 kernels no longer ending on long ranges started last. Off for equal
+// lengths (one bucket) and tiny segments: the bucketed builds cost 2-9% there
+// (2^10 segments of 2^14: 0.159 -> 0.168 ms), so the table sets it only for the
+// signatures where it measured faster, and the default leaves it off.
+constexpr int TUNE_LPT = 0x100;
 
 constexpr Tuning DEFAULT_TUNING = {256, 0xEF, 1, 0};
 
@@ -417,6 +427,7 @@ struct Plan {
     int kind[MAX_TILE_CLASSES];
     int nclass;
     int tile_max;
+    bool lpt;  // batched form: length buckets for the 16-item tile classes (TUNE_LPT)
 };
 
 // Classes in increasing capacity: the enabled wave-group classes, the wave
@@ -425,6 +436,7 @@ template <typename C>
 Plan make_plan(const Tuning& tu) {
     Plan p;
     p.nclass = 0;
+    p.lpt = (tu.tile_enable & TUNE_LPT) != 0;
     int last = 1;
     for (int g = 0; g < 4; ++g) {
         if (!(tu.tile_enable & (0x10 << g))) continue;
@@ -483,13 +495,10 @@ Layout make_layout(const Plan& p, int num_items, int num_segments, int levels, b
         L.tq_cap[c] = S + bucket_bound;
         L.tq[c] = take(L.tq_cap[c] * sizeof(TileItem));
     }
-    // the batched form's large tile classes (16 items per thread) take their
-    // binned ranges longest first: LPT_BUCKETS length buckets, each able to hold
-    // every binned range of the class (at most min(S, N / shortest length)).
-    // Measured (2^24 items in 2^16 lognormal segments): 0.217 -> 0.200 ms, the
-    // 1024x16 and 256x16 kernels no longer ending on long ranges started last;
-    // equal segments (one bucket) within 1-3%.
-    for (int c = 0; c < p.nclass && !plain; ++c) {
+    // TUNE_LPT: the batched form's large tile classes (16 items per thread) take
+    // their binned ranges longest first: LPT_BUCKETS length buckets, each able to
+    // hold every binned range of the class (at most min(S, N / shortest length))
+    for (int c = 0; c < p.nclass && !plain && p.lpt; ++c) {
         if (p.kind[c] < KIND_TILE + 1 || p.kind[c] > KIND_TILE + 3) continue;
         const size_t lo = c > 0 ? (size_t)p.cc.cap[c - 1] + 1 : 2;
         L.bcap[c] = std::min(S, N / lo + 1);
@@ -531,8 +540,11 @@ struct Launcher {
 
     template <int NT, int IPT>
     static void launch_tile(Bufs b, Sched sc, int cls, KeyMode km, hipStream_t st, int grid, bool persistent,
-                            int grid_mult) {
-        auto k = k_tile_tier<NT, IPT, U, V, HAS_V>;
+                            int grid_mult, bool lpt) {
+        // the length-bucket build exists for the 16-item tiles only
+        auto k = k_tile_tier<NT, IPT, U, V, HAS_V, false>;
+        if constexpr (IPT > 4)
+            if (lpt) k = k_tile_tier<NT, IPT, U, V, HAS_V, true>;
         // persistent: every resident slot, but no more workgroups than the class
         // can have ranges (`grid` > 0: that bound)
         int g = persistent ? resident_blocks(k, NT) * grid_mult : grid;
@@ -559,7 +571,7 @@ struct Launcher {
     // ranges to sort (non-persistent, the plain form), or (persistent) a bound on
     // them that caps the grid (0: none).
     static void launch_class(int kind, Bufs b, Sched sc, int cls, KeyMode km, hipStream_t st, int grid, bool persistent,
-                             int grid_mult, int wave_max) {
+                             int grid_mult, int wave_max, bool lpt) {
         switch (kind) {
         case 0: {
             // ranges of at most 256 items: the smaller-LDS, higher-occupancy build
@@ -571,10 +583,10 @@ struct Launcher {
                                (k<<<g, 256, 0, st>>>(b, sc, cls, km, persistent ? 1 : 0, WAVE_RANK_MAX)));
             break;
         }
-        case 1: launch_tile<C::NT0, C::IPT0>(b, sc, cls, km, st, grid, persistent, grid_mult); break;
-        case 2: launch_tile<C::NT1, C::IPT1>(b, sc, cls, km, st, grid, persistent, grid_mult); break;
-        case 3: launch_tile<C::NT2, C::IPT2>(b, sc, cls, km, st, grid, persistent, grid_mult); break;
-        case 4: launch_tile<C::NT3, C::IPT3>(b, sc, cls, km, st, grid, persistent, grid_mult); break;
+        case 1: launch_tile<C::NT0, C::IPT0>(b, sc, cls, km, st, grid, persistent, grid_mult, lpt); break;
+        case 2: launch_tile<C::NT1, C::IPT1>(b, sc, cls, km, st, grid, persistent, grid_mult, lpt); break;
+        case 3: launch_tile<C::NT2, C::IPT2>(b, sc, cls, km, st, grid, persistent, grid_mult, lpt); break;
+        case 4: launch_tile<C::NT3, C::IPT3>(b, sc, cls, km, st, grid, persistent, grid_mult, lpt); break;
         case KIND_GROUP + 0: launch_group<4>(b, sc, cls, km, st, grid, persistent, grid_mult); break;
         case KIND_GROUP + 1: launch_group<8>(b, sc, cls, km, st, grid, persistent, grid_mult); break;
         case KIND_GROUP + 2: launch_group<16>(b, sc, cls, km, st, grid, persistent, grid_mult); break;
@@ -582,7 +594,8 @@ struct Launcher {
         }
     }
 
-    // The classes of one queue set (queue q0 + c for class c), smallest first.
+    // The classes of one queue set (queue q0 + c for class c), smallest first;
+    // the length buckets belong to the binned ranges' set (q0 = 0).
     // A wave tier of ranges up to 256 items followed by the 256 x 4 tile class
     // runs as one launch (k_wave_tile_tier), for 32-bit keys together with the
     // wave-group classes (all enabled ones come first in the plan). Measured:
@@ -632,7 +645,8 @@ struct Launcher {
                 ++c;
                 continue;
             }
-            launch_class(p.kind[c], b, sc, q0 + c, km, st, max_ranges(p, c, n_items), true, grid_mult, p.cc.cap[c]);
+            launch_class(p.kind[c], b, sc, q0 + c, km, st, max_ranges(p, c, n_items), true, grid_mult, p.cc.cap[c],
+                         p.lpt && q0 == 0);
         }
     }
 
@@ -748,16 +762,16 @@ struct Launcher {
         const size_t mark = Profiler::get().mark();
         SEGSORT_LAUNCH("init", st, -1, 0.0, 0.0, (k_init<<<1, 64, 0, st>>>(sc.ctr)));
         const int large_mode = fallback ? 2 : 1;  // k_bin: segments above the largest tile to MSD level 0 or the fallback
+        auto kbin = p.lpt ? k_bin<U, V, HAS_V, true> : k_bin<U, V, HAS_V, false>;
         if (split && !capturing) {
             // the fork event completes with the binning kernel itself (no separate marker on the caller's stream)
             SEGSORT_LAUNCH("bin", st, -1, 0.0, 8.0 * S,
-                           hipExtLaunchKernelGGL(k_bin<U, V, HAS_V>, dim3(bin_grid(S)), dim3(BIN_NT), 0, st, nullptr,
-                                                 ss->fork, 0, db, de, S, b, sc, p.cc, km.end_bit, large_mode,
-                                                 bin_spt(S)));
+                           hipExtLaunchKernelGGL(kbin, dim3(bin_grid(S)), dim3(BIN_NT), 0, st, nullptr, ss->fork, 0,
+                                                 db, de, S, b, sc, p.cc, km.end_bit, large_mode, bin_spt(S)));
         } else {
             SEGSORT_LAUNCH("bin", st, -1, 0.0, 8.0 * S,
-                           (k_bin<U, V, HAS_V><<<bin_grid(S), BIN_NT, 0, st>>>(db, de, S, b, sc, p.cc, km.end_bit,
-                                                                             large_mode, bin_spt(S))));
+                           (kbin<<<bin_grid(S), BIN_NT, 0, st>>>(db, de, S, b, sc, p.cc, km.end_bit, large_mode,
+                                                                 bin_spt(S))));
         }
         hipError_t e = hipSuccess;
         if (split) {
@@ -943,7 +957,8 @@ struct Launcher {
         const size_t mark = Profiler::get().mark();
         SEGSORT_LAUNCH("init", st, -1, 0.0, 0.0, (k_init<<<1, 64, 0, st>>>(sc.ctr)));
         SEGSORT_LAUNCH("bin", st, -1, 0.0, 8.0 * S,
-                       (k_bin<U, V, HAS_V><<<bin_grid(S), BIN_NT, 0, st>>>(db, de, S, b, sc, p.cc, km.end_bit, 0, bin_spt(S))));
+                       (k_bin<U, V, HAS_V, false><<<bin_grid(S), BIN_NT, 0, st>>>(db, de, S, b, sc, p.cc, km.end_bit, 0,
+                                                                                  bin_spt(S))));
         SEGSORT_LAUNCH("bin_large", st, -1, 0.0, 8.0 * S,
                        (k_bin_large_ids<<<(S + 255) / 256, 256, 0, st>>>(db, de, S, sc, p.cc)));
         Counters h;
@@ -952,7 +967,8 @@ struct Launcher {
         e = hipStreamSynchronize(st);
         if (e != hipSuccess) return fail(e, "segsort_sort(plain): sync");
         for (int c = 0; c < p.nclass; ++c)
-            if (h.tile_count[c]) launch_class(p.kind[c], b, sc, c, km, st, (int)h.tile_count[c], false, 1, p.cc.cap[c]);
+            if (h.tile_count[c])
+                launch_class(p.kind[c], b, sc, c, km, st, (int)h.tile_count[c], false, 1, p.cc.cap[c], false);
         const int nlarge = (int)h.level[MAX_LEVELS];
         if (nlarge > 0) {
             if (km.end_bit > km.begin_bit)
@@ -1288,7 +1304,7 @@ int segsort_perm_scatter_f32(const float* d_grad_out, const int* d_perm, const i
 
 int segsort_set_tuning(int key_width, int value_bytes, int wave_cap, int tile_enable, int grid_mult, int concurrent) {
     const int s = tuning_slot(key_width, value_bytes);
-    if (s < 0 || wave_cap < 2 || wave_cap > 512 || tile_enable < 0 || tile_enable > 0xFF || grid_mult < 1 ||
+    if (s < 0 || wave_cap < 2 || wave_cap > 512 || tile_enable < 0 || tile_enable > (0xFF | TUNE_LPT) || grid_mult < 1 ||
         grid_mult > 8 || concurrent < 0 || concurrent > 1)
         return fail(hipErrorInvalidValue, "segsort_set_tuning: out of range");
     std::lock_guard<std::mutex> lk(g_tuning_mu);

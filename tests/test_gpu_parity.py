@@ -122,18 +122,23 @@ def test_ragged_lognormal_2e24(pkg, torch_cuda, orc, dup):
     check(pkg, torch_cuda, orc, keys, vals, off)
 
 
+@pytest.mark.parametrize("order", ["table", "flipped"])
 @pytest.mark.parametrize("which", ["ragged", "ragged_dup", "equal_256"])
-def test_table_tuning_2e24(pkg, torch_cuda, orc, which):
+def test_table_tuning_2e24(pkg, torch_cuda, orc, which, order):
     """The 2^24-pair workloads of the stored tuning table's headline signature, sorted with
     the tuning the table picks for them (as the benchmark runs them: several persistent
     workgroups per slot, the binned ranges on a second stream while the MSD tier runs),
-    twice in a row, bit-exact against the oracle."""
+    twice in a row, bit-exact against the oracle; `flipped`: the same tuning with the
+    16-item tiles' order switched (longest first <-> binning order)."""
     if which == "equal_256":
         keys, vals, off = pkg.gen.equal_segments(1 << 24, 1 << 16, seed=0x5E650A7 + 1)
     else:
         keys, vals, off = pkg.gen.ragged_lognormal(0x5E650A7 + 2, dup_heavy=which == "ragged_dup")
     try:
         t = pkg.autotune.apply(pkg, keys.size, off.size - 1, 4, 4)  # the library's own table choice
+        if order == "flipped":
+            t = dict(t, tile_enable=t["tile_enable"] ^ 0x100)
+            pkg.set_tuning(4, 4, t["wave_cap"], t["tile_enable"], t["grid_mult"], t["concurrent"])
         print(f"[case] table tuning {t}", flush=True)
         for _ in range(2):
             check(pkg, torch_cuda, orc, keys, vals, off, algos=("batched",))
@@ -535,11 +540,13 @@ TUNINGS = [
     {"wave_cap": 2, "tile_enable": 0xFF, "grid_mult": 1},    # every group class, no wave tier
     {"wave_cap": 64, "tile_enable": 0xF0, "grid_mult": 1},   # no LDS tiles: everything above 64 is MSD
     {"wave_cap": 2, "tile_enable": 0x00, "grid_mult": 1},    # nothing but the MSD tiers above 2 items
+    {"wave_cap": 256, "tile_enable": 0x1EF, "grid_mult": 1},  # 16-item tiles longest first (length buckets)
+    {"wave_cap": 128, "tile_enable": 0x10B, "grid_mult": 3},  # the same with 256x4, 256x16 and 1024x16 only
 ]
 
 
 @pytest.mark.parametrize("concurrent", [1, 0], ids=["overlap", "one-stream"])
-@pytest.mark.parametrize("tuning", TUNINGS, ids=lambda t: f"wc{t['wave_cap']}-te{t['tile_enable']:02x}")
+@pytest.mark.parametrize("tuning", TUNINGS, ids=lambda t: f"wc{t['wave_cap']}-te{t['tile_enable']:03x}")
 def test_tuning_variants(pkg, torch_cuda, orc, tuning, concurrent):
     """Every class layout the tuning can select sorts bit-exactly (batched and plain),
     for 32- and 64-bit keys, on lengths that hit every group, wave, tile and MSD path,

@@ -22,7 +22,9 @@ constexpr int MAX_LEVELS = 8;        // MSD levels of the large tier (64-bit key
 // that started late while most CUs idle.
 constexpr int LPT_BUCKETS = 8;
 
-// Device-resident counters of one call; cleared by k_init at the start.
+// Device-resident counters of one call; cleared at the start of the call (by
+// the binning kernel's first workgroup, which then opens `gate`; k_init in the
+// fix-pass form). Everything before `gate` is cleared.
 struct Counters {
     uint32_t tile_count[MAX_QUEUES];
     uint32_t pad0[2];
@@ -31,6 +33,9 @@ struct Counters {
     unsigned long long level_items[MAX_LEVELS + 1];   // elements queued per MSD level
     uint32_t claim;                                   // next range of the depth-first tier
     uint32_t bcount[MAX_TILE_CLASSES][LPT_BUCKETS];   // binned ranges per length bucket (bucketed classes)
+    uint32_t bin_done;                                // binning workgroups past the gate
+    uint32_t pad1;
+    unsigned long long gate;  // the call's token once the counters above are clear; 0 again after binning
 };
 
 // MSD levels of the batched form: two multi-workgroup levels (the long
@@ -153,6 +158,35 @@ __global__ void k_init(Counters* ctr) {
     for (int i = threadIdx.x; i < (int)(sizeof(Counters) / 4); i += blockDim.x) p[i] = 0;
 }
 
+// The binning kernel clears the counters itself (no launch before it): its
+// workgroup 0 clears them and then stores the call's token (a 64-bit value the
+// host never repeats) into `gate`; every other workgroup does its loads and
+// LDS work first and waits for the token only before its first global atomic.
+// The last workgroup past the gate sets it back to 0, so a graph replaying the
+// call (same token) waits again. Workgroup 0 is dispatched first and never
+// waits, so the wait ends; it is bounded all the same (about 0.3 s).
+constexpr int GATE_SPIN_MAX = 1 << 21;
+
+// The counters are only ever read and updated by device-scope atomics after the
+// clear (and by later kernels), so the clear is made of device-scope atomic
+// stores, acknowledged by every wave before the barrier and so before the
+// gate's store: no cache write-back or invalidation on either side.
+__device__ __forceinline__ void clear_counters_and_open(Counters* ctr, unsigned long long token) {
+    uint32_t* p = (uint32_t*)ctr;
+    for (int i = threadIdx.x; i < (int)(offsetof(Counters, gate) / 4); i += blockDim.x)
+        __hip_atomic_store(p + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);  // every wave's clearing stores acknowledged (gfx9: vmcnt counts stores)
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(&ctr->gate, token, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void wait_gate(Counters* ctr, unsigned long long token) {
+    for (int i = 0; i < GATE_SPIN_MAX; ++i) {
+        if (__hip_atomic_load(&ctr->gate, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == token) return;
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
 // Queue slots taken by a whole workgroup at once: per-wave LDS counts, then
 // one global atomic per class for the slots (and one for the class's element
 // total), so that the few global queue counters see few atomics.
@@ -263,7 +297,7 @@ constexpr int BIN_SPT = 8;
 template <typename U, typename V, bool HAS_V, bool LPT>
 __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin, const int* __restrict__ d_end,
                                                 int num_segments, Bufs b, Sched sc, ClassCaps cc, int end_bit,
-                                                int large_to_level0, int spt) {
+                                                int large_to_level0, int spt, unsigned long long token) {
     constexpr int NB = LPT ? MAX_TILE_CLASSES * LPT_BUCKETS : 1;
     __shared__ uint32_t lcount[MAX_TILE_CLASSES];
     __shared__ uint32_t lbase[MAX_TILE_CLASSES];
@@ -275,7 +309,8 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
         litems[threadIdx.x] = 0;
     }
     if (LPT && threadIdx.x < NB) lbcount[threadIdx.x] = 0;
-    __syncthreads();
+    if (blockIdx.x == 0) clear_counters_and_open(sc.ctr, token);  // includes a barrier
+    else __syncthreads();
     int beg[BIN_SPT], n[BIN_SPT], cls[BIN_SPT];
     uint32_t slot[BIN_SPT];
     int bkt[BIN_SPT];  // LPT: length bucket of a bucketed class's range, else -1
@@ -299,8 +334,6 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
             if constexpr (HAS_V) buf_ptr<V>(b.v[BUF_OUT])[beg[j]] = ((const V*)b.v[BUF_IN])[beg[j]];
         }
         cls[j] = n[j] >= 2 ? class_of(cc, n[j]) : -2;
-        if (cls[j] == -1 && large_to_level0 == 1) push_large(sc, MSD_LEVEL0, beg[j], n[j], BUF_IN, end_bit, true);
-        if (cls[j] == -1 && large_to_level0 == 2) push_oversize(sc, beg[j], n[j]);
         bkt[j] = -1;
     }
     // per class: the wave's slots counted with ballots in registers, then one LDS
@@ -342,7 +375,13 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
         for (int j = 0; j < BIN_SPT; ++j)
             if (cls[j] == c && bkt[j] < 0) slot[j] += wb;
     }
+    if (threadIdx.x == 0 && blockIdx.x != 0) wait_gate(sc.ctr, token);
     __syncthreads();
+#pragma unroll
+    for (int j = 0; j < BIN_SPT; ++j) {
+        if (cls[j] == -1 && large_to_level0 == 1) push_large(sc, MSD_LEVEL0, beg[j], n[j], BUF_IN, end_bit, true);
+        if (cls[j] == -1 && large_to_level0 == 2) push_oversize(sc, beg[j], n[j]);
+    }
     // the block's global slots: every atomic issued before any result is used
     uint32_t g0 = 0, g1 = 0;
     if (threadIdx.x < (unsigned)cc.nclass && lcount[threadIdx.x])
@@ -351,6 +390,8 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
         g1 = atomicAdd(&sc.ctr->bcount[threadIdx.x / LPT_BUCKETS][threadIdx.x % LPT_BUCKETS], lbcount[threadIdx.x]);
     if (threadIdx.x < (unsigned)cc.nclass && litems[threadIdx.x])
         atomicAdd(&sc.ctr->tile_items[threadIdx.x], litems[threadIdx.x]);
+    if (threadIdx.x == BIN_NT - 1 && atomicAdd(&sc.ctr->bin_done, 1u) == gridDim.x - 1)
+        __hip_atomic_store(&sc.ctr->gate, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (threadIdx.x < (unsigned)cc.nclass) lbase[threadIdx.x] = g0;
     if (LPT && threadIdx.x < (unsigned)(cc.nclass * LPT_BUCKETS)) lbbase[threadIdx.x] = g1;
     __syncthreads();

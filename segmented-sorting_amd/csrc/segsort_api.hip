@@ -7,12 +7,14 @@
 #include "../../include/segsort.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -47,6 +49,20 @@ size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 // Workgroups of the oversize fallback (k_seg_lsd over the segments that break
 // a caller's length bound): empty in every correct call, so small.
 constexpr int FALLBACK_GRID = 32;
+
+// The binning kernel's gate token of one call: a process-wide counter, from a
+// random start, through a bijective 64-bit mix (splitmix64's finaliser), so no
+// two calls of the process share one and stale temp memory matches one only
+// by chance (2^-64).
+unsigned long long next_gate_token() {
+    static std::atomic<unsigned long long> counter{((unsigned long long)std::random_device{}() << 32) ^
+                                                   (unsigned long long)std::random_device{}()};
+    unsigned long long z = counter.fetch_add(1, std::memory_order_relaxed) + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return z ? z : 1;  // 0 is the closed gate
+}
 
 // Segments per binning thread: one while that keeps >= 64 blocks, at most BIN_SPT.
 int bin_spt(int S) { return std::min(BIN_SPT, std::max(1, (S + BIN_NT * 64 - 1) / (BIN_NT * 64))); }
@@ -760,18 +776,18 @@ struct Launcher {
         const Sched sc = make_sched(base, L, p, false, split, (size_t)S);
         const Bufs b = make_bufs(base, L, ki, ko, vi, vo);
         const size_t mark = Profiler::get().mark();
-        SEGSORT_LAUNCH("init", st, -1, 0.0, 0.0, (k_init<<<1, 64, 0, st>>>(sc.ctr)));
+        const unsigned long long token = next_gate_token();  // the binning kernel clears the counters
         const int large_mode = fallback ? 2 : 1;  // k_bin: segments above the largest tile to MSD level 0 or the fallback
         auto kbin = p.lpt ? k_bin<U, V, HAS_V, true> : k_bin<U, V, HAS_V, false>;
         if (split && !capturing) {
             // the fork event completes with the binning kernel itself (no separate marker on the caller's stream)
             SEGSORT_LAUNCH("bin", st, -1, 0.0, 8.0 * S,
                            hipExtLaunchKernelGGL(kbin, dim3(bin_grid(S)), dim3(BIN_NT), 0, st, nullptr, ss->fork, 0,
-                                                 db, de, S, b, sc, p.cc, km.end_bit, large_mode, bin_spt(S)));
+                                                 db, de, S, b, sc, p.cc, km.end_bit, large_mode, bin_spt(S), token));
         } else {
             SEGSORT_LAUNCH("bin", st, -1, 0.0, 8.0 * S,
                            (kbin<<<bin_grid(S), BIN_NT, 0, st>>>(db, de, S, b, sc, p.cc, km.end_bit, large_mode,
-                                                                 bin_spt(S))));
+                                                                 bin_spt(S), token)));
         }
         hipError_t e = hipSuccess;
         if (split) {
@@ -955,10 +971,9 @@ struct Launcher {
         const Sched sc = make_sched(base, L, p, true);
         const Bufs b = make_bufs(base, L, ki, ko, vi, vo);
         const size_t mark = Profiler::get().mark();
-        SEGSORT_LAUNCH("init", st, -1, 0.0, 0.0, (k_init<<<1, 64, 0, st>>>(sc.ctr)));
         SEGSORT_LAUNCH("bin", st, -1, 0.0, 8.0 * S,
                        (k_bin<U, V, HAS_V, false><<<bin_grid(S), BIN_NT, 0, st>>>(db, de, S, b, sc, p.cc, km.end_bit, 0,
-                                                                                  bin_spt(S))));
+                                                                                  bin_spt(S), next_gate_token())));
         SEGSORT_LAUNCH("bin_large", st, -1, 0.0, 8.0 * S,
                        (k_bin_large_ids<<<(S + 255) / 256, 256, 0, st>>>(db, de, S, sc, p.cc)));
         Counters h;

@@ -26,6 +26,7 @@ def _setup(pkg, torch):
                      32, stream, 0, 0, 4, "batched")
         assert rc == 0, pkg.last_error()
 
+    call.temp = temp
     return keys, vals, off, kout, vout, call
 
 
@@ -210,3 +211,39 @@ def test_segmented_sort_on_side_stream(pkg, torch_cuda, orc):
         del dk, dv
     with pytest.raises(pkg.SegsortError):
         pkg.segmented_sort(src_k.view(torch.uint32), od.to(torch.int64), src_v.view(torch.uint32))
+
+
+
+@pytest.mark.parametrize("fill", [0x00, 0xA5, 0xFF])
+def test_replays_after_temp_overwritten(pkg, torch_cuda, orc, fill):
+    """The call clears its own counters (no launch before the binning kernel): a graph
+    of the call replayed after the caller overwrote the whole temp storage (zeros,
+    0xA5 bytes, all ones), and a direct call after the same, are correct each time."""
+    torch = torch_cuda
+    keys, vals, off, kout, vout, call = _setup(pkg, torch)
+    ek, ev = orc.seg_sort(keys, vals, offsets=off)
+    st = torch.cuda.Stream()
+    with torch.cuda.stream(st):
+        call(st.cuda_stream)  # warm-up outside the capture; sizes the temp storage
+    torch.cuda.synchronize()
+    temp = call.temp[0]
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=st):
+        call(torch.cuda.current_stream().cuda_stream)
+    for rep in range(3):
+        print(f"[case] replay {rep} after temp filled with {fill:#04x}", flush=True)
+        kout.fill_(0)
+        vout.fill_(0)
+        temp.fill_(fill)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(kout.cpu().numpy().view(np.uint32), ek), (rep, "replay")
+        assert np.array_equal(vout.cpu().numpy().view(np.uint32), ev), (rep, "replay")
+        kout.fill_(0)
+        vout.fill_(0)
+        temp.fill_(fill)
+        call(torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        assert np.array_equal(kout.cpu().numpy().view(np.uint32), ek), (rep, "direct")
+        assert np.array_equal(vout.cpu().numpy().view(np.uint32), ev), (rep, "direct")

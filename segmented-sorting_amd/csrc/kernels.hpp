@@ -377,12 +377,8 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
     }
     if (threadIdx.x == 0 && blockIdx.x != 0) wait_gate(sc.ctr, token);
     __syncthreads();
-#pragma unroll
-    for (int j = 0; j < BIN_SPT; ++j) {
-        if (cls[j] == -1 && large_to_level0 == 1) push_large(sc, MSD_LEVEL0, beg[j], n[j], BUF_IN, end_bit, true);
-        if (cls[j] == -1 && large_to_level0 == 2) push_oversize(sc, beg[j], n[j]);
-    }
     // the block's global slots: every atomic issued before any result is used
+    // (the long segments' queue atomics below included)
     uint32_t g0 = 0, g1 = 0;
     if (threadIdx.x < (unsigned)cc.nclass && lcount[threadIdx.x])
         g0 = atomicAdd(&sc.ctr->tile_count[threadIdx.x], lcount[threadIdx.x]);
@@ -392,6 +388,11 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
         atomicAdd(&sc.ctr->tile_items[threadIdx.x], litems[threadIdx.x]);
     if (threadIdx.x == BIN_NT - 1 && atomicAdd(&sc.ctr->bin_done, 1u) == gridDim.x - 1)
         __hip_atomic_store(&sc.ctr->gate, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int j = 0; j < BIN_SPT; ++j) {
+        if (cls[j] == -1 && large_to_level0 == 1) push_large(sc, MSD_LEVEL0, beg[j], n[j], BUF_IN, end_bit, true);
+        if (cls[j] == -1 && large_to_level0 == 2) push_oversize(sc, beg[j], n[j]);
+    }
     if (threadIdx.x < (unsigned)cc.nclass) lbase[threadIdx.x] = g0;
     if (LPT && threadIdx.x < (unsigned)(cc.nclass * LPT_BUCKETS)) lbbase[threadIdx.x] = g1;
     __syncthreads();

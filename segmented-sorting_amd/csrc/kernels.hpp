@@ -299,14 +299,30 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
                                                 int num_segments, Bufs b, Sched sc, ClassCaps cc, int end_bit,
                                                 int large_to_level0, int spt, unsigned long long token) {
     constexpr int NB = LPT ? MAX_TILE_CLASSES * LPT_BUCKETS : 1;
+    constexpr int NC = LPT ? MAX_TILE_CLASSES : 1;
     __shared__ uint32_t lcount[MAX_TILE_CLASSES];
     __shared__ uint32_t lbase[MAX_TILE_CLASSES];
     __shared__ unsigned long long litems[MAX_TILE_CLASSES];
+    // per-class tables that the per-range steps index by the range's class (an
+    // LDS read instead of a select over the kernel arguments)
+    __shared__ TileItem* ltq[MAX_TILE_CLASSES];
+    __shared__ uint32_t ltq_cap[MAX_TILE_CLASSES];
     __shared__ uint32_t lbcount[NB];  // LPT: per class and length bucket
     __shared__ uint32_t lbbase[NB];
+    __shared__ TileItem* lbq[NC];     // LPT: bucket regions (nullptr: class not bucketed)
+    __shared__ uint32_t lbcap[NC];
+    __shared__ int lcap[NC];
     if (threadIdx.x < MAX_TILE_CLASSES) {
-        lcount[threadIdx.x] = 0;
-        litems[threadIdx.x] = 0;
+        const int c = threadIdx.x;
+        lcount[c] = 0;
+        litems[c] = 0;
+        ltq[c] = sc.tq[c];
+        ltq_cap[c] = sc.tq_cap[c];
+        if constexpr (LPT) {
+            lbq[c] = sc.bq[c];
+            lbcap[c] = sc.bcap[c];
+            lcap[c] = cc.cap[c];
+        }
     }
     if (LPT && threadIdx.x < NB) lbcount[threadIdx.x] = 0;
     if (blockIdx.x == 0) clear_counters_and_open(sc.ctr, token);  // includes a barrier
@@ -336,6 +352,18 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
         cls[j] = n[j] >= 2 ? class_of(cc, n[j]) : -2;
         bkt[j] = -1;
     }
+    if constexpr (LPT) {
+        // ranges of a bucketed class take a slot of their length bucket: an LDS
+        // atomic per range (the long ranges of a mixed call spread over the buckets)
+#pragma unroll
+        for (int j = 0; j < BIN_SPT; ++j) {
+            if (cls[j] >= 0 && lbq[cls[j]] != nullptr) {
+                const int c = cls[j];
+                bkt[j] = lpt_bucket(c > 0 ? lcap[c - 1] + 1 : 2, lcap[c], n[j]);
+                slot[j] = atomicAdd(&lbcount[c * LPT_BUCKETS + bkt[j]], 1u);
+            }
+        }
+    }
     // per class: the wave's slots counted with ballots in registers, then one LDS
     // atomic per class and wave for the wave's base inside the block
     uint32_t wbase[MAX_TILE_CLASSES];
@@ -344,22 +372,14 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
         wbase[c] = 0;
         if (c >= cc.nclass) continue;
         uint32_t run = 0, items = 0;
-        const bool bucketed = LPT && sc.bq[c] != nullptr;  // wave-uniform
 #pragma unroll
         for (int j = 0; j < BIN_SPT; ++j) {
             const bool mine = cls[j] == c;
-            const uint64_t m = __ballot(mine);
+            const bool queued = mine && bkt[j] < 0;  // bucketed ranges have their slot
+            const uint64_t m = __ballot(queued);
             if (mine) items += (uint32_t)n[j];
-            if (!bucketed) {
-                if (mine) slot[j] = run + lanes_below(m);
-                run += (uint32_t)__popcll(m);
-            } else if (mine) {
-                // a slot of the range's length bucket: an LDS atomic per range (the
-                // long ranges of a mixed call spread over the buckets)
-                const int lo = c > 0 ? cc.cap[c - 1] + 1 : 2;
-                bkt[j] = lpt_bucket(lo, cc.cap[c], n[j]);
-                slot[j] = atomicAdd(&lbcount[c * LPT_BUCKETS + bkt[j]], 1u);
-            }
+            if (queued) slot[j] = run + lanes_below(m);
+            run += (uint32_t)__popcll(m);
         }
         if (__ballot(items != 0) == 0) continue;
         items = wave_sum(items);
@@ -400,10 +420,11 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
     for (int j = 0; j < BIN_SPT; ++j) {
         if (cls[j] >= 0 && bkt[j] < 0) {
             const uint32_t g = lbase[cls[j]] + slot[j];
-            if (g < sc.tq_cap[cls[j]]) sc.tq[cls[j]][g] = make_ti(beg[j], n[j], BUF_IN);
+            if (g < ltq_cap[cls[j]]) ltq[cls[j]][g] = make_ti(beg[j], n[j], BUF_IN);
         } else if (LPT && cls[j] >= 0) {
             const uint32_t g = lbbase[cls[j] * LPT_BUCKETS + bkt[j]] + slot[j];
-            if (g < sc.bcap[cls[j]]) sc.bq[cls[j]][(size_t)bkt[j] * sc.bcap[cls[j]] + g] = make_ti(beg[j], n[j], BUF_IN);
+            const uint32_t cap = lbcap[cls[j]];
+            if (g < cap) lbq[cls[j]][(size_t)bkt[j] * cap + g] = make_ti(beg[j], n[j], BUF_IN);
         }
     }
 }

@@ -216,7 +216,9 @@ int segsort_perm_scatter_f32(const float* d_grad_out, const int* d_perm,
  * one-workgroup-per-segment LSD (plain). Bit 8 (0x100): the batched form's
  * 16-item tile classes take their ranges longest first (length buckets
  * filled by the binning pass); it pays off for mixed lengths and costs a few
- * percent for equal ones. Default 0xEF.
+ * percent for equal ones. Bit 9 (0x200): the 4096-item tile class without its
+ * half-depth build for ranges of at most 2048 items (faster when that class
+ * holds full-size ranges). Default 0xEF.
  * grid_mult: persistent workgroups per CU per resident slot (1..8).
  * concurrent: 1 sorts the ranges that fit a class on a second stream while
  * the multi-workgroup tier partitions the longer segments on the caller's

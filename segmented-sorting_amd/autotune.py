@@ -7,9 +7,10 @@ the enabled LDS tile classes (`tile_enable` bits 0..3, capacities from
 segsort_tile_caps), and the multi-workgroup MSD tier above the largest enabled
 class; `grid_mult` sizes the persistent grids.
 `concurrent` (the binned ranges sorted on a second stream while the MSD tier
-runs), the larger grid_mult values and `tile_enable` bit 8 (the 16-item tile
-classes take their ranges longest first, from length buckets) are tried on the
-finalists of the screening stage.
+runs), the larger grid_mult values, `tile_enable` bit 8 (the 16-item tile
+classes take their ranges longest first, from length buckets) and bit 9 (the
+4096-item tile class without its half-depth build) are tried on the finalists of
+the screening stage.
 
 `tune()` times every candidate on a resident workload and keeps the fastest one
 whose output is bit-identical to the default tuning's. Results are stored in
@@ -59,6 +60,7 @@ def _parse(sig: str):
 TILE_MASKS = (0xF, 0xB, 0x7, 0xD, 0xE, 0x3)
 GROUP_MASKS = (0x00, 0xE0, 0xA0, 0x60, 0xF0)
 LPT = 0x100  # bit 8: the 16-item tile classes longest first
+NO_HALF = 0x200  # bit 9: the 4096-item tile class without its half-depth build
 
 
 # persistent grid sizes tried on the finalists (workgroups per resident slot): more than one
@@ -219,9 +221,9 @@ def main(argv=None):
     """Two stages per signature: every candidate timed briefly on every workload of
     the signature, then the tier layouts (wave_cap, tile_enable) of the best few
     re-timed with six times the steps at each grid_mult of FINAL_GRID_MULTS, with
-    `concurrent` off and on and the 16-item tiles in binning order and longest
-    first; the stored tuning minimises the sum of times relative to each workload's
-    best."""
+    `concurrent` off and on, the 16-item tiles in binning order and longest first,
+    and (when the 4096-item class is on) with and without its half-depth build; the
+    stored tuning minimises the sum of times relative to each workload's best."""
     import argparse
 
     import torch
@@ -262,8 +264,9 @@ def main(argv=None):
                 shapes.append((t["wave_cap"], t["tile_enable"]))
             if len(shapes) == args.finalists:
                 break
-        finalists = [{"wave_cap": wc, "tile_enable": te | lp, "grid_mult": gm, "concurrent": k}
-                     for wc, te in shapes for gm in FINAL_GRID_MULTS for k in (0, 1) for lp in (0, LPT)]
+        finalists = [{"wave_cap": wc, "tile_enable": te | lp | nh, "grid_mult": gm, "concurrent": k}
+                     for wc, te in shapes for gm in FINAL_GRID_MULTS for k in (0, 1) for lp in (0, LPT)
+                     for nh in ((0, NO_HALF) if te & 0x2 else (0,))]
         refined = []
         for wl, dev, _, mx in runs:
             _, res = tune(pkg, torch, *dev, steps=args.steps * 6, verbose=False, cands=finalists, max_len=mx)

@@ -565,7 +565,10 @@ __device__ __forceinline__ void tile_loop_pipelined(typename RadixTile<NT, IPT, 
 // LPT (16-item tiles): the class's length buckets are taken longest first,
 // then its queue. A separate build: the bucket walk's state costs the kernel
 // scalar registers (spills) that equal-length segments gain nothing for.
-template <int NT, int IPT, typename U, typename V, bool HAS_V, bool LPT>
+// HALF (tiles of at most 4096 items): ranges of at most half the tile use the
+// half-depth build below. Also a separate build: the two builds in one kernel
+// make it spill (256x16: 12 bytes per lane) and slow full-size ranges.
+template <int NT, int IPT, typename U, typename V, bool HAS_V, bool LPT, bool HALF>
 __global__ __launch_bounds__(NT, 4) void k_tile_tier(Bufs b, Sched sc, int cls, KeyMode km, int persistent) {
     using T = RadixTile<NT, IPT, U, V, HAS_V>;
     if constexpr (IPT > 4) {
@@ -598,7 +601,7 @@ __global__ __launch_bounds__(NT, 4) void k_tile_tier(Bufs b, Sched sc, int cls, 
             const V* vs = (const V*)b.v[src];
             U* ko = buf_ptr<U>(b.k[BUF_OUT]);
             V* vo = buf_ptr<V>(b.v[BUF_OUT]);
-            if (NT * IPT <= 4096 && n <= T2::TILE) T2::sort(sm.half, ks, vs, ko, vo, cur.begin, n, km);
+            if (HALF && NT * IPT <= 4096 && n <= T2::TILE) T2::sort(sm.half, ks, vs, ko, vo, cur.begin, n, km);
             else T::sort(sm.full, ks, vs, ko, vo, cur.begin, n, km);
         }
         return;

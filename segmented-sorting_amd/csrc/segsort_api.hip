@@ -106,6 +106,12 @@ struct TableEntry {
 // signatures where it measured faster, and the default leaves it off.
 constexpr int TUNE_LPT = 0x100;
 
+// tile_enable bit 9: the 4096-item tile (256x16) without its half-depth build
+// (8 items per thread for ranges of at most 2048 items), which otherwise makes
+// the kernel spill. Measured: 2^12 equal segments of 4096 0.1181 -> 0.1137 ms
+// without it, ragged 2^24 0.1953 -> 0.2029 ms (its 1025..2048-item ranges want it).
+constexpr int TUNE_NO_HALF = 0x200;
+
 constexpr Tuning DEFAULT_TUNING = {256, 0xEF, 1, 0};
 
 // Overrides installed by segsort_set_tuning, per (key width, value width) slot:
@@ -443,7 +449,8 @@ struct Plan {
     int kind[MAX_TILE_CLASSES];
     int nclass;
     int tile_max;
-    bool lpt;  // batched form: length buckets for the 16-item tile classes (TUNE_LPT)
+    bool lpt;   // batched form: length buckets for the 16-item tile classes (TUNE_LPT)
+    bool half;  // the 4096-item tile's half-depth build (not TUNE_NO_HALF)
 };
 
 // Classes in increasing capacity: the enabled wave-group classes, the wave
@@ -453,6 +460,7 @@ Plan make_plan(const Tuning& tu) {
     Plan p;
     p.nclass = 0;
     p.lpt = (tu.tile_enable & TUNE_LPT) != 0;
+    p.half = (tu.tile_enable & TUNE_NO_HALF) == 0;
     int last = 1;
     for (int g = 0; g < 4; ++g) {
         if (!(tu.tile_enable & (0x10 << g))) continue;
@@ -556,11 +564,16 @@ struct Launcher {
 
     template <int NT, int IPT>
     static void launch_tile(Bufs b, Sched sc, int cls, KeyMode km, hipStream_t st, int grid, bool persistent,
-                            int grid_mult, bool lpt) {
-        // the length-bucket build exists for the 16-item tiles only
-        auto k = k_tile_tier<NT, IPT, U, V, HAS_V, false>;
-        if constexpr (IPT > 4)
-            if (lpt) k = k_tile_tier<NT, IPT, U, V, HAS_V, true>;
+                            int grid_mult, bool lpt, bool half) {
+        // the length-bucket builds exist for the 16-item tiles, the half-depth
+        // ones for those of at most 4096 items
+        auto k = k_tile_tier<NT, IPT, U, V, HAS_V, false, false>;
+        if constexpr (IPT > 4 && NT * IPT <= 4096) {
+            k = lpt ? (half ? k_tile_tier<NT, IPT, U, V, HAS_V, true, true> : k_tile_tier<NT, IPT, U, V, HAS_V, true, false>)
+                    : (half ? k_tile_tier<NT, IPT, U, V, HAS_V, false, true> : k_tile_tier<NT, IPT, U, V, HAS_V, false, false>);
+        } else if constexpr (IPT > 4) {
+            if (lpt) k = k_tile_tier<NT, IPT, U, V, HAS_V, true, false>;
+        }
         // persistent: every resident slot, but no more workgroups than the class
         // can have ranges (`grid` > 0: that bound)
         int g = persistent ? resident_blocks(k, NT) * grid_mult : grid;
@@ -587,7 +600,7 @@ struct Launcher {
     // ranges to sort (non-persistent, the plain form), or (persistent) a bound on
     // them that caps the grid (0: none).
     static void launch_class(int kind, Bufs b, Sched sc, int cls, KeyMode km, hipStream_t st, int grid, bool persistent,
-                             int grid_mult, int wave_max, bool lpt) {
+                             int grid_mult, int wave_max, bool lpt, bool half) {
         switch (kind) {
         case 0: {
             // ranges of at most 256 items: the smaller-LDS, higher-occupancy build
@@ -599,10 +612,10 @@ struct Launcher {
                                (k<<<g, 256, 0, st>>>(b, sc, cls, km, persistent ? 1 : 0, WAVE_RANK_MAX)));
             break;
         }
-        case 1: launch_tile<C::NT0, C::IPT0>(b, sc, cls, km, st, grid, persistent, grid_mult, lpt); break;
-        case 2: launch_tile<C::NT1, C::IPT1>(b, sc, cls, km, st, grid, persistent, grid_mult, lpt); break;
-        case 3: launch_tile<C::NT2, C::IPT2>(b, sc, cls, km, st, grid, persistent, grid_mult, lpt); break;
-        case 4: launch_tile<C::NT3, C::IPT3>(b, sc, cls, km, st, grid, persistent, grid_mult, lpt); break;
+        case 1: launch_tile<C::NT0, C::IPT0>(b, sc, cls, km, st, grid, persistent, grid_mult, lpt, half); break;
+        case 2: launch_tile<C::NT1, C::IPT1>(b, sc, cls, km, st, grid, persistent, grid_mult, lpt, half); break;
+        case 3: launch_tile<C::NT2, C::IPT2>(b, sc, cls, km, st, grid, persistent, grid_mult, lpt, half); break;
+        case 4: launch_tile<C::NT3, C::IPT3>(b, sc, cls, km, st, grid, persistent, grid_mult, lpt, half); break;
         case KIND_GROUP + 0: launch_group<4>(b, sc, cls, km, st, grid, persistent, grid_mult); break;
         case KIND_GROUP + 1: launch_group<8>(b, sc, cls, km, st, grid, persistent, grid_mult); break;
         case KIND_GROUP + 2: launch_group<16>(b, sc, cls, km, st, grid, persistent, grid_mult); break;
@@ -662,7 +675,7 @@ struct Launcher {
                 continue;
             }
             launch_class(p.kind[c], b, sc, q0 + c, km, st, max_ranges(p, c, n_items), true, grid_mult, p.cc.cap[c],
-                         p.lpt && q0 == 0);
+                         p.lpt && q0 == 0, p.half);
         }
     }
 
@@ -983,7 +996,7 @@ struct Launcher {
         if (e != hipSuccess) return fail(e, "segsort_sort(plain): sync");
         for (int c = 0; c < p.nclass; ++c)
             if (h.tile_count[c])
-                launch_class(p.kind[c], b, sc, c, km, st, (int)h.tile_count[c], false, 1, p.cc.cap[c], false);
+                launch_class(p.kind[c], b, sc, c, km, st, (int)h.tile_count[c], false, 1, p.cc.cap[c], false, p.half);
         const int nlarge = (int)h.level[MAX_LEVELS];
         if (nlarge > 0) {
             if (km.end_bit > km.begin_bit)
@@ -1319,7 +1332,7 @@ int segsort_perm_scatter_f32(const float* d_grad_out, const int* d_perm, const i
 
 int segsort_set_tuning(int key_width, int value_bytes, int wave_cap, int tile_enable, int grid_mult, int concurrent) {
     const int s = tuning_slot(key_width, value_bytes);
-    if (s < 0 || wave_cap < 2 || wave_cap > 512 || tile_enable < 0 || tile_enable > (0xFF | TUNE_LPT) || grid_mult < 1 ||
+    if (s < 0 || wave_cap < 2 || wave_cap > 512 || tile_enable < 0 || tile_enable > (0xFF | TUNE_LPT | TUNE_NO_HALF) || grid_mult < 1 ||
         grid_mult > 8 || concurrent < 0 || concurrent > 1)
         return fail(hipErrorInvalidValue, "segsort_set_tuning: out of range");
     std::lock_guard<std::mutex> lk(g_tuning_mu);

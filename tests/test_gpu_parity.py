@@ -542,6 +542,8 @@ TUNINGS = [
     {"wave_cap": 2, "tile_enable": 0x00, "grid_mult": 1},    # nothing but the MSD tiers above 2 items
     {"wave_cap": 256, "tile_enable": 0x1EF, "grid_mult": 1},  # 16-item tiles longest first (length buckets)
     {"wave_cap": 128, "tile_enable": 0x10B, "grid_mult": 3},  # the same with 256x4, 256x16 and 1024x16 only
+    {"wave_cap": 128, "tile_enable": 0x207, "grid_mult": 2},  # 256x16 without its half-depth build
+    {"wave_cap": 256, "tile_enable": 0x30F, "grid_mult": 1},  # the same, longest first
 ]
 
 

@@ -1391,11 +1391,18 @@ constexpr bool wave_finish_by_slot() { return R >= 4; }
 // The wave's LDS region for wave_msd_sort, for ranges of at most MAXN items.
 // The bucket path of 32-bit keys reads keys[] and vals[] together as one
 // array of MAXN 64-bit sort words (they are adjacent and 16-byte aligned).
+// Ranges of at most 256 items with 32-bit keys also stage their values there
+// (by original position) for the bucket finish, which then gathers them from
+// LDS instead of reading them from global memory a second time.
+template <typename U, int MAXN>
+constexpr bool wave_stage_values() { return sizeof(U) == 4 && MAXN <= 256; }
+
 template <typename U, typename V, int MAXN = WAVE_SORT_MAX>
 struct alignas(16) WaveSmem {
     U keys[MAXN];        // raw keys in bucket order; 64-bit keys: order keys of the bucket path
     V vals[MAXN];        // values by original position
     uint16_t pos[sizeof(U) == 8 ? MAXN : 1];  // 64-bit keys: original position of each bucket-ordered key
+    V vstage[wave_stage_values<U, MAXN>() ? MAXN : 1];  // values by original position (bucket finish)
     uint32_t cnt[RADIX + 1];      // running digit counts, then bucket starts (cnt[buckets] = n)
     uint32_t dummy[WAVE];         // store target of the lanes that do not advance a count
     static_assert(sizeof(U) == 8 || sizeof(U) * MAXN + sizeof(V) * MAXN >= 8 * MAXN, "sort words overlay");
@@ -1532,12 +1539,14 @@ __device__ bool wave_msd_sort(const U* ks, const V* vs, U* ko, V* vo, int64_t be
     }
     if (wave_scan_counts(ws, nbk, n) > (uint32_t)guard) return false;
     // sort words in bucket order: (order key, original position)
+    constexpr bool STAGE_V = HAS_V && wave_finish_by_slot<R>() && wave_stage_values<U, MAXN>();
     uint32_t st[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int p = r * WAVE + l;
         st[r] = ws.cnt[dg[r]];
         if (p < n) {
+            if constexpr (STAGE_V) ws.vstage[p] = v[r];
             const uint32_t at = st[r] + rk[r];
             if constexpr (sizeof(U) == 4) {
                 reinterpret_cast<uint64_t*>(ws.keys)[at] = (uint64_t(ok[r]) << 32) | uint64_t(p);
@@ -1578,7 +1587,8 @@ __device__ bool wave_msd_sort(const U* ks, const V* vs, U* ko, V* vo, int64_t be
                     }
                 }
                 at32(ko + begin, s + (int)c) = full ? twiddle_out<U>(mk, km.kind, km.desc) : at32(ks + begin, p);
-                if constexpr (HAS_V) at32(vo + begin, s + (int)c) = at32(vs + begin, p);
+                if constexpr (STAGE_V) at32(vo + begin, s + (int)c) = ws.vstage[p];
+                else if constexpr (HAS_V) at32(vo + begin, s + (int)c) = at32(vs + begin, p);
             }
         }
         return true;

@@ -311,6 +311,7 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
     // LDS read instead of a select over the kernel arguments)
     __shared__ TileItem* ltq[MAX_TILE_CLASSES];
     __shared__ uint32_t ltq_cap[MAX_TILE_CLASSES];
+    __shared__ uint32_t lwbase[BIN_NT / WAVE][MAX_TILE_CLASSES];  // each wave's slot base per class
     __shared__ uint32_t lbcount[NB];  // LPT: per class and length bucket
     __shared__ uint32_t lbbase[NB];
     __shared__ TileItem* lbq[NC];     // LPT: bucket regions (nullptr: class not bucketed)
@@ -369,12 +370,12 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
         }
     }
     // per class: the wave's slots counted with ballots in registers, then one LDS
-    // atomic per class and wave for the wave's base inside the block
-    uint32_t wbase[MAX_TILE_CLASSES];
-#pragma unroll
-    for (int c = 0; c < MAX_TILE_CLASSES; ++c) {
-        wbase[c] = 0;
-        if (c >= cc.nclass) continue;
+    // atomic per class and wave for the wave's base inside the block (a loop over
+    // the call's classes, not unrolled: this kernel starts every call with a cold
+    // instruction cache)
+    const int w = (int)(threadIdx.x / WAVE);
+#pragma unroll 1
+    for (int c = 0; c < cc.nclass; ++c) {
         uint32_t run = 0, items = 0;
 #pragma unroll
         for (int j = 0; j < BIN_SPT; ++j) {
@@ -385,20 +386,17 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
             if (queued) slot[j] = run + lanes_below(m);
             run += (uint32_t)__popcll(m);
         }
-        if (__ballot(items != 0) == 0) continue;
+        if (__ballot(items != 0) == 0) continue;  // no range of the class in this wave
         items = wave_sum(items);
         if (lane_id() == 0) {
-            if (run) wbase[c] = atomicAdd(&lcount[c], run);
+            lwbase[w][c] = run ? atomicAdd(&lcount[c], run) : 0u;
             atomicAdd(&litems[c], (unsigned long long)items);
         }
     }
+    // (the wave's own LDS writes above complete before its reads here)
 #pragma unroll
-    for (int c = 0; c < MAX_TILE_CLASSES; ++c) {
-        const uint32_t wb = (uint32_t)__builtin_amdgcn_readfirstlane((int)wbase[c]);
-#pragma unroll
-        for (int j = 0; j < BIN_SPT; ++j)
-            if (cls[j] == c && bkt[j] < 0) slot[j] += wb;
-    }
+    for (int j = 0; j < BIN_SPT; ++j)
+        if (cls[j] >= 0 && bkt[j] < 0) slot[j] += lwbase[w][cls[j]];
     if (threadIdx.x == 0 && blockIdx.x != 0) wait_gate(sc.ctr, token);
     __syncthreads();
     // the block's global slots: every atomic issued before any result is used

@@ -493,8 +493,13 @@ struct RadixTile {
                         }
                     }
                     const int r = s0 + (int)c;
-                    at32(ko + begin, r) = full ? twiddle_out<U>(mk, km.kind, km.desc) : at32(ks + begin, p);
-                    if constexpr (HAS_V) at32(vo + begin, r) = at32(vs + begin, p);
+                    // loads before stores: the memory counter completes in order, so a
+                    // load issued after the key store would also wait for that store
+                    V val{};
+                    if constexpr (HAS_V) val = at32(vs + begin, p);
+                    const U kout = full ? twiddle_out<U>(mk, km.kind, km.desc) : at32(ks + begin, p);
+                    at32(ko + begin, r) = kout;
+                    if constexpr (HAS_V) at32(vo + begin, r) = val;
                 }
             }
         }
@@ -887,8 +892,12 @@ struct RadixTile {
                                     }
                                 }
                                 const int r = s0 + (int)c;
-                                at32(ko + begin, r) = full ? twiddle_out<U>(mk, km.kind, km.desc) : at32(ks + begin, p);
-                                if constexpr (HAS_V) at32(vo + begin, r) = at32(vs + begin, p);
+                                V val{};  // loads before stores (in-order memory counter)
+                                if constexpr (HAS_V) val = at32(vs + begin, p);
+                                const U kout =
+                                    full ? twiddle_out<U>(mk, km.kind, km.desc) : at32(ks + begin, p);
+                                at32(ko + begin, r) = kout;
+                                if constexpr (HAS_V) at32(vo + begin, r) = val;
                             }
                         }
                     }
@@ -1586,9 +1595,12 @@ __device__ bool wave_msd_sort(const U* ks, const V* vs, U* ko, V* vo, int64_t be
                         c += (kq < mk) | ((kq == mk) & ((uint32_t)ws.pos[q] < p));
                     }
                 }
-                at32(ko + begin, s + (int)c) = full ? twiddle_out<U>(mk, km.kind, km.desc) : at32(ks + begin, p);
-                if constexpr (STAGE_V) at32(vo + begin, s + (int)c) = ws.vstage[p];
-                else if constexpr (HAS_V) at32(vo + begin, s + (int)c) = at32(vs + begin, p);
+                V val{};  // loads before stores (in-order memory counter)
+                if constexpr (STAGE_V) val = ws.vstage[p];
+                else if constexpr (HAS_V) val = at32(vs + begin, p);
+                const U kout = full ? twiddle_out<U>(mk, km.kind, km.desc) : at32(ks + begin, p);
+                at32(ko + begin, s + (int)c) = kout;
+                if constexpr (HAS_V) at32(vo + begin, s + (int)c) = val;
             }
         }
         return true;

@@ -863,6 +863,18 @@ struct RadixTile {
                     // broadcast reads); the sort word names the item's input position
                     const bool full = rm == Bits<U>::ALL;
                     if (act) {
+                        // every value gather issued before any store (the memory
+                        // counter completes in order); the positions are read
+                        // from the sort words again below
+                        V vg[IPT];
+                        if constexpr (HAS_V && sizeof(U) == 4) {
+                            const uint64_t* w = reinterpret_cast<const uint64_t*>(sm.keys);
+#pragma unroll
+                            for (int j = 0; j < IPT; ++j) {
+                                const int q0 = (lb + j * WAVE);
+                                vg[j] = at32(vs + begin, q0 < n ? (uint32_t)w[q0] : 0u);
+                            }
+                        }
 #pragma unroll
                         for (int j = 0; j < IPT; ++j) {
                             const int q0 = (lb + j * WAVE);
@@ -893,7 +905,8 @@ struct RadixTile {
                                 }
                                 const int r = s0 + (int)c;
                                 V val{};  // loads before stores (in-order memory counter)
-                                if constexpr (HAS_V) val = at32(vs + begin, p);
+                                if constexpr (HAS_V && sizeof(U) == 4) val = vg[j];
+                                else if constexpr (HAS_V) val = at32(vs + begin, p);
                                 const U kout =
                                     full ? twiddle_out<U>(mk, km.kind, km.desc) : at32(ks + begin, p);
                                 at32(ko + begin, r) = kout;
@@ -1568,6 +1581,18 @@ __device__ bool wave_msd_sort(const U* ks, const V* vs, U* ko, V* vo, int64_t be
     if constexpr (wave_finish_by_slot<R>()) {
         // lane l finishes the items at slots r*64 + l of the bucket order
         const bool full = rm == Bits<U>::ALL;
+        // values not staged in LDS: every gather issued before any store (the
+        // memory counter completes in order), positions from the sort words
+        constexpr bool GATHER_FIRST = HAS_V && !STAGE_V && sizeof(U) == 4;
+        V vg[R];
+        if constexpr (GATHER_FIRST) {
+            const uint64_t* w = reinterpret_cast<const uint64_t*>(ws.keys);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int q0 = r * WAVE + l;
+                vg[r] = at32(vs + begin, q0 < n ? (uint32_t)w[q0] : 0u);
+            }
+        }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int q0 = r * WAVE + l;
@@ -1597,6 +1622,7 @@ __device__ bool wave_msd_sort(const U* ks, const V* vs, U* ko, V* vo, int64_t be
                 }
                 V val{};  // loads before stores (in-order memory counter)
                 if constexpr (STAGE_V) val = ws.vstage[p];
+                else if constexpr (GATHER_FIRST) val = vg[r];
                 else if constexpr (HAS_V) val = at32(vs + begin, p);
                 const U kout = full ? twiddle_out<U>(mk, km.kind, km.desc) : at32(ks + begin, p);
                 at32(ko + begin, s + (int)c) = kout;

@@ -3,6 +3,8 @@
 // the batched sequence needs no host synchronisation and can be captured.
 #pragma once
 
+#include <type_traits>
+
 #include "tile.hpp"
 
 namespace ssort {
@@ -821,17 +823,6 @@ __global__ __launch_bounds__(NT) void k_large_hist(Bufs b, Sched sc, int level, 
     }
 }
 
-// Items of digit d in a chunk (record ci, histogram word h of digit d) at the
-// level's position: the chunk's histogram when that is its own top varying
-// position, else all of its n items or none. The caller reads the histogram
-// word whether or not it is used (it may be stale when the chunk's top is
-// lower), so that both loads of a chunk are in flight at once.
-__device__ __forceinline__ uint32_t chunk_count_of(const ChunkInfo& ci, uint32_t h, int pstar, int shift,
-                                                   unsigned long long dmask, uint32_t d, int n) {
-    if (ci.lp == pstar) return h;
-    return (uint32_t)((ci.kand >> shift) & dmask) == d ? (uint32_t)n : 0u;
-}
-
 // Pass 2 of a level: the stable partition of every chunk by the range's
 // highest varying digit. Each workgroup derives its chunk's destinations
 // itself from the range's chunk records (no separate scan launch): the range's
@@ -857,7 +848,9 @@ __global__ __launch_bounds__(NT) void k_large_scatter(Bufs b, Sched sc, int leve
     __shared__ uint32_t wsum[RADIX / WAVE];
     __shared__ unsigned long long red_or[NW];
     __shared__ unsigned long long red_and[NW];
+    __shared__ int csel[MSD_MAX_CHUNKS];  // per chunk of the range: -1 or its one digit (below)
     __shared__ BlockPush bp;
+    static_assert(MSD_MAX_CHUNKS <= NT, "one chunk record per thread");
     const int t = threadIdx.x;
     const int d = t & (RADIX - 1);
     const int grp = t / RADIX;
@@ -872,12 +865,15 @@ __global__ __launch_bounds__(NT) void k_large_scatter(Bufs b, Sched sc, int leve
         const int64_t start = (int64_t)it.begin + (int64_t)c * it.chunk_len;
         const U* ks = (const U*)b.k[it.src];
         const V* vs = (const V*)b.v[it.src];
-        // the range's OR / AND over its chunks
-        unsigned long long o = 0, a = ~0ull;
-        for (int x = t; x < it.nch; x += NT) {
-            const ChunkInfo ci = sc.cinfo[level][it.chunk_base + x];
-            o |= ci.kor;
-            a &= ci.kand;
+        // the range's OR / AND over its chunks (thread x holds chunk x's record)
+        unsigned long long o = 0, a = ~0ull, my_and = 0;
+        int my_lp = 0;
+        if (t < it.nch) {
+            const ChunkInfo ci = sc.cinfo[level][it.chunk_base + t];
+            o = ci.kor;
+            a = ci.kand;
+            my_and = ci.kand;
+            my_lp = ci.lp;
         }
         o = wave_or(o);
         a = wave_and(a);
@@ -908,36 +904,42 @@ __global__ __launch_bounds__(NT) void k_large_scatter(Bufs b, Sched sc, int leve
         const int bits = (it.hi_bit - shift) < RADIX_BITS ? (it.hi_bit - shift) : RADIX_BITS;
         const unsigned long long dmask = (1ull << bits) - 1;
         const bool final_ = (var & ((1ull << shift) - 1)) == 0;
+        // each chunk's digit selector: -1 when its histogram counts this level's
+        // digit position, else the one digit value all of its items hold there
+        if (t < it.nch) csel[t] = my_lp == pstar ? -1 : (int)((my_and >> shift) & dmask);
+        __syncthreads();
         // digit d's total over the range and over the chunks before this one:
-        // the group's share of the chunk records is read XB chunks at a time
-        // (all loads of a batch in flight together; a range of 256 chunks costs
-        // its groups 8 round trips instead of 64), the records through the
-        // scalar unit (the share is wave-uniform)
+        // the group's share of the histograms is read XB chunks at a time, all
+        // loads of a batch in flight together (a share of 64 chunks takes two
+        // round trips with XB = 32; a short share takes the 8-chunk batches)
         uint32_t tot = 0, pre = 0;
         {
-            constexpr int XB = 8;
             const int x0 = __builtin_amdgcn_readfirstlane((it.nch * grp) / G);
             const int x1 = __builtin_amdgcn_readfirstlane((it.nch * (grp + 1)) / G);
-            for (int xb = x0; xb < x1; xb += XB) {
-                uint32_t h[XB];
-                ChunkInfo ci[XB];
+            auto totals = [&](auto xbc) {
+                constexpr int XB = decltype(xbc)::value;
+                for (int xb = x0; xb < x1; xb += XB) {
+                    uint32_t h[XB];
 #pragma unroll
-                for (int q = 0; q < XB; ++q) {
-                    const size_t gc = (size_t)it.chunk_base + (size_t)(xb + q < x1 ? xb + q : x1 - 1);
-                    h[q] = sc.hist[level][gc * RADIX + d];
-                    ci[q] = sc.cinfo[level][gc];
-                }
+                    for (int q = 0; q < XB; ++q) {
+                        const size_t gc = (size_t)it.chunk_base + (size_t)(xb + q < x1 ? xb + q : x1 - 1);
+                        h[q] = sc.hist[level][gc * RADIX + d];  // read even when unused: one batch in flight
+                    }
 #pragma unroll
-                for (int q = 0; q < XB; ++q) {
-                    const int x = xb + q;
-                    if (x < x1) {
-                        const uint32_t k = chunk_count_of(ci[q], h[q], pstar, shift, dmask, (uint32_t)d,
-                                                          chunk_items(it, x));
-                        tot += k;
-                        pre += x < c ? k : 0u;
+                    for (int q = 0; q < XB; ++q) {
+                        const int x = xb + q;
+                        if (x < x1) {
+                            const int sel = csel[x];
+                            const uint32_t k =
+                                sel < 0 ? h[q] : (sel == d ? (uint32_t)chunk_items(it, x) : 0u);
+                            tot += k;
+                            pre += x < c ? k : 0u;
+                        }
                     }
                 }
-            }
+            };
+            if (x1 - x0 > 8) totals(std::integral_constant<int, 32>{});
+            else totals(std::integral_constant<int, 8>{});
         }
         uint32_t* part = sm.wcnt;  // free until the partition's first ranking pass
         part[grp * RADIX + d] = tot;

@@ -995,13 +995,17 @@ __global__ __launch_bounds__(NT) void k_large_local(Bufs b, Sched sc, int level,
                                                     int tile_max) {
     using T = RadixTile<NT, IPT, U, V, HAS_V>;
     constexpr int NPOS = int(sizeof(U));  // 8-bit digit positions of a key
+    constexpr int NW = NT / WAVE;
     __shared__ union {
         typename T::Smem sm;
-        uint32_t hist[NPOS * RADIX];
+        struct {  // the histogram phase, before the tile's LDS is used
+            uint32_t hist[RADIX];
+            U red_or[NW];
+            U red_and[NW];
+        };
     } u;
     __shared__ uint32_t cstart[NPOS][RADIX + 1];  // child starts (range-relative) at each depth
     __shared__ uint32_t run[RADIX];               // running destinations of a scatter
-    __shared__ uint32_t nz[NPOS];
     __shared__ uint32_t wsum[RADIX / WAVE];
     __shared__ int fb[NPOS + 1], fn[NPOS + 1], fsrc[NPOS + 1], fhi[NPOS + 1], fdst[NPOS + 1], fshift[NPOS + 1],
         fnext[NPOS + 1];
@@ -1033,24 +1037,45 @@ __global__ __launch_bounds__(NT) void k_large_local(Bufs b, Sched sc, int level,
                 const int npos = npos_of(hi, km.begin_bit);
                 const U* ks = (const U*)b.k[src];
                 const V* vs = (const V*)b.v[src];
-                for (int x = t; x < NPOS * RADIX; x += NT) u.hist[x] = 0;
-                if (t < NPOS) nz[t] = 0;
+                // the histogram of the range's top digit position, with the OR and
+                // AND of its order keys: their highest differing bit names the
+                // position to partition on, which is the top one unless the keys
+                // share their top digit (then that position is counted again)
+                const U rmask = range_mask<U>(km.begin_bit, hi);
+                auto digit_mask = [&](int sh) {
+                    return U((U(1) << ((hi - sh) < RADIX_BITS ? (hi - sh) : RADIX_BITS)) - 1);
+                };
+                const int top = npos - 1;
+                for (int x = t; x < RADIX; x += NT) u.hist[x] = 0;
                 __syncthreads();
-                T::histogram(u.hist, ks, beg, n, km, km.begin_bit, npos, hi);
-                __syncthreads();
-                if (t < RADIX) {
-                    for (int p = 0; p < npos; ++p) {
-                        const uint64_t m = __ballot(u.hist[p * RADIX + t] > 0);
-                        if (lane_id() == 0 && m) atomicAdd(&nz[p], (uint32_t)__popcll(m));
-                    }
+                U o = 0, a = Bits<U>::ALL;
+                {
+                    const int sh = km.begin_bit + top * RADIX_BITS;
+                    T::histogram_or_and(u.hist, ks, beg, n, km, rmask, sh, digit_mask(sh), o, a);
+                }
+                o = wave_or(o);
+                a = wave_and(a);
+                if ((t & 63) == 0) {
+                    u.red_or[t >> 6] = o;
+                    u.red_and[t >> 6] = a;
                 }
                 __syncthreads();
-                int pstar = -1;
-                for (int p = npos - 1; p >= 0; --p)
-                    if (nz[p] > 1) {
-                        pstar = p;
-                        break;
-                    }
+                o = 0;
+                a = Bits<U>::ALL;
+                for (int w = 0; w < NW; ++w) {
+                    o |= u.red_or[w];
+                    a &= u.red_and[w];
+                }
+                const U var = U(o ^ a);
+                const int pstar = var ? (bit_length<U>(var) - 1 - km.begin_bit) / RADIX_BITS : -1;
+                if (pstar >= 0 && pstar != top) {
+                    for (int x = t; x < RADIX; x += NT) u.hist[x] = 0;
+                    __syncthreads();
+                    U o2 = 0, a2 = Bits<U>::ALL;
+                    const int sh = km.begin_bit + pstar * RADIX_BITS;
+                    T::histogram_or_and(u.hist, ks, beg, n, km, rmask, sh, digit_mask(sh), o2, a2);
+                    __syncthreads();
+                }
                 if (pstar < 0) {
                     U* ko = buf_ptr<U>(b.k[BUF_OUT]);
                     V* vo = buf_ptr<V>(b.v[BUF_OUT]);
@@ -1063,15 +1088,13 @@ __global__ __launch_bounds__(NT) void k_large_local(Bufs b, Sched sc, int level,
                     fresh = false;
                     continue;
                 }
-                bool final_ = true;
-                for (int p = 0; p < pstar; ++p)
-                    if (nz[p] > 1) final_ = false;
                 const int shift = km.begin_bit + pstar * RADIX_BITS;
+                const bool final_ = (var & U((U(1) << shift) - 1)) == 0;  // no lower bit varies
                 const int bits = (hi - shift) < RADIX_BITS ? (hi - shift) : RADIX_BITS;
                 const int dst = final_ ? BUF_OUT : (src == BUF_TMP_A ? BUF_TMP_B : BUF_TMP_A);
                 uint32_t mine = 0, incl = 0;
                 if (t < RADIX) {
-                    mine = u.hist[pstar * RADIX + t];
+                    mine = u.hist[t];
                     incl = wave_incl_scan(mine);
                     if (lane_id() == 63) wsum[t >> 6] = incl;
                 }

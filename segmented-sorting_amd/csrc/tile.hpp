@@ -1160,6 +1160,44 @@ struct RadixTile {
             }
         }
     }
+
+    // Digit histogram of the n items at ks[begin..) at one position (shift sh,
+    // digit mask dm) of their order keys (twiddled, masked by rmask), counted
+    // into the LDS array hist[RADIX], with this thread's OR and AND of those
+    // order keys accumulated into o and a (the caller reduces them over the
+    // workgroup). One LDS atomic per item, or one per wave when the wave's
+    // digits are all equal.
+    __device__ static void histogram_or_and(uint32_t* hist, const U* ks, int64_t begin, int n, const KeyMode& km,
+                                            U rmask, int sh, U dm, U& o, U& a) {
+        const int t = threadIdx.x;
+        for (int c0 = 0; c0 < n; c0 += TILE) {
+            U key[IPT];
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int i = c0 + j * NT + t;
+                key[j] = i < n ? at32(ks + begin + c0, j * NT + t) : U(0);  // offsets inside one tile
+            }
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const int i = c0 + j * NT + t;
+                if (c0 + j * NT + (t & ~63) >= n) break;  // this and later rows: whole wave past the end
+                const bool v = i < n;
+                const U k = U(twiddle_in<U>(key[j], km.kind, km.desc) & rmask);
+                if (v) {
+                    o |= k;
+                    a &= k;
+                }
+                const uint64_t act = __ballot(v);
+                const uint32_t d = (uint32_t)((k >> sh) & dm);
+                const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);
+                if (__ballot(v && d != d0) == 0) {
+                    if ((t & 63) == 0) atomicAdd(&hist[d0], (uint32_t)__popcll(act));
+                } else if (v) {
+                    atomicAdd(&hist[d], 1u);
+                }
+            }
+        }
+    }
 };
 
 // ---- wave sorts of short ranges ------------------------------------------------

@@ -824,14 +824,15 @@ __global__ __launch_bounds__(NT) void k_large_hist(Bufs b, Sched sc, int level, 
 }
 
 // Pass 2 of a level: the stable partition of every chunk by the range's
-// highest varying digit. Each workgroup derives its chunk's destinations
-// itself from the range's chunk records (no separate scan launch): the range's
-// OR/AND give the digit, and NT/256 groups of 256 threads (one per digit) total
-// the digit over a share of the range's chunks each, separately for the chunks
-// before this one; the digit bases are the exclusive scan of the totals. The
-// chunk is then partitioned tile by tile into the child buffer (or straight
-// into the output when no lower digit varies), with running destinations in
-// LDS. The range's first chunk queues the children: to the tile classes, to
+// highest varying digit. A workgroup takes a contiguous span of chunks and
+// derives the destinations of the span's part in each range itself from the
+// range's chunk records (no separate scan launch): the range's OR/AND give the
+// digit, and NT/256 groups of 256 threads (one per digit) total the digit over
+// a share of the range's chunks each, separately for the chunks before the
+// part; the digit bases are the exclusive scan of the totals. The part is then
+// partitioned tile by tile into the child buffer (or straight into the output
+// when no lower digit varies), with running destinations in LDS. The workgroup
+// holding the range's first chunk queues the children: to the tile classes, to
 // the next multi-workgroup level (children of level 0 with more than `big`
 // items), or to the depth-first tier. A range whose keys are all equal is
 // copied to the output.
@@ -858,10 +859,20 @@ __global__ __launch_bounds__(NT) void k_large_scatter(Bufs b, Sched sc, int leve
     level_counts(sc, level, count, chunks);
     const LargeItem* items = sc.lq[level];
     const int child_buf = 1 + (level & 1);  // BUF_TMP_A below level 0, BUF_TMP_B below level 1
-    for (int g = blockIdx.x; g < chunks; g += gridDim.x) {
+    // Each workgroup partitions a contiguous span of the level's chunks. The part
+    // of the span inside one range is partitioned as one region (the running
+    // digit destinations carry from tile to tile), so the range lookup, the chunk
+    // records and the digit totals are read once per range part, not per chunk.
+    const int per = (chunks + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int g_stop = min(((int)blockIdx.x + 1) * per, chunks);
+    int g_next = 0;
+    for (int g = (int)blockIdx.x * per; g < g_stop; g = g_next) {
         const LargeItem it = chunk_owner(items, count, g);
-        const int c = g - it.chunk_base;
-        const int len = chunk_items(it, c);
+        const int c = g - it.chunk_base;  // chunks [c, c_stop) of this range
+        const int c_stop = min(g_stop - it.chunk_base, it.nch);
+        g_next = max(g + 1, it.chunk_base + c_stop);
+        const int len = (c_stop == it.nch ? it.n : c_stop * it.chunk_len) - c * it.chunk_len;
+        if (len <= 0) continue;  // an emptied record of a level past its chunk capacity
         const int64_t start = (int64_t)it.begin + (int64_t)c * it.chunk_len;
         const U* ks = (const U*)b.k[it.src];
         const V* vs = (const V*)b.v[it.src];
@@ -1014,6 +1025,9 @@ __global__ __launch_bounds__(NT) void k_large_local(Bufs b, Sched sc, int level,
     const int t = threadIdx.x;
     int count, chunks;
     level_counts(sc, level, count, chunks);
+    // workgroups beyond the range count (all of them when the tier has no work)
+    // leave without a claim: workgroup 0 keeps claiming until the ranges run out
+    if ((int)blockIdx.x >= count) return;
     const LargeItem* items = sc.lq[level];
     for (;;) {
         if (t == 0) sel = (int)atomicAdd(&sc.ctr->claim, 1u);

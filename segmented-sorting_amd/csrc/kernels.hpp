@@ -722,8 +722,14 @@ __global__ __launch_bounds__(NT) void k_large_hist(Bufs b, Sched sc, int level, 
     int count, chunks;
     level_counts(sc, level, count, chunks);
     const LargeItem* items = sc.lq[level];
-    for (int g = blockIdx.x; g < chunks; g += gridDim.x) {
-        const LargeItem it = chunk_owner(items, count, g);
+    // a contiguous span of chunks per workgroup: consecutive chunks of one range
+    // share its record, which is looked up once
+    const int per = (chunks + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int g_stop = min(((int)blockIdx.x + 1) * per, chunks);
+    LargeItem it;
+    it.chunk_base = it.nch = 0;
+    for (int g = (int)blockIdx.x * per; g < g_stop; ++g) {
+        if (g >= it.chunk_base + it.nch) it = chunk_owner(items, count, g);
         const int c = g - it.chunk_base;
         const int len = chunk_items(it, c);
         const U rmask = range_mask<U>(km.begin_bit, it.hi_bit);

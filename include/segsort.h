@@ -218,7 +218,9 @@ int segsort_perm_scatter_f32(const float* d_grad_out, const int* d_perm,
  * filled by the binning pass); it pays off for mixed lengths and costs a few
  * percent for equal ones. Bit 9 (0x200): the 4096-item tile class without its
  * half-depth build for ranges of at most 2048 items (faster when that class
- * holds full-size ranges). Default 0xEF.
+ * holds full-size ranges). Bit 10 (0x400): with wave_cap above 256, ranges of
+ * at most 256 items and the longer wave-tier ranges are two classes with
+ * builds and launches of their own. Default 0xEF.
  * grid_mult: persistent workgroups per CU per resident slot (1..8).
  * concurrent: 1 sorts the ranges that fit a class on a second stream while
  * the multi-workgroup tier partitions the longer segments on the caller's

@@ -8,9 +8,11 @@ segsort_tile_caps), and the multi-workgroup MSD tier above the largest enabled
 class; `grid_mult` sizes the persistent grids.
 `concurrent` (the binned ranges sorted on a second stream while the MSD tier
 runs), the larger grid_mult values, `tile_enable` bit 8 (the 16-item tile
-classes take their ranges longest first, from length buckets) and bit 9 (the
-4096-item tile class without its half-depth build) are tried on the finalists of
-the screening stage.
+classes take their ranges longest first, from length buckets), bit 9 (the
+4096-item tile class without its half-depth build) and bit 10 (a wave cap above
+256 split into a class of ranges of at most 256 items and one of longer ranges,
+each with its own build and launch) are tried on the finalists of the screening
+stage.
 
 `tune()` times every candidate on a resident workload and keeps the fastest one
 whose output is bit-identical to the default tuning's. Results are stored in
@@ -61,6 +63,7 @@ TILE_MASKS = (0xF, 0xB, 0x7, 0xD, 0xE, 0x3)
 GROUP_MASKS = (0x00, 0xE0, 0xA0, 0x60, 0xF0)
 LPT = 0x100  # bit 8: the 16-item tile classes longest first
 NO_HALF = 0x200  # bit 9: the 4096-item tile class without its half-depth build
+SPLIT_WAVE = 0x400  # bit 10: wave caps above 256 split into a <= 256 class and a longer one
 
 
 # persistent grid sizes tried on the finalists (workgroups per resident slot): more than one
@@ -264,9 +267,10 @@ def main(argv=None):
                 shapes.append((t["wave_cap"], t["tile_enable"]))
             if len(shapes) == args.finalists:
                 break
-        finalists = [{"wave_cap": wc, "tile_enable": te | lp | nh, "grid_mult": gm, "concurrent": k}
+        finalists = [{"wave_cap": wc, "tile_enable": te | lp | nh | sw, "grid_mult": gm, "concurrent": k}
                      for wc, te in shapes for gm in FINAL_GRID_MULTS for k in (0, 1) for lp in (0, LPT)
-                     for nh in ((0, NO_HALF) if te & 0x2 else (0,))]
+                     for nh in ((0, NO_HALF) if te & 0x2 else (0,))
+                     for sw in ((0, SPLIT_WAVE) if wc > 256 else (0,))]
         refined = []
         for wl, dev, _, mx in runs:
             _, res = tune(pkg, torch, *dev, steps=args.steps * 6, verbose=False, cands=finalists, max_len=mx)

@@ -112,6 +112,15 @@ constexpr int TUNE_LPT = 0x100;
 // without it, ragged 2^24 0.1953 -> 0.2029 ms (its 1025..2048-item ranges want it).
 constexpr int TUNE_NO_HALF = 0x200;
 
+// tile_enable bit 10: with a wave cap above 256, the wave tier is two classes:
+// ranges of at most 256 items in the 256-item build (8 waves per SIMD, no
+// spills) and the longer ones in the 512-item build, in separate launches,
+// instead of every range in the 512-item build (7 waves per SIMD, about 23
+// VGPRs spilled per lane). For the ranges of 256 +- 16 items that an 8-bit
+// partition of 2^16-item ranges leaves.
+constexpr int TUNE_SPLIT_WAVE = 0x400;
+constexpr int TUNE_BITS = 0xFF | TUNE_LPT | TUNE_NO_HALF | TUNE_SPLIT_WAVE;
+
 constexpr Tuning DEFAULT_TUNING = {256, 0xEF, 1, 0};
 
 // Overrides installed by segsort_set_tuning, per (key width, value width) slot:
@@ -469,12 +478,19 @@ Plan make_plan(const Tuning& tu) {
         last = 4 << g;
     }
     const int wave_cap = std::min(512, std::max(2, tu.wave_cap));
+    const int enable = tu.tile_enable & 0xF;
     if (wave_cap > last) {
+        int tiles = 0;  // tile classes above the wave cap
+        for (int i = 0; i < 4; ++i) tiles += (enable & (1 << i)) && C::CAP[i] > wave_cap;
+        if ((tu.tile_enable & TUNE_SPLIT_WAVE) && wave_cap > 256 && last < 256 &&
+            p.nclass + 2 + tiles <= MAX_TILE_CLASSES) {
+            p.cc.cap[p.nclass] = 256;
+            p.kind[p.nclass++] = KIND_WAVE;
+        }
         p.cc.cap[p.nclass] = wave_cap;
         p.kind[p.nclass++] = KIND_WAVE;
         last = wave_cap;
     }
-    const int enable = tu.tile_enable & 0xF;
     for (int i = 0; i < 4; ++i) {
         if (!(enable & (1 << i))) continue;
         if (C::CAP[i] <= last) continue;
@@ -1332,7 +1348,7 @@ int segsort_perm_scatter_f32(const float* d_grad_out, const int* d_perm, const i
 
 int segsort_set_tuning(int key_width, int value_bytes, int wave_cap, int tile_enable, int grid_mult, int concurrent) {
     const int s = tuning_slot(key_width, value_bytes);
-    if (s < 0 || wave_cap < 2 || wave_cap > 512 || tile_enable < 0 || tile_enable > (0xFF | TUNE_LPT | TUNE_NO_HALF) || grid_mult < 1 ||
+    if (s < 0 || wave_cap < 2 || wave_cap > 512 || tile_enable < 0 || (tile_enable & ~TUNE_BITS) != 0 || grid_mult < 1 ||
         grid_mult > 8 || concurrent < 0 || concurrent > 1)
         return fail(hipErrorInvalidValue, "segsort_set_tuning: out of range");
     std::lock_guard<std::mutex> lk(g_tuning_mu);

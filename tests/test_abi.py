@@ -85,7 +85,9 @@ def test_tuning_table(pkg):
     with pytest.raises(pkg.SegsortError):
         pkg.set_tuning(4, 4, 32, 0xF, 1, 2)
     with pytest.raises(pkg.SegsortError):
-        pkg.set_tuning(4, 4, 32, 0x400, 1)  # bit 10 and above are not defined
+        pkg.set_tuning(4, 4, 32, 0x800, 1)  # bit 11 and above are not defined
+    pkg.set_tuning(4, 4, 512, 0x40F, 1)  # the wave tier split at 256 items
+    assert pkg.get_tuning(4, 4)["tile_enable"] == 0x40F
     pkg.set_tuning(4, 4, 64, 0x3FF, 1)  # ... and the 4096-item tile without its half-depth build
     assert pkg.get_tuning(4, 4)["tile_enable"] == 0x3FF
     pkg.set_tuning(4, 4, 64, 0x1FF, 1)  # every wave-group and tile class, longest first

@@ -544,6 +544,9 @@ TUNINGS = [
     {"wave_cap": 128, "tile_enable": 0x10B, "grid_mult": 3},  # the same with 256x4, 256x16 and 1024x16 only
     {"wave_cap": 128, "tile_enable": 0x207, "grid_mult": 2},  # 256x16 without its half-depth build
     {"wave_cap": 256, "tile_enable": 0x30F, "grid_mult": 1},  # the same, longest first
+    {"wave_cap": 512, "tile_enable": 0x40F, "grid_mult": 1},  # wave tier split: <= 256 and 257..512 items
+    {"wave_cap": 512, "tile_enable": 0x583, "grid_mult": 2},  # the same with groups of 32, longest first
+    {"wave_cap": 512, "tile_enable": 0x4FF, "grid_mult": 1},  # every class: too many to split, one wave class
 ]
 
 

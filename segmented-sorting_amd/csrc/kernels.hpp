@@ -332,13 +332,12 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
         }
     }
     if (LPT && threadIdx.x < NB) lbcount[threadIdx.x] = 0;
-    if (blockIdx.x == 0) clear_counters_and_open(sc.ctr, token);  // includes a barrier
-    else __syncthreads();
     int beg[BIN_SPT], n[BIN_SPT], cls[BIN_SPT];
     uint32_t slot[BIN_SPT];
     int bkt[BIN_SPT];  // LPT: length bucket of a bucketed class's range, else -1
     const int s0 = blockIdx.x * (BIN_NT * spt) + threadIdx.x;
-    // every offset load issued before any is used
+    // every offset load issued before any is used (in workgroup 0 also before the
+    // counters are cleared, so that the clearing's wait covers them too)
 #pragma unroll
     for (int j = 0; j < BIN_SPT; ++j) {
         const int s = s0 + j * BIN_NT;
@@ -349,12 +348,18 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
             n[j] = d_end[s];
         }
     }
+    if (blockIdx.x == 0) clear_counters_and_open(sc.ctr, token);  // includes a barrier
+    else __syncthreads();
+    // length-1 segments are copied: their loads are issued here and their stores
+    // come last, so that no wave waits for the loads in between
+    U one_k[BIN_SPT] = {};
+    V one_v[BIN_SPT] = {};
 #pragma unroll
     for (int j = 0; j < BIN_SPT; ++j) {
         n[j] = n[j] > beg[j] ? n[j] - beg[j] : 0;
         if (n[j] == 1) {
-            buf_ptr<U>(b.k[BUF_OUT])[beg[j]] = ((const U*)b.k[BUF_IN])[beg[j]];
-            if constexpr (HAS_V) buf_ptr<V>(b.v[BUF_OUT])[beg[j]] = ((const V*)b.v[BUF_IN])[beg[j]];
+            one_k[j] = ((const U*)b.k[BUF_IN])[beg[j]];
+            if constexpr (HAS_V) one_v[j] = ((const V*)b.v[BUF_IN])[beg[j]];
         }
         cls[j] = n[j] >= 2 ? class_of(cc, n[j]) : -2;
         bkt[j] = -1;
@@ -429,6 +434,13 @@ __global__ __launch_bounds__(BIN_NT) void k_bin(const int* __restrict__ d_begin,
             const uint32_t g = lbbase[cls[j] * LPT_BUCKETS + bkt[j]] + slot[j];
             const uint32_t cap = lbcap[cls[j]];
             if (g < cap) lbq[cls[j]][(size_t)bkt[j] * cap + g] = make_ti(beg[j], n[j], BUF_IN);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < BIN_SPT; ++j) {
+        if (n[j] == 1) {
+            buf_ptr<U>(b.k[BUF_OUT])[beg[j]] = one_k[j];
+            if constexpr (HAS_V) buf_ptr<V>(b.v[BUF_OUT])[beg[j]] = one_v[j];
         }
     }
 }

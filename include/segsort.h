@@ -224,8 +224,10 @@ int segsort_perm_scatter_f32(const float* d_grad_out, const int* d_perm,
  * grid_mult: persistent workgroups per CU per resident slot (1..8).
  * concurrent: 1 sorts the ranges that fit a class on a second stream while
  * the multi-workgroup tier partitions the longer segments on the caller's
- * stream (joined back before the call's work ends); 0 keeps every
- * launch on the caller's stream. Under stream capture (the caller's graph or
+ * stream (joined back before the call's work ends); 2 does so only for the
+ * ranges of the 16-item tile classes (256x16, 512x16, 1024x16), the smaller
+ * classes following the multi-workgroup tier on the caller's stream; 0 keeps
+ * every launch on the caller's stream. Under stream capture (the caller's graph or
  * the library's own replay) the fork and join are recorded too, as two
  * parallel branches of the graph.
  * key_width: 4 or 8; value_bytes: 0, 4 or 8. Return a hipError_t value.

@@ -224,7 +224,8 @@ def main(argv=None):
     """Two stages per signature: every candidate timed briefly on every workload of
     the signature, then the tier layouts (wave_cap, tile_enable) of the best few
     re-timed with six times the steps at each grid_mult of FINAL_GRID_MULTS, with
-    `concurrent` off and on, the 16-item tiles in binning order and longest first,
+    `concurrent` off, on, and on for the 16-item tile classes only, the 16-item tiles in
+    binning order and longest first,
     and (when the 4096-item class is on) with and without its half-depth build; the
     stored tuning minimises the sum of times relative to each workload's best."""
     import argparse
@@ -268,7 +269,7 @@ def main(argv=None):
             if len(shapes) == args.finalists:
                 break
         finalists = [{"wave_cap": wc, "tile_enable": te | lp | nh | sw, "grid_mult": gm, "concurrent": k}
-                     for wc, te in shapes for gm in FINAL_GRID_MULTS for k in (0, 1) for lp in (0, LPT)
+                     for wc, te in shapes for gm in FINAL_GRID_MULTS for k in (0, 1, 2) for lp in (0, LPT)
                      for nh in ((0, NO_HALF) if te & 0x2 else (0,))
                      for sw in ((0, SPLIT_WAVE) if wc > 256 else (0,))]
         refined = []

@@ -78,7 +78,7 @@ struct Sched {
     uint32_t tq_cap[MAX_QUEUES];
     uint32_t lq_cap;
     uint32_t chunk_cap;
-    int child_q;  // queue of class c's MSD children: child_q + c (0: the binned ranges' queue)
+    int child_q[MAX_TILE_CLASSES];  // queue of class c's MSD children (c: the binned ranges' queue)
     // Bucketed classes (nullptr: none): class c's binned ranges in length bucket
     // b are at bq[c][b * bcap[c] + i]; its queue tq[c] then holds the MSD
     // children only.
@@ -225,12 +225,12 @@ __device__ __forceinline__ void push_tiles_block(const Sched& sc, BlockPush& bp,
     if (threadIdx.x < MAX_TILE_CLASSES) {
         const int c = threadIdx.x;
         const uint32_t k = bp.cnt[c];
-        bp.base[c] = k ? atomicAdd(&sc.ctr->tile_count[sc.child_q + c], k) : 0u;
-        if (k) atomicAdd(&sc.ctr->tile_items[sc.child_q + c], bp.items[c]);
+        bp.base[c] = k ? atomicAdd(&sc.ctr->tile_count[sc.child_q[c]], k) : 0u;
+        if (k) atomicAdd(&sc.ctr->tile_items[sc.child_q[c]], bp.items[c]);
     }
     __syncthreads();
     if (cls >= 0) {
-        const int q = sc.child_q + cls;
+        const int q = sc.child_q[cls];
         const uint32_t g = bp.base[cls] + slot;
         if (g < sc.tq_cap[q]) sc.tq[q][g] = it;
     }

@@ -79,7 +79,8 @@ struct Tuning {
     int tile_enable;  // bit i < 4: LDS tile class i on; bit 4 + g: wave-group class of 4 << g items on;
                       // TUNE_LPT: the 16-item tile classes take their ranges longest first
     int grid_mult;    // persistent workgroups per resident slot
-    int concurrent;   // binned ranges sorted on a second stream while the MSD tier runs (direct launches only)
+    int concurrent;   // 1: binned ranges sorted on a second stream while the MSD tier runs; 2: only those of the
+                      // 16-item tile classes
 };
 
 // The built-in tuning table (generated from tuning.json by build.py): one entry
@@ -657,7 +658,7 @@ struct Launcher {
     }
 
     static void launch_classes(const Plan& p, Bufs b, Sched sc, int q0, KeyMode km, hipStream_t st, int grid_mult,
-                               long long n_items) {
+                               long long n_items, uint32_t only = ~0u) {  // only: bit c launches class c
         int w = 0;  // the wave class, after the group classes it takes in
         if (sizeof(U) == 4)
             while (w < p.nclass && p.kind[w] >= KIND_GROUP) ++w;
@@ -667,6 +668,7 @@ struct Launcher {
         const bool merged = w + 1 < p.nclass && p.kind[w] == KIND_WAVE && p.cc.cap[w] <= 256 &&
                             p.kind[w + 1] == KIND_TILE + 0;
         for (int c = 0; c < p.nclass; ++c) {
+            if (!(only >> c & 1)) continue;
             if (merged && c >= g0 && c < w) continue;  // sorted by the shared launch below
             if (merged && c == w) {
                 GroupQueues gq = {{-1, -1, -1, -1}};
@@ -705,10 +707,11 @@ struct Launcher {
         Profiler::get().attribute(from, h.tile_items, MAX_QUEUES, h.level_items, MAX_LEVELS + 1);
     }
 
-    // split: the MSD tier queues its children behind the binned ranges of each
-    // class (from entry S on) as separate queues, so that the binned ranges can
-    // be sorted while the MSD tier runs.
-    static Sched make_sched(char* base, const Layout& L, const Plan& p, bool plain, bool split = false,
+    // split (bit c: class c): the MSD tier queues class c's children behind its
+    // binned ranges (from entry S on) as a separate queue, so that the binned
+    // ranges can be sorted while the MSD tier runs; the other classes' children
+    // join their binned ranges' queue.
+    static Sched make_sched(char* base, const Layout& L, const Plan& p, bool plain, uint32_t split = 0,
                             size_t S = 0) {
         Sched sc;
         memset(&sc, 0, sizeof(sc));
@@ -729,9 +732,10 @@ struct Launcher {
             sc.bq[c] = L.bcap[c] ? (TileItem*)(base + L.bq[c]) : nullptr;
             sc.bcap[c] = (uint32_t)L.bcap[c];
         }
-        if (split) {
-            sc.child_q = MAX_TILE_CLASSES;
-            for (int c = 0; c < p.nclass; ++c) {
+        for (int c = 0; c < MAX_TILE_CLASSES; ++c) sc.child_q[c] = c;
+        for (int c = 0; c < p.nclass; ++c) {
+            if (split >> c & 1) {
+                sc.child_q[c] = MAX_TILE_CLASSES + c;
                 const size_t own = std::min(S, L.tq_cap[c]);
                 sc.tq[MAX_TILE_CLASSES + c] = sc.tq[c] + own;
                 sc.tq_cap[MAX_TILE_CLASSES + c] = (uint32_t)std::min<size_t>(L.tq_cap[c] - own, 0xFFFFFFFFu);
@@ -792,16 +796,25 @@ struct Launcher {
         // apart, so the tuning table chooses per signature.
         // Under stream capture the fork and join are recorded as well: the second
         // stream joins the capture and the graph gets parallel branches.
+        // `concurrent` 2 forks only the 16-item tile classes: their binned ranges
+        // are sorted on the second stream, and the smaller classes (binned ranges
+        // and MSD children in one queue each) on the caller's stream after the
+        // MSD tier, so that an empty MSD tier does not wait behind them.
+        uint32_t fork_mask = 0;
+        if (levels > 0 && tu.concurrent == 1) fork_mask = (1u << p.nclass) - 1u;
+        if (levels > 0 && tu.concurrent == 2)
+            for (int c = 0; c < p.nclass; ++c)
+                if (p.kind[c] >= KIND_TILE + 1 && p.kind[c] <= KIND_TILE + 3) fork_mask |= 1u << c;
         SideStreams* ss = nullptr;
         bool capturing = false;
-        if (levels > 0 && tu.concurrent) {
+        if (fork_mask) {
             hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
             if (hipStreamIsCapturing(st, &cs) == hipSuccess) {
                 capturing = cs != hipStreamCaptureStatusNone;
                 ss = side_streams(st, capturing);  // nullptr: one stream it is
             }
         }
-        const bool split = ss != nullptr;
+        const uint32_t split = ss != nullptr ? fork_mask : 0u;
         const Sched sc = make_sched(base, L, p, false, split, (size_t)S);
         const Bufs b = make_bufs(base, L, ki, ko, vi, vo);
         const size_t mark = Profiler::get().mark();
@@ -823,7 +836,7 @@ struct Launcher {
             if ((e = hipGetLastError()) != hipSuccess) return e;
             if (capturing && (e = hipEventRecord(ss->fork, st)) != hipSuccess) return e;
             if ((e = hipStreamWaitEvent(ss->s, ss->fork, 0)) != hipSuccess) return e;
-            launch_classes(p, b, sc, 0, km, ss->s, tu.grid_mult, N);
+            launch_classes(p, b, sc, 0, km, ss->s, tu.grid_mult, N, split);
             if ((e = hipEventRecord(ss->join, ss->s)) != hipSuccess) return e;
         }
         if (levels > 0) {
@@ -861,7 +874,14 @@ struct Launcher {
             SEGSORT_LAUNCH("large_local", st, 100 + MSD_LOCAL, PAIR_BYTES + sizeof(U), 0.0,
                            (kl<<<gl, C::NT3, 0, st>>>(b, sc, MSD_LOCAL, km, p.cc, p.tile_max)));
         }
-        launch_classes(p, b, sc, sc.child_q, km, st, tu.grid_mult, N);
+        if (split) {
+            // the classes that did not fork (their queues hold the MSD children
+            // too), then the forked classes' MSD children
+            launch_classes(p, b, sc, 0, km, st, tu.grid_mult, N, ~split);
+            launch_classes(p, b, sc, MAX_TILE_CLASSES, km, st, tu.grid_mult, N, split);
+        } else {
+            launch_classes(p, b, sc, 0, km, st, tu.grid_mult, N);
+        }
         if (fallback) {
             SEGSORT_LAUNCH("oversize_lsd", st, 100 + MAX_LEVELS, PAIR_BYTES, 0.0,
                            (k_seg_lsd<C::NT3, C::IPT3, U, V, HAS_V><<<FALLBACK_GRID, C::NT3, 0, st>>>(b, sc, km)));
@@ -1349,7 +1369,7 @@ int segsort_perm_scatter_f32(const float* d_grad_out, const int* d_perm, const i
 int segsort_set_tuning(int key_width, int value_bytes, int wave_cap, int tile_enable, int grid_mult, int concurrent) {
     const int s = tuning_slot(key_width, value_bytes);
     if (s < 0 || wave_cap < 2 || wave_cap > 512 || tile_enable < 0 || (tile_enable & ~TUNE_BITS) != 0 || grid_mult < 1 ||
-        grid_mult > 8 || concurrent < 0 || concurrent > 1)
+        grid_mult > 8 || concurrent < 0 || concurrent > 2)
         return fail(hipErrorInvalidValue, "segsort_set_tuning: out of range");
     std::lock_guard<std::mutex> lk(g_tuning_mu);
     g_override[s] = {wave_cap, tile_enable, grid_mult, concurrent};

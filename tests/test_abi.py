@@ -83,7 +83,7 @@ def test_tuning_table(pkg):
     with pytest.raises(pkg.SegsortError):
         pkg.set_tuning(4, 4, 513, 0xF, 1)
     with pytest.raises(pkg.SegsortError):
-        pkg.set_tuning(4, 4, 32, 0xF, 1, 2)
+        pkg.set_tuning(4, 4, 32, 0xF, 1, 3)
     with pytest.raises(pkg.SegsortError):
         pkg.set_tuning(4, 4, 32, 0x800, 1)  # bit 11 and above are not defined
     pkg.set_tuning(4, 4, 512, 0x40F, 1)  # the wave tier split at 256 items

@@ -30,7 +30,7 @@ def _setup(pkg, torch):
     return keys, vals, off, kout, vout, call
 
 
-@pytest.mark.parametrize("concurrent", [0, 1], ids=["one-stream", "overlap"])
+@pytest.mark.parametrize("concurrent", [0, 1, 2], ids=["one-stream", "overlap", "overlap-large-tiles"])
 def test_caller_captures_and_replays(pkg, torch_cuda, orc, concurrent):
     """With `concurrent` on, the direct call forks a second stream (its result is checked
     too), and so does the captured call: the second stream joins the capture and the
@@ -62,7 +62,7 @@ def test_caller_captures_and_replays(pkg, torch_cuda, orc, concurrent):
         pkg.reset_tuning(4, 4)
 
 
-@pytest.mark.parametrize("concurrent", [0, 1], ids=["one-stream", "overlap"])
+@pytest.mark.parametrize("concurrent", [0, 1, 2], ids=["one-stream", "overlap", "overlap-large-tiles"])
 def test_library_graph_replay_matches_direct(pkg, torch_cuda, orc, concurrent):
     torch = torch_cuda
     keys, vals, off, kout, vout, call = _setup(pkg, torch)
@@ -84,7 +84,7 @@ def test_library_graph_replay_matches_direct(pkg, torch_cuda, orc, concurrent):
         pkg.reset_tuning(4, 4)
 
 
-@pytest.mark.parametrize("concurrent", [0, 1], ids=["one-stream", "overlap"])
+@pytest.mark.parametrize("concurrent", [0, 1, 2], ids=["one-stream", "overlap", "overlap-large-tiles"])
 def test_graph_cache_eviction_after_replays(pkg, torch_cuda, orc, concurrent):
     """The library's graph-replay mode over 24 distinct argument sets (more than its
     16 cached graphs), three rounds with no host synchronisation inside a round: later

@@ -550,7 +550,7 @@ TUNINGS = [
 ]
 
 
-@pytest.mark.parametrize("concurrent", [1, 0], ids=["overlap", "one-stream"])
+@pytest.mark.parametrize("concurrent", [1, 2, 0], ids=["overlap", "overlap-large-tiles", "one-stream"])
 @pytest.mark.parametrize("tuning", TUNINGS, ids=lambda t: f"wc{t['wave_cap']}-te{t['tile_enable']:03x}")
 def test_tuning_variants(pkg, torch_cuda, orc, tuning, concurrent):
     """Every class layout the tuning can select sorts bit-exactly (batched and plain),

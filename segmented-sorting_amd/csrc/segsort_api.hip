@@ -118,7 +118,9 @@ constexpr int TUNE_NO_HALF = 0x200;
 // spills) and the longer ones in the 512-item build, in separate launches,
 // instead of every range in the 512-item build (7 waves per SIMD, about 23
 // VGPRs spilled per lane). For the ranges of 256 +- 16 items that an 8-bit
-// partition of 2^16-item ranges leaves.
+// partition of 2^16-item ranges leaves. Measured: 2^8 equal segments of 2^16
+// 0.2202 -> 0.2177 ms, one 2^24 segment unchanged (the 512-item build's time is
+// in its longer ranges themselves); the table sets it for the former only.
 constexpr int TUNE_SPLIT_WAVE = 0x400;
 constexpr int TUNE_BITS = 0xFF | TUNE_LPT | TUNE_NO_HALF | TUNE_SPLIT_WAVE;
 
@@ -789,17 +791,20 @@ struct Launcher {
         // hundred workgroups at most, latency-bound) partitions the long ones on
         // the caller's stream; the MSD tier's children are then sorted on the
         // caller's stream and the second stream is joined back. Measured on
-        // MI355X (three persistent workgroups per slot): about 10% faster for
-        // ragged lengths with long segments, but up to 20% slower when no
-        // segment is long, because the (then empty) MSD launches wait for CUs
-        // behind the persistent class grids. The host cannot tell the two
-        // apart, so the tuning table chooses per signature.
+        // MI355X back to back (ragged 2^24, two persistent workgroups per slot):
+        // 0.1852 ms on one stream, 0.1821 with `concurrent` 1, 0.1808 with 2; but
+        // 2^16 equal segments of 256 (no segment long) 0.0722, 0.0885 and 0.0834
+        // ms, because the (then empty) MSD launches wait for CUs behind the
+        // persistent class grids, and calls synchronised one at a time are slower
+        // with it. The host cannot tell the cases apart, so the tuning table
+        // chooses per signature (every stored entry keeps one stream).
         // Under stream capture the fork and join are recorded as well: the second
         // stream joins the capture and the graph gets parallel branches.
         // `concurrent` 2 forks only the 16-item tile classes: their binned ranges
         // are sorted on the second stream, and the smaller classes (binned ranges
         // and MSD children in one queue each) on the caller's stream after the
-        // MSD tier, so that an empty MSD tier does not wait behind them.
+        // MSD tier, so that an empty MSD tier waits behind fewer workgroups (the
+        // empty launches of both streams still contend, as measured above).
         uint32_t fork_mask = 0;
         if (levels > 0 && tu.concurrent == 1) fork_mask = (1u << p.nclass) - 1u;
         if (levels > 0 && tu.concurrent == 2)

@@ -412,6 +412,30 @@ def test_second_msd_level_paths(pkg, torch_cuda, orc):
     check(pkg, torch_cuda, orc, k64, vals, off, algos=("batched",))
 
 
+def test_depth_first_digit_positions(pkg, torch_cuda, orc):
+    """Segments of 5 * 2^20 items, so that every child of the first MSD level is larger
+    than a tile and goes to the depth-first tier, whose histogram counts the range's top
+    digit position and takes the keys' OR/AND: the byte below the top varies (counted
+    once), is constant (counted again one position lower), or the two bytes below it are
+    (the lowest position, written straight to the output); children whose keys are all
+    equal (copied); the same with 64-bit keys; ascending and descending, with values."""
+    rng = np.random.default_rng(123)
+    n = 5 << 20
+    top = rng.integers(0, 256, n, dtype=np.uint32) << np.uint32(24)
+    r24 = rng.integers(0, 1 << 24, n, dtype=np.uint32)
+    segs = [top | r24,                                                      # top digit varies
+            top | np.uint32(0x5A << 16) | (r24 & np.uint32(0xFFFF)),        # byte below the top constant
+            top | np.uint32(0x5A5A << 8) | (r24 & np.uint32(0xFF)),         # two bytes below the top constant
+            np.where(top < np.uint32(128 << 24), top | r24, top | np.uint32(0x5A5A5A))]  # half the children all equal
+    keys = np.concatenate(segs).astype(np.uint32)
+    off = np.arange(0, keys.size + 1, n, dtype=np.int64)
+    vals = np.arange(keys.size, dtype=np.uint32)
+    for desc in (False, True):
+        check(pkg, torch_cuda, orc, keys, vals, off, descending=desc, algos=("batched",))
+    k64 = (segs[1].astype(np.uint64) << np.uint64(32)) | rng.integers(0, 1 << 32, n, dtype=np.uint64)
+    check(pkg, torch_cuda, orc, k64, np.arange(n, dtype=np.uint32), np.array([0, n]), algos=("batched",))
+
+
 def _runs_segment(rng, n, run_len, head_uniform):
     """n u32 keys whose top 16 bits come in runs of about run_len items (low 16 bits
     random), optionally preceded by 64 keys with uniformly random top bits."""

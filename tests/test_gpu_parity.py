@@ -436,6 +436,26 @@ def test_depth_first_digit_positions(pkg, torch_cuda, orc):
     check(pkg, torch_cuda, orc, k64, np.arange(n, dtype=np.uint32), np.array([0, n]), algos=("batched",))
 
 
+def test_msd_chunk_spans_cross_ranges(pkg, torch_cuda, orc):
+    """The multi-workgroup MSD passes give each workgroup a contiguous span of the
+    level's chunks (the chunk count over the grid, rounded up). Segments of 2, 4, 9, 2
+    and 5 chunks of 16K items (150 segments, 660 chunks) make spans of three chunks that
+    start and end inside ranges and cover whole short ones: keys and values against the
+    oracle, both directions, keys only, and 64-bit keys."""
+    rng = np.random.default_rng(77)
+    lengths = np.array([16385, 50000, 131172, 20000, 70001] * 30)
+    rng.shuffle(lengths)
+    off = pkg.gen.offsets_from_lengths(lengths)
+    n = int(off[-1])
+    keys = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+    vals = np.arange(n, dtype=np.uint32)
+    for desc in (False, True):
+        check(pkg, torch_cuda, orc, keys, vals, off, descending=desc, algos=("batched",))
+    check(pkg, torch_cuda, orc, keys, None, off, algos=("batched",))
+    k64 = rng.integers(0, 1 << 63, n, dtype=np.uint64) * np.uint64(2) + (keys & np.uint32(1)).astype(np.uint64)
+    check(pkg, torch_cuda, orc, k64, vals, off, algos=("batched",))
+
+
 def _runs_segment(rng, n, run_len, head_uniform):
     """n u32 keys whose top 16 bits come in runs of about run_len items (low 16 bits
     random), optionally preceded by 64 keys with uniformly random top bits."""

@@ -31,6 +31,24 @@ def test_signature_and_lookup(pkg, tmp_path):
     assert json.loads(p.read_text())["k4v0-n20-m10"]["ms"] == 1.5
 
 
+def test_keep_overlap_needs_isolated_calls_too():
+    """A second-stream candidate that wins back to back is stored only if it also beats
+    the best one-stream candidate on single synchronised calls."""
+    import json
+
+    from segsort_amd import autotune as at
+
+    def key(conc, gm):
+        return json.dumps({"wave_cap": 256, "tile_enable": 0x10B, "grid_mult": gm, "concurrent": conc}, sort_keys=True)
+
+    over, one, one_b = key(2, 2), key(0, 2), key(0, 3)
+    score = {over: 1.96, one: 2.01, one_b: 2.03, key(1, 2): None}
+    assert at.keep_overlap(over, score, {over: 0.19, one: 0.20}.get) == over   # faster both ways
+    assert at.keep_overlap(over, score, {over: 0.21, one: 0.20}.get) == one    # slower alone: best one-stream
+    assert at.keep_overlap(one_b, score, lambda c: 1 / 0) == one_b             # one stream already: not timed
+    assert at.keep_overlap(over, {over: 1.0}, lambda c: 1 / 0) == over          # no one-stream candidate
+
+
 def test_apply_sets_library_tuning(pkg):
     table = {"k4v4-n24-m8": {"wave_cap": 128, "tile_enable": 12, "grid_mult": 2, "concurrent": 1}}
     t = pkg.autotune.apply(pkg, 1 << 24, 1 << 16, 4, 4, table)
